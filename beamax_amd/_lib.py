@@ -1,0 +1,153 @@
+"""ctypes binding of ``libbeamax_amd.so`` (the C-ABI in ``include/beamax_amd.h``).
+
+The library is built in-tree by ``beamax_amd/build.py``.  Loading fails loudly
+when it is missing: there is no CPU fallback for any operator.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libbeamax_amd.so"
+
+BH_MAX_DIM = 3
+BH_MAX_LEVELS = 12
+WINDOW = {"rectangular": 1, "rectangular_mirror": 2}
+INPUT_REAL_SPATIAL, INPUT_COMPLEX_SPATIAL, INPUT_COMPLEX_FOURIER = 0, 1, 2
+PRECISION = {"fp64": 0, "fp32_split": 1}
+
+
+class Tiling(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("num_levels", C.c_int32),
+        ("redundancy", C.c_int32),
+        ("windowing", C.c_int32),
+        ("N", C.c_int64 * BH_MAX_DIM),
+        ("aspect", C.c_int32 * BH_MAX_DIM),
+        ("box_length", C.c_int32 * BH_MAX_LEVELS),
+        ("support", (C.c_int32 * BH_MAX_DIM) * BH_MAX_LEVELS),
+        ("boxes_cumsum", C.c_int64 * (BH_MAX_LEVELS + 1)),
+        ("coeffs_cumsum", C.c_int64 * (BH_MAX_LEVELS + 1)),
+        ("centres", C.c_void_p),
+    ]
+
+
+class FwdBeams(C.Structure):
+    _fields_ = [("count", C.c_int64), ("x0", C.c_void_p), ("p0", C.c_void_p),
+                ("omega", C.c_void_p), ("alpha", C.c_void_p), ("a0", C.c_void_p)]
+
+
+class StaticBeams(C.Structure):
+    _fields_ = [("count", C.c_int64), ("x", C.c_void_p), ("p", C.c_void_p),
+                ("M", C.c_void_p), ("A", C.c_void_p), ("omega", C.c_void_p)]
+
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_SZ = C.c_size_t
+
+# name -> (restype, argtypes); mirrors include/beamax_amd.h
+SIGNATURES = {
+    "bh_last_error": (C.c_char_p, []),
+    "bh_version": (C.c_int, []),
+    "bh_fft_workspace_size": (_SZ, [_I64, _I64]),
+    "bh_fft": (C.c_int, [_P, _I64, _I64, _I64, _I64, _I64, _I64, C.c_int, C.c_double, _P, _SZ, _P]),
+    "bh_mswpt_setup": (C.c_int, [C.POINTER(Tiling), _P, _P, _P]),
+    "bh_mswpt_workspace_size": (_SZ, [C.POINTER(Tiling)]),
+    "bh_mswpt_forward": (C.c_int, [C.POINTER(Tiling), _P, _P, _P, C.c_int, C.c_double, C.c_int,
+                                   _P, _P, _SZ, _P]),
+    "bh_mswpt_inverse": (C.c_int, [C.POINTER(Tiling), _P, _P, C.c_int, _P, _P, _SZ, _P]),
+    "bh_topn_workspace_size": (_SZ, [_I64, _I64]),
+    "bh_topn": (C.c_int, [_P, _I64, _I64, _P, _P, _P, _SZ, _P]),
+    "bh_forward_params": (C.c_int, [C.POINTER(Tiling), C.POINTER(C.c_double), _P, _P, _I64,
+                                    FwdBeams, _P]),
+    "bh_forward_sum": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, C.c_double, _P, _I64, _P,
+                                 _I64, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int, _P,
+                                 _P]),
+    "bh_solve_hom_diag": (C.c_int, [_I64, C.c_int, _P, _P, _P, _P, _P, C.c_double, _P, _I64, _P,
+                                    _P, _P, _P, _P]),
+    "bh_adjoint_source_workspace_size": (_SZ, [_I64, _I64]),
+    "bh_adjoint_source": (C.c_int, [_P, _I64, _I64, C.c_double, _P, C.c_int, _P, _P, _P, _SZ, _P]),
+    "bh_fourier_crop_time_workspace_size": (_SZ, [_I64, _I64, _I64]),
+    "bh_fourier_crop_time": (C.c_int, [_P, _I64, _I64, _P, _P, _I64, _I64, _P, _P, _SZ, _P]),
+    "bh_adjoint_params": (C.c_int, [C.POINTER(Tiling), C.POINTER(C.c_double), _P, _P, _I64,
+                                    C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_int, StaticBeams, _P]),
+    "bh_static_sum_workspace_size": (_SZ, [_I64, C.c_int]),
+    "bh_static_sum": (C.c_int, [C.POINTER(StaticBeams), C.c_int, _P, _I64, C.c_double, C.c_int,
+                                _P, _P, _SZ, _P]),
+}
+
+_lib = None
+
+
+class BeamaxHIPError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+
+def load():
+    """Load the HIP library (once). Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m beamax_amd.build` "
+            "(the MSGB operator has no CPU fallback)."
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call C-ABI function `name`; raise BeamaxHIPError on a non-zero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.bh_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        raise BeamaxHIPError(f"{name} failed ({rc}): {msg}")
+    return rc
+
+
+def ptr(t):
+    """Raw device pointer of a torch tensor (None -> NULL)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def stream_handle():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def dbl_array(vals):
+    arr = (C.c_double * max(1, len(vals)))()
+    for i, v in enumerate(vals):
+        arr[i] = float(v)
+    return arr
+
+
+def int32_array(vals):
+    arr = (C.c_int32 * max(1, len(vals)))()
+    for i, v in enumerate(vals):
+        arr[i] = int(v)
+    return arr
+
+
+def workspace(nbytes, device):
+    import torch
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def as_f64_cols(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))

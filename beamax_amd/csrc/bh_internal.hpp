@@ -1,0 +1,49 @@
+// Internal (non-exported) interfaces shared by the library's translation units.
+#pragma once
+
+#include "bh_common.hpp"
+
+namespace bh {
+
+// Batched strided 1-D complex FFT in place (see bh_fft in the public header).
+// ws may be null when n <= kFftLdsMax (LDS-resident path needs no workspace).
+constexpr int64_t kFftLdsMax = 1024;
+size_t fft_workspace_bytes(int64_t n, int64_t batch);
+int fft_strided(double2* data, int64_t n, int64_t batch, int64_t inner, int64_t inner_stride,
+                int64_t outer_stride, int64_t elem_stride, int dir, double scale, void* ws,
+                size_t ws_bytes, hipStream_t s);
+
+// Multi-dimensional unitary FFT of a C-order array of shape N[0..d) in place:
+// dir -1 = fftn(ortho), +1 = ifftn(ortho).  No shifts.
+int fftn_ortho(double2* data, int ndim, const int64_t* N, int dir, void* ws, size_t ws_bytes,
+               hipStream_t s);
+
+inline int64_t prod(const int64_t* v, int n) {
+  int64_t p = 1;
+  for (int i = 0; i < n; ++i) p *= v[i];
+  return p;
+}
+
+inline unsigned grid1d(int64_t work, int block) {
+  int64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 0x7fffffff) g = 0x7fffffff;
+  return (unsigned)g;
+}
+
+// device-side copy of the tiling scalars (everything but the centre pointer)
+struct TilingDev {
+  int32_t ndim, num_levels, redundancy, windowing;
+  int64_t N[BH_MAX_DIM];
+  int32_t aspect[BH_MAX_DIM];
+  int32_t box_length[BH_MAX_LEVELS];
+  int32_t support[BH_MAX_LEVELS][BH_MAX_DIM];
+  int64_t boxes_cumsum[BH_MAX_LEVELS + 1];
+  int64_t coeffs_cumsum[BH_MAX_LEVELS + 1];
+  int64_t gpacked_off[BH_MAX_LEVELS + 1];
+};
+
+int check_tiling(const bh_tiling* t);
+TilingDev to_dev(const bh_tiling* t);
+
+}  // namespace bh

@@ -1,0 +1,352 @@
+// top_n selection: the k largest |c| under the order (|c|, index), ascending.
+//
+// 1. key[i] = bit pattern of |c_i| = sqrt(re*re + im*im) (IEEE, each op
+//    correctly rounded; for non-negative doubles the bit pattern orders like
+//    the value).
+// 2. MSB-first radix select, 8 bits per pass: a grid-wide LDS histogram of the
+//    keys that match the prefix fixed so far, then one small block picks the
+//    digit.  After 8 passes the k-th largest key T is known, together with how
+//    many of the entries equal to T are needed (the largest indices win).
+// 3. Stable compaction of the winners (per-block counts, one scan block,
+//    per-block ordered writes).
+// 4. Bitonic sort of the winners on (key, index): 2048-element LDS tiles, then
+//    global merge passes for the long strides.
+// 5. Gather of indices and values.
+#include "bh_internal.hpp"
+
+namespace {
+
+constexpr int kSelBlock = 256;
+constexpr int kItems = 16;                       // keys per thread per block pass
+constexpr int kChunk = kSelBlock * kItems;       // keys per block
+constexpr int kSortTile = 2048;
+
+struct SelState {
+  unsigned long long prefix;   // fixed high bits of T
+  unsigned long long mask;     // which bits are fixed
+  long long need;              // entries still to take at or below the prefix
+  long long eq_total;          // number of keys == T (after the last pass)
+};
+
+__global__ void k_keys(const double2* __restrict__ c, unsigned long long* __restrict__ key,
+                       long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 v = c[i];
+  const double m = __dsqrt_rn(__dadd_rn(__dmul_rn(v.x, v.x), __dmul_rn(v.y, v.y)));
+  key[i] = (unsigned long long)__double_as_longlong(m);
+}
+
+__global__ void k_init_state(SelState* st, unsigned int* hist, long long k) {
+  if (threadIdx.x == 0) {
+    st->prefix = 0ull;
+    st->mask = 0ull;
+    st->need = k;
+    st->eq_total = 0;
+  }
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0u;
+}
+
+__global__ __launch_bounds__(kSelBlock) void k_hist(const unsigned long long* __restrict__ key,
+                                                    long long n, const SelState* st, int shift,
+                                                    unsigned int* __restrict__ hist) {
+  __shared__ unsigned int h[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  const unsigned long long prefix = st->prefix, mask = st->mask;
+  const long long base = (long long)blockIdx.x * kChunk;
+  for (int it = 0; it < kItems; ++it) {
+    const long long i = base + (long long)it * kSelBlock + threadIdx.x;
+    if (i < n) {
+      const unsigned long long kv = key[i];
+      if ((kv & mask) == prefix) atomicAdd(&h[(kv >> shift) & 0xffu], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 256; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// one block of 256: choose the digit of the k-th largest key, clear the histogram
+__global__ void k_pick(SelState* st, unsigned int* hist, int shift) {
+  __shared__ unsigned int h[256];
+  const int t = threadIdx.x;
+  h[t] = hist[t];
+  __syncthreads();
+  if (t == 0) {
+    long long need = st->need;
+    long long above = 0;
+    int digit = 0;
+    for (int b = 255; b >= 0; --b) {
+      if (above + (long long)h[b] >= need) { digit = b; break; }
+      above += h[b];
+    }
+    st->prefix |= ((unsigned long long)digit) << shift;
+    st->mask |= 0xffull << shift;
+    st->need = need - above;
+    st->eq_total = h[digit];
+  }
+  __syncthreads();
+  hist[t] = 0u;
+}
+
+// per-block counts of keys > T and keys == T
+__global__ __launch_bounds__(kSelBlock) void k_count(const unsigned long long* __restrict__ key,
+                                                     long long n, const SelState* st,
+                                                     long long* __restrict__ cnt) {
+  __shared__ int sgt, seq;
+  if (threadIdx.x == 0) { sgt = 0; seq = 0; }
+  __syncthreads();
+  const unsigned long long T = st->prefix;
+  int gt = 0, eq = 0;
+  const long long base = (long long)blockIdx.x * kChunk;
+  for (int it = 0; it < kItems; ++it) {
+    const long long i = base + (long long)it * kSelBlock + threadIdx.x;
+    if (i < n) {
+      const unsigned long long kv = key[i];
+      gt += kv > T;
+      eq += kv == T;
+    }
+  }
+  atomicAdd(&sgt, gt);
+  atomicAdd(&seq, eq);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cnt[2 * blockIdx.x] = sgt;
+    cnt[2 * blockIdx.x + 1] = seq;
+  }
+}
+
+// single block: exclusive scans of the per-block counts (gt and eq) in place
+__global__ void k_scan_counts(long long* cnt, long long nblk) {
+  if (threadIdx.x != 0) return;
+  long long sg = 0, se = 0;
+  for (long long b = 0; b < nblk; ++b) {
+    const long long g = cnt[2 * b], e = cnt[2 * b + 1];
+    cnt[2 * b] = sg;
+    cnt[2 * b + 1] = se;
+    sg += g;
+    se += e;
+  }
+}
+
+// ordered compaction: winners are keys > T and the last `need` keys == T in
+// index order.  Output slot = (#winners before i).
+__global__ __launch_bounds__(kSelBlock) void k_compact(const unsigned long long* __restrict__ key,
+                                                       long long n, const SelState* st,
+                                                       const long long* __restrict__ cnt,
+                                                       unsigned long long* __restrict__ okey,
+                                                       long long* __restrict__ oidx) {
+  __shared__ int wgt[kSelBlock / 64], weq[kSelBlock / 64];
+  const unsigned long long T = st->prefix;
+  const long long skip_eq = st->eq_total - st->need;   // equal keys not taken (lowest indices)
+  long long gt_base = cnt[2 * blockIdx.x];
+  long long eq_base = cnt[2 * blockIdx.x + 1];
+  const long long base = (long long)blockIdx.x * kChunk;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int it = 0; it < kItems; ++it) {
+    const long long i = base + (long long)it * kSelBlock + threadIdx.x;
+    unsigned long long kv = 0;
+    bool g = false, e = false;
+    if (i < n) {
+      kv = key[i];
+      g = kv > T;
+      e = kv == T;
+    }
+    const unsigned long long bg = __ballot(g), be = __ballot(e);
+    const unsigned long long lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int rg = __popcll(bg & lower), re = __popcll(be & lower);
+    if (lane == 0) { wgt[wave] = __popcll(bg); weq[wave] = __popcll(be); }
+    __syncthreads();
+    int og = 0, oe = 0, tg = 0, te = 0;
+    for (int w = 0; w < kSelBlock / 64; ++w) {
+      if (w < wave) { og += wgt[w]; oe += weq[w]; }
+      tg += wgt[w];
+      te += weq[w];
+    }
+    const long long my_gt = gt_base + og + rg;   // rank among keys > T
+    const long long my_eq = eq_base + oe + re;   // rank among keys == T
+    if (g) {
+      const long long taken_eq_before = my_eq - skip_eq > 0 ? my_eq - skip_eq : 0;
+      const long long slot = my_gt + taken_eq_before;
+      okey[slot] = kv;
+      oidx[slot] = i;
+    } else if (e && my_eq >= skip_eq) {
+      const long long slot = my_gt + (my_eq - skip_eq);
+      okey[slot] = kv;
+      oidx[slot] = i;
+    }
+    gt_base += tg;
+    eq_base += te;
+    __syncthreads();
+  }
+}
+
+__device__ inline bool pair_less(unsigned long long ka, long long ia, unsigned long long kb,
+                                 long long ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+__global__ void k_pad(unsigned long long* key, long long* idx, long long k, long long P) {
+  const long long i = k + blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < P) {
+    key[i] = ~0ull;
+    idx[i] = 0x7fffffffffffffffll;
+  }
+}
+
+// Full bitonic stages k = 2..kmax (kmax <= tile) inside 2048-element LDS tiles,
+// or (merge_only) the j < tile steps of stage `kstage`.
+__global__ __launch_bounds__(256) void k_bitonic_tile(unsigned long long* key, long long* idx,
+                                                      long long P, int merge_only,
+                                                      long long kstage) {
+  __shared__ unsigned long long sk[kSortTile];
+  __shared__ long long si[kSortTile];
+  const long long base = (long long)blockIdx.x * kSortTile;
+  const int tile = (int)((P < kSortTile) ? P : kSortTile);
+  for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+    sk[i] = key[base + i];
+    si[i] = idx[base + i];
+  }
+  __syncthreads();
+  long long k0 = merge_only ? kstage : 2;
+  long long k1 = merge_only ? kstage : tile;
+  for (long long kk = k0; kk <= k1; kk <<= 1) {
+    long long jstart = merge_only ? (tile >> 1) : (kk >> 1);
+    for (long long j = jstart; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < tile / 2; t += blockDim.x) {
+        const int lo = (int)(((t / j) * 2 * j) + (t % j));
+        const int hi = lo + (int)j;
+        const long long gpos = base + lo;
+        const bool up = ((gpos & kk) == 0);
+        const bool sw = up ? pair_less(sk[hi], si[hi], sk[lo], si[lo])
+                           : pair_less(sk[lo], si[lo], sk[hi], si[hi]);
+        if (sw) {
+          unsigned long long tk = sk[lo]; sk[lo] = sk[hi]; sk[hi] = tk;
+          long long ti = si[lo]; si[lo] = si[hi]; si[hi] = ti;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+    key[base + i] = sk[i];
+    idx[base + i] = si[i];
+  }
+}
+
+__global__ void k_bitonic_global(unsigned long long* key, long long* idx, long long P,
+                                 long long kk, long long j) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= P / 2) return;
+  const long long lo = (t / j) * 2 * j + (t % j);
+  const long long hi = lo + j;
+  const bool up = ((lo & kk) == 0);
+  const unsigned long long kl = key[lo], kh = key[hi];
+  const long long il = idx[lo], ih = idx[hi];
+  const bool sw = up ? pair_less(kh, ih, kl, il) : pair_less(kl, il, kh, ih);
+  if (sw) {
+    key[lo] = kh; key[hi] = kl;
+    idx[lo] = ih; idx[hi] = il;
+  }
+}
+
+__global__ void k_gather_out(const double2* __restrict__ c, const long long* __restrict__ sidx,
+                             long long k, long long* __restrict__ idx_out,
+                             double2* __restrict__ val_out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const long long j = sidx[i];
+  idx_out[i] = j;
+  val_out[i] = c[j];
+}
+
+long long pow2_at_least(long long x) {
+  long long p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+struct TopnWs {
+  unsigned long long* key;
+  unsigned long long* okey;
+  long long* oidx;
+  long long* cnt;
+  unsigned int* hist;
+  SelState* st;
+  size_t bytes;
+};
+
+TopnWs carve(void* ws, long long n, long long k) {
+  const long long P = pow2_at_least(k < kSortTile ? kSortTile : k);
+  const long long nblk = (n + kChunk - 1) / kChunk;
+  char* p = (char*)ws;
+  TopnWs w;
+  auto take = [&](size_t bytes) {
+    char* r = p;
+    p += (bytes + 255) / 256 * 256;
+    return (void*)r;
+  };
+  w.key = (unsigned long long*)take(n * 8);
+  w.okey = (unsigned long long*)take(P * 8);
+  w.oidx = (long long*)take(P * 8);
+  w.cnt = (long long*)take(2 * nblk * 8);
+  w.hist = (unsigned int*)take(256 * 4);
+  w.st = (SelState*)take(sizeof(SelState));
+  w.bytes = (size_t)(p - (char*)ws);
+  return w;
+}
+
+}  // namespace
+
+extern "C" size_t bh_topn_workspace_size(int64_t n, int64_t k) {
+  if (n <= 0 || k <= 0) return 0;
+  if (k > n) k = n;
+  TopnWs w = carve(nullptr, n, k);
+  return w.bytes;
+}
+
+extern "C" int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out,
+                       double* val_out, void* ws, size_t ws_bytes, bh_stream s) {
+  bh::clear_error();
+  if (n <= 0) BH_FAIL(BH_ERR_ARG, "bh_topn: empty input (n=%lld)", (long long)n);
+  if (k <= 0) BH_FAIL(BH_ERR_ARG, "top_n threshold must be a positive integer.");
+  if (k > n) k = n;
+  if (!c || !idx_out || !val_out || !ws) BH_FAIL(BH_ERR_ARG, "bh_topn: null pointer");
+  const size_t need = bh_topn_workspace_size(n, k);
+  if (ws_bytes < need) BH_FAIL(BH_ERR_WORKSPACE, "bh_topn: workspace %zu < %zu", ws_bytes, need);
+  hipStream_t st = (hipStream_t)s;
+  TopnWs w = carve(ws, n, k);
+  const long long nblk = (n + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(k_keys, dim3(bh::grid1d(n, 256)), dim3(256), 0, st, (const double2*)c,
+                     w.key, (long long)n);
+  hipLaunchKernelGGL(k_init_state, dim3(1), dim3(256), 0, st, w.st, w.hist, (long long)k);
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_hist, dim3((unsigned)nblk), dim3(kSelBlock), 0, st, w.key,
+                       (long long)n, w.st, shift, w.hist);
+    hipLaunchKernelGGL(k_pick, dim3(1), dim3(256), 0, st, w.st, w.hist, shift);
+  }
+  hipLaunchKernelGGL(k_count, dim3((unsigned)nblk), dim3(kSelBlock), 0, st, w.key, (long long)n,
+                     w.st, w.cnt);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(64), 0, st, w.cnt, nblk);
+  const long long P = pow2_at_least(k < kSortTile ? kSortTile : k);
+  hipLaunchKernelGGL(k_pad, dim3(bh::grid1d(P - k, 256)), dim3(256), 0, st, w.okey, w.oidx,
+                     (long long)k, P);
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)nblk), dim3(kSelBlock), 0, st, w.key,
+                     (long long)n, w.st, w.cnt, w.okey, w.oidx);
+  BH_LAUNCH_CHECK();
+  // bitonic sort of P (power of two >= 2048) pairs, ascending
+  const unsigned tiles = (unsigned)(P / kSortTile);
+  hipLaunchKernelGGL(k_bitonic_tile, dim3(tiles), dim3(256), 0, st, w.okey, w.oidx, P, 0, 0ll);
+  for (long long kk = 2 * kSortTile; kk <= P; kk <<= 1) {
+    for (long long j = kk >> 1; j >= kSortTile; j >>= 1)
+      hipLaunchKernelGGL(k_bitonic_global, dim3(bh::grid1d(P / 2, 256)), dim3(256), 0, st,
+                         w.okey, w.oidx, P, kk, j);
+    hipLaunchKernelGGL(k_bitonic_tile, dim3(tiles), dim3(256), 0, st, w.okey, w.oidx, P, 1, kk);
+  }
+  hipLaunchKernelGGL(k_gather_out, dim3(bh::grid1d(k, 256)), dim3(256), 0, st,
+                     (const double2*)c, w.oidx, (long long)k, (long long*)idx_out,
+                     (double2*)val_out);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}

@@ -1,0 +1,201 @@
+/*
+ * beamax_amd: C-ABI of the MI355X (gfx950) MSGB forward/adjoint operator.
+ *
+ * Every entry point takes plain pointers and sizes; device pointers are HIP
+ * device memory owned by the caller (the Python host layer allocates them as
+ * PyTorch-ROCm tensors).  Work is enqueued on the caller's stream `s`
+ * (hipStream_t; NULL = the legacy default stream); nothing synchronises the
+ * device except where a function says so.  Every function returns 0 on
+ * success or a negative BH_ERR_* code; bh_last_error() returns the message of
+ * the last failure on the calling thread.  Complex arrays are interleaved
+ * (re, im) FP64.  The library allocates no device memory except inside
+ * bh_topn / bh_*_sum / bh_mswpt_* through the workspace the caller passes.
+ *
+ * Reference interfaces replaced (elma16/beamax 0.2.2, file:line):
+ *   bh_mswpt_setup      MSWPT.__init__ sum_gsquare + packed filters
+ *                       (beamax/transforms.py:423-589)
+ *   bh_mswpt_forward    MSWPT.forward (beamax/transforms.py:696-733, 591-694)
+ *                       + compute_coefficients 'pos_only' epilogue
+ *                       (beamax/solvers/msgb_solvers/forward_solver_utils.py:716-786)
+ *   bh_mswpt_inverse    MSWPT.inverse (beamax/transforms.py:735-850)
+ *   bh_fft              utils.unitary_fft/unitary_ifft building block
+ *                       (beamax/utils/fft.py:5-38)
+ *   bh_topn             _threshold_top_n (forward_solver_utils.py:65-89)
+ *   bh_forward_params   compute_forward_parameters (forward_solver_utils.py:255-373)
+ *   bh_forward_sum      compute_forward_result with solve_hom_diag and
+ *                       compute_gaussian_beam_real (forward_solver_utils.py:677-713,
+ *                       beamax/gb/gb_solvers.py:356-422, beamax/gb/core.py:194-329)
+ *   bh_solve_hom_diag   gb_solvers.solve_hom_diag (beamax/gb/gb_solvers.py:356-422)
+ *   bh_adjoint_source   _form_adjoint_source (msgb_solver.py:68-120)
+ *   bh_adjoint_params   compute_adj_parameters + compute_TR_parameters +
+ *                       solve_hom_TR at t = 0 (adjoint_solver_utils.py:75-220,
+ *                       tr_solver_utils.py:205-435, gb_solvers.py:484-594)
+ *   bh_fourier_crop_time  prepare_data_domain_for_msgb Fourier crop
+ *                       (examples/reconstruction/2d_time_reversal_and_adjoint.py:111-165)
+ *   bh_static_sum       compute_TR_result + compute_gaussian_beam_real_TR at
+ *                       t = 0 + c^-2 weight (tr_solver_utils.py:677-750,
+ *                       beamax/gb/core.py:332-425, msgb_solver.py:123-127)
+ */
+#ifndef BEAMAX_AMD_H
+#define BEAMAX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* bh_stream;
+
+#define BH_OK 0
+#define BH_ERR_ARG -1
+#define BH_ERR_HIP -2
+#define BH_ERR_UNSUPPORTED -3
+#define BH_ERR_WORKSPACE -4
+
+#define BH_MAX_DIM 3
+#define BH_MAX_LEVELS 12
+
+#define BH_WINDOW_RECTANGULAR 1
+#define BH_WINDOW_RECTANGULAR_MIRROR 2
+
+#define BH_INPUT_REAL_SPATIAL 0
+#define BH_INPUT_COMPLEX_SPATIAL 1
+#define BH_INPUT_COMPLEX_FOURIER 2
+
+#define BH_PRECISION_FP64 0
+#define BH_PRECISION_FP32_SPLIT 1
+
+/* Dyadic Fourier tiling + MSWPT layout (host struct; `centres` is device). */
+typedef struct {
+  int32_t ndim;                           /* 1..3 */
+  int32_t num_levels;                     /* 1..BH_MAX_LEVELS */
+  int32_t redundancy;                     /* 1 or 2 */
+  int32_t windowing;                      /* BH_WINDOW_* */
+  int64_t N[BH_MAX_DIM];                  /* grid shape (even) */
+  int32_t aspect[BH_MAX_DIM];             /* box aspect ratio per axis */
+  int32_t box_length[BH_MAX_LEVELS];      /* smallest-axis box length per level */
+  int32_t support[BH_MAX_LEVELS][BH_MAX_DIM]; /* coefficient block shape per level */
+  int64_t boxes_cumsum[BH_MAX_LEVELS + 1];
+  int64_t coeffs_cumsum[BH_MAX_LEVELS + 1];
+  const int32_t* centres;                 /* device, total_boxes x ndim, raw centres */
+} bh_tiling;
+
+/* Forward beams (device SoA, FP64). Beam i: x0[i*d..], p0[i*d..], omega[i],
+ * alpha[2*(i*d+a)..] (diagonal of M0), a0[2*i..] (amplitude incl. the
+ * coefficient).  bh_forward_sum mirrors every beam to mode -1 when asked. */
+typedef struct {
+  int64_t count;
+  double* x0;
+  double* p0;
+  double* omega;
+  double* alpha;
+  double* a0;
+} bh_fwd_beams;
+
+/* Beams frozen at one time level (device SoA, FP64): adjoint beams at t = 0.
+ * M is the full d x d complex Hessian, row-major, interleaved. */
+typedef struct {
+  int64_t count;
+  double* x;
+  double* p;
+  double* M;
+  double* A;
+  double* omega;
+} bh_static_beams;
+
+const char* bh_last_error(void);
+int bh_version(void);
+
+/* Batched 1-D complex FFT (FP64) along one strided axis, in place.
+ * Transform t in [0,batch) starts at (t/inner)*outer_stride +
+ * (t%inner)*inner_stride, element e at + e*elem_stride (units: complex
+ * elements).  dir = -1 forward (exp(-2 pi i jk/n)), +1 inverse; result
+ * multiplied by `scale`.  `ws` needs bh_fft_workspace_size bytes. */
+size_t bh_fft_workspace_size(int64_t n, int64_t batch);
+int bh_fft(double* data, int64_t n, int64_t batch, int64_t inner, int64_t inner_stride,
+           int64_t outer_stride, int64_t elem_stride, int dir, double scale, void* ws,
+           size_t ws_bytes, bh_stream s);
+
+/* MSWPT plan tables: sum_gsquare (prod N doubles, centred index order) and
+ * the packed first-box filter of every level (sum_l prod support[l] doubles,
+ * levels concatenated). */
+int bh_mswpt_setup(const bh_tiling* t, double* sum_gsquare, double* gpacked, bh_stream s);
+size_t bh_mswpt_workspace_size(const bh_tiling* t);
+/* coeffs[total] = scale * MSWPT(input); if half_mask != 0 the second half
+ * (flat entries >= len_l/2) of every level is set to zero. */
+int bh_mswpt_forward(const bh_tiling* t, const double* sum_gsquare, const double* gpacked,
+                     const double* input, int input_kind, double scale, int half_mask,
+                     double* coeffs, void* ws, size_t ws_bytes, bh_stream s);
+/* out[prod N] complex = MSWPT^-1(coeffs) in the spatial (output_fourier = 0)
+ * or centred Fourier (1) domain. */
+int bh_mswpt_inverse(const bh_tiling* t, const double* gpacked, const double* coeffs,
+                     int output_fourier, double* out, void* ws, size_t ws_bytes,
+                     bh_stream s);
+
+/* top_n: the k largest |c| under the order (|c|, index), returned ascending.
+ * |c| = sqrt(re*re + im*im), each operation correctly rounded. */
+size_t bh_topn_workspace_size(int64_t n, int64_t k);
+int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out, double* val_out,
+            void* ws, size_t ws_bytes, bh_stream s);
+
+/* Forward beam parameters for flat coefficient indices idx[k] with values
+ * vals[k] (complex); grid_size = N*dx (host, ndim). */
+int bh_forward_params(const bh_tiling* t, const double* grid_size, const int64_t* idx,
+                      const double* vals, int64_t k, bh_fwd_beams out, bh_stream s);
+
+/* Real beam sum at sensors:  out[t*S + s] = sum_b 2|A_b(t)| cos(...) exp(...).
+ * ts: device, nt values; sensors: device S x d; mirror != 0 adds every beam
+ * again with mode -1 (after the +1 block).  precision BH_PRECISION_FP64 writes
+ * double, BH_PRECISION_FP32_SPLIT writes float. periodic (host, d) applies
+ * the minimum-image offset with period domain_size (host, d). */
+int bh_forward_sum(const bh_fwd_beams* b, int ndim, int mirror, double c, const double* ts,
+                   int64_t nt, const double* sensors, int64_t S, const double* domain_size,
+                   const int32_t* periodic, int precision, void* out, bh_stream s);
+
+/* Closed-form trajectories (SolverFn): x0,p0 (b x d), alpha (b x d complex),
+ * a0 (b complex), mode (b); outputs xt,pt (b,nt,d), Mt (b,nt,d,d complex),
+ * At (b,nt complex). */
+int bh_solve_hom_diag(int64_t b, int ndim, const double* x0, const double* p0,
+                      const double* alpha, const double* a0, const double* mode, double c,
+                      const double* ts, int64_t nt, double* xt, double* pt, double* Mt,
+                      double* At, bh_stream s);
+
+/* Adjoint source -c^2 Re IFFT_t(2 pi i f FFT_t(w r)) along axis 0 of an
+ * (nt, ns) FP64 array; window (nullable) has nt entries (window_2d = 0) or
+ * nt*ns (window_2d = 1); c_src device ns. */
+size_t bh_adjoint_source_workspace_size(int64_t nt, int64_t ns);
+int bh_adjoint_source(const double* data, int64_t nt, int64_t ns, double dt,
+                      const double* window, int window_2d, const double* c_src, double* out,
+                      void* ws, size_t ws_bytes, bh_stream s);
+
+/* Fourier crop along time (the multi-face adjoint glue): columns cols[ns]
+ * (nullable = 0..ns-1) of an (nt, ns_in) real array, times col_weight[ns]
+ * (nullable), are resampled to nt_out samples by keeping the centred nt_out
+ * temporal frequencies of the unitary FFT; out is (nt_out, ns). */
+size_t bh_fourier_crop_time_workspace_size(int64_t nt, int64_t nt_out, int64_t ns);
+int bh_fourier_crop_time(const double* data, int64_t nt, int64_t ns_in, const int64_t* cols,
+                         const double* col_weight, int64_t ns, int64_t nt_out, double* out,
+                         void* ws, size_t ws_bytes, bh_stream s);
+
+/* Adjoint beams: data-domain tiling t, data grid size L (host, d), selected
+ * indices/values (k), detector plane normal axis / value / inward sign (host
+ * geometry), homogeneous speed c, relative guard; beams are propagated to t = 0.
+ * `swap_axes` permutes spatial axes 0 and 1 of the result (face rotation). */
+int bh_adjoint_params(const bh_tiling* t, const double* data_grid_size, const int64_t* idx,
+                      const double* vals, int64_t k, int normal_axis, double normal_value,
+                      double inward_sign, double c, double guard, int swap_axes,
+                      bh_static_beams out, bh_stream s);
+
+/* out[P] = scale * sum_b field_b(points[p]) for beams frozen at one time. */
+size_t bh_static_sum_workspace_size(int64_t count, int ndim);
+int bh_static_sum(const bh_static_beams* b, int ndim, const double* points, int64_t P,
+                  double scale, int precision, void* out, void* ws, size_t ws_bytes,
+                  bh_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEAMAX_AMD_H */

@@ -1,0 +1,190 @@
+"""Oracle: principal-symbol MSGB adjoint (test infrastructure only).
+
+FP64 NumPy restatement of ``msgb_solver.py:68-127, 1168-1386``,
+``tr_solver_utils.py:116-435, 524-750`` and ``adjoint_solver_utils.py``
+(reference), homogeneous medium.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .beams import decode_indices, safe_angle, solve_hom_TR
+from .transforms import MSWPT, frame_phase
+
+
+def adjoint_source(data, dt, c_at_sources, window=None):
+    """msgb_solver.py:68-120: -c^2 Re IFFT_t(2 pi i f FFT_t(w r))."""
+    x = np.asarray(data, dtype=float)
+    if window is not None:
+        w = np.asarray(window, dtype=float)
+        if w.ndim == 1 and x.ndim > 1:
+            w = w.reshape((w.shape[0],) + (1,) * (x.ndim - 1))
+        x = w * x
+    f = np.fft.fftfreq(x.shape[0], d=dt)
+    mult = (2j * np.pi * f).reshape((f.shape[0],) + (1,) * (x.ndim - 1))
+    der = np.fft.ifft(mult * np.fft.fft(x, axis=0), axis=0).real
+    c = np.asarray(c_at_sources, dtype=float)
+    if c.ndim == 1 and c.shape != der.shape[1:]:
+        c = c.reshape(der.shape[1:])
+    return -(c ** 2) * der
+
+
+def mT_inverse(xdot, mT_st):
+    """tr_solver_utils.py:116-170 with pdot = 0 (homogeneous medium).
+
+    Axis 0 is treated as the detector normal, as in the reference."""
+    b, d = xdot.shape
+    A = mT_st[:, 0, 0]
+    B = mT_st[:, 0, 1:]
+    C = mT_st[:, 1:, 1:]
+    x1 = xdot[:, 0:1]
+    xs = xdot[:, 1:]
+    pd1 = np.zeros_like(x1)
+    pds = np.zeros_like(xs)
+    ur = ((pds - np.einsum("bij,bj->bi", C, xs) - B) / x1).reshape(b, 1, d - 1)
+    num = (A + np.einsum("bi,bi->b", pd1, x1) - np.einsum("bi,bi->b", pds, xs)
+           + np.einsum("bi,bij,bj->b", xs, C, xs) + 2 * np.einsum("bi,bi->b", B, xs))
+    ul = (num / x1[:, 0] ** 2).reshape(b, 1, 1)
+    top = np.concatenate([ul, ur], axis=2)
+    bot = np.concatenate([np.transpose(ur, (0, 2, 1)), C], axis=2)
+    return np.concatenate([top, bot], axis=1)
+
+
+def _isclose(a, b):
+    return abs(a - b) <= 1e-8 + 1e-5 * abs(b)
+
+
+def tr_parameters(wpt: MSWPT, idx, data_N, data_dx, src_positions, src_N, src_dx, c):
+    """tr_solver_utils.py:205-435 (d_spatial > 1 branch and 1-D branch).
+
+    Returns pts, Mts, xts, omega, ats (B,), signum (B,), t_start (B,).
+    """
+    t = wpt.tiling
+    pos = np.asarray(src_positions, dtype=float)
+    d_sp = pos.shape[1]
+    const = np.all(pos == pos[0], axis=0)
+    cax = np.flatnonzero(const)
+    normal_axis = int(cax[0]) if cax.size else 0
+    normal_value = float(pos[0, normal_axis]) if cax.size else 0.0
+    L = np.asarray(data_N, float) * np.asarray(data_dx, float)
+    lvl, multi = decode_indices(wpt, idx)
+    box = multi[0] + t.boxes_cumsum[lvl]
+    ch = t.centres[box, :] / L
+    norm = np.linalg.norm(ch, axis=-1, keepdims=True)
+    cn = ch / norm
+    tau = ch[:, 0:1]
+    ktan = ch[:, 1:]
+    bl = np.asarray(t.box_lengths)[lvl][:, None] / L * np.asarray(t.aspect)
+    Ll = bl * wpt.redundancy
+    sig = bl / 2.0
+    signum = -np.sign(ch[:, 0])
+    om = np.maximum(np.abs(tau), 1e-6)[:, 0]
+    t_start = multi[1, :] / Ll[:, 0]
+    xstar_idx = multi[1:, :] if d_sp == 1 else multi[2:, :]
+    xstar = xstar_idx.T / Ll[:, 1:] if d_sp > 1 else xstar_idx.T / Ll[:, 0:1]
+    B = len(idx)
+    mask_tan = np.arange(d_sp) != normal_axis
+    tan_index = np.where(mask_tan, np.cumsum(mask_tan) - 1, 0)
+    if d_sp == 1:
+        xts = np.full((B, 1), normal_value)
+    else:
+        xts = np.where(mask_tan, xstar[:, tan_index], normal_value)
+    cx = np.full((B, 1), float(c))
+    if d_sp == 1:
+        ptan = cn[:, 1:]
+        rad = np.maximum((cn[:, :1] / cx) ** 2 - np.sum(cn[:, 1:] ** 2, axis=1, keepdims=True), 0.0)
+    else:
+        ta = np.maximum(np.abs(tau), 1e-6)
+        ptan = cx * ktan / ta
+        rad = np.maximum(1.0 - np.sum(ptan ** 2, axis=1, keepdims=True), 0.0)
+    pn = np.sqrt(rad)
+    sdx = np.asarray(src_dx, float)
+    ssz = np.asarray(src_N, float) * sdx
+    gmax = ssz[normal_axis] - sdx[normal_axis]
+    if _isclose(normal_value, 0.0):
+        inward = 1.0
+    elif _isclose(normal_value, gmax):
+        inward = -1.0
+    else:
+        inward = float(np.sign(gmax / 2.0 - normal_value))
+    if inward == 0.0:
+        inward = 1.0
+    pn = inward * pn
+    if d_sp == 1:
+        pu = pn.copy()
+    else:
+        pu = np.where(mask_tan, ptan[:, tan_index], pn)
+    pts = 2.0 * np.pi * pu
+    if d_sp != 1:
+        pts = pts / cx
+    alpha = 2j * (np.pi * sig) ** 2 / om[:, None]
+    Minit = np.einsum("bi,ij->bij", alpha, np.eye(d_sp))
+    if d_sp == 1:
+        # mT_inverse with d = 1: M = (A) / xdot^2
+        xdot = signum[:, None] * c * pts / np.linalg.norm(pts, axis=-1, keepdims=True)
+        Mts = (Minit[:, 0, 0] / xdot[:, 0] ** 2).reshape(B, 1, 1)
+    else:
+        xdot = signum[:, None] * c * pts / np.linalg.norm(pts, axis=-1, keepdims=True)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            Mts = mT_inverse(xdot, Minit)
+    ats = np.prod(np.sqrt((np.pi * L[None, :]) / (Ll * np.asarray(data_N, float)[None, :])) * sig,
+                  axis=1)
+    ats = ats * frame_phase(t, box, multi[1:, :].T, wpt.redundancy)
+    graz = np.abs(pts[:, normal_axis]) == 0.0
+    pts = np.where(graz[:, None], 1.0, pts)
+    ats = np.where(graz, 0.0, ats)
+    Mts = np.where(graz[:, None, None], 1j * np.eye(d_sp)[None], Mts)
+    return pts, Mts, xts, om, ats, signum, t_start, tau[:, 0], ktan
+
+
+def principal_b_inverse(tau, ktan, c, guard):
+    """adjoint_solver_utils.py:17-72."""
+    om = 2.0 * np.pi * tau
+    q = 2.0 * np.pi * ktan
+    qsq = np.sum(q ** 2, axis=1) if ktan.shape[1] > 0 else np.zeros_like(om)
+    rad = om ** 2 - (c ** 2) * qsq
+    gam = np.sqrt(np.maximum(rad, 0.0))
+    valid = (rad > 0.0) & (np.abs(om) > 0.0) & (gam > guard * np.abs(om))
+    sg = np.where(valid, gam, 1.0)
+    return np.where(valid, -1j * np.sign(om) / (2.0 * c * sg), 0.0)
+
+
+def adj_parameters(wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, guard=5e-2):
+    """adjoint_solver_utils.py:75-220: TR geometry with a <- a * (-B^-1)."""
+    pts, Mts, xts, om, ats, signum, t_start, tau, ktan = tr_parameters(
+        wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c)
+    binv = -principal_b_inverse(tau, ktan, c, guard)
+    ex = binv == 0.0
+    pts = np.where(ex[:, None], 1.0, pts)
+    Mts = np.where(ex[:, None, None], 1j * np.eye(Mts.shape[-1])[None], Mts)
+    return pts, Mts, xts, om, ats * binv, signum, t_start
+
+
+def tr_field_sum(x, p, M, A, omega, points):
+    """gb/core.py:332-425 at the t = 0 column (tr_solver_utils.py:524-674):
+    sum_b 2|A| cos(w(p.d + d^T ReM d/2) + arg A) exp(-w d^T ImM d / 2)."""
+    out = np.zeros(points.shape[0])
+    ang = safe_angle(A)
+    for i in range(x.shape[0]):
+        dl = points - x[i]
+        ph = omega[i] * (dl @ p[i] + 0.5 * np.einsum("si,ij,sj->s", dl, M[i].real, dl)) + ang[i]
+        chi = 0.5 * np.einsum("si,ij,sj->s", dl, M[i].imag, dl)
+        out += 2.0 * np.abs(A[i]) * np.cos(ph) * np.exp(-omega[i] * chi)
+    return out
+
+
+def adjoint_pipeline(data, dt, data_wpt: MSWPT, data_N, data_dx, src_positions, src_N,
+                     src_dx, c, points, thr, guard=5e-2, window=None):
+    """MSGBSolver.adjoint (msgb_solver.py:1299-1386) with solve_hom_TR."""
+    from .topn import top_n
+    src = adjoint_source(data, dt, np.full(np.asarray(src_positions).shape[0], float(c)), window)
+    coeffs = data_wpt.forward(src, "spatial")
+    n_eff = min(int(thr), data_wpt.total_coeffs)
+    idx, val = top_n(coeffs, n_eff)
+    pts, Mts, xts, om, ats, signum, t_start = adj_parameters(
+        data_wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, guard)
+    ats = ats * val
+    x, p, M, A = solve_hom_TR(xts, pts, Mts, ats, signum, t_start, c)
+    q = tr_field_sum(x, p, M, A, om, np.asarray(points, float))
+    return q / (c ** 2), (idx, val, x, p, M, A, om)
