@@ -1,0 +1,447 @@
+"""MSGBSolver: MSGB forward and principal-symbol adjoint on one MI355X.
+
+API of ``beamax.solvers.MSGBSolver`` (reference
+``beamax/solvers/msgb_solvers/msgb_solver.py:356-1386``).  Host validation and
+error messages follow the reference; every numerical step runs as a HIP
+kernel through the C-ABI:
+
+forward:  bh_mswpt_forward (0.5*a*half_mask) -> bh_topn -> bh_forward_params
+          -> bh_forward_sum (mirrored beams, closed-form trajectories fused)
+adjoint:  bh_adjoint_source -> bh_mswpt_forward (data tiling) -> bh_topn ->
+          bh_adjoint_params (TR geometry, B^-1, propagation to t = 0) ->
+          bh_static_sum (times c^-2)
+
+``precision`` selects the summation arithmetic: "fp32_split" (FP64 phase,
+FP32 cos/exp/partial sums, FP64 accumulation; returns float32) or "fp64"
+(returns float64).  All other stages are FP64.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from numbers import Integral, Real
+from typing import Optional, Tuple, Union
+
+import numpy as np
+
+from .. import _lib
+from ..geometry import Domain, Sensor
+from ..gb import gb_solvers
+from ..transforms import MSWPT
+
+_THR_STRATS = {"hard", "top_n", "percentile", "hard_reassign", "bao_energy", "perc_max_abs"}
+_SUM_METHODS = {"all_real", "scan_real", "vmap_real", "all_complex", "scan_complex",
+                "vmap_complex"}
+
+
+def _validate_time_grid(ts, *, allow_singleton: bool = False) -> Optional[float]:
+    """msgb_solver.py:41-65."""
+    t = np.asarray(_np(ts), dtype=float)
+    if t.ndim != 1 or t.size == 0:
+        raise ValueError("ts must be a non-empty one-dimensional array.")
+    if not np.all(np.isfinite(t)):
+        raise ValueError("ts must contain only finite values.")
+    if t.size == 1:
+        if allow_singleton:
+            return None
+        raise ValueError("ts must be one-dimensional with at least two points.")
+    dd = np.diff(t)
+    if np.any(dd <= 0) or not np.allclose(dd, dd[0], rtol=1e-6, atol=0.0):
+        raise ValueError("ts must be finite, strictly increasing, and uniformly spaced.")
+    return float(dd[0])
+
+
+def _np(x):
+    if hasattr(x, "detach"):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+def batch_params(params, batch_size: int, zero_padded_args=(4,)):
+    """utils.batch_data (coeff_index.py:29-87) on torch tensors: pad to a batch
+    multiple (zero for `zero_padded_args`, else repeat the last entry)."""
+    import torch
+    b = params[0].shape[0]
+    nb = (b + batch_size - 1) // batch_size
+    out = []
+    for i, p in enumerate(params):
+        pad = nb * batch_size - b
+        if pad:
+            if i in zero_padded_args:
+                fill = torch.zeros((pad,) + tuple(p.shape[1:]), dtype=p.dtype, device=p.device)
+            else:
+                fill = p[-1:].expand((pad,) + tuple(p.shape[1:]))
+            p = torch.cat([p, fill], dim=0)
+        out.append(p.reshape((nb, batch_size) + tuple(p.shape[1:])))
+    return tuple(out)
+
+
+class ForwardBeams:
+    """Device beam table (unique beams; the summation mirrors them to mode -1)."""
+
+    def __init__(self, k: int, d: int, device):
+        import torch
+        self.count = int(k)
+        self.d = d
+        self.x0 = torch.empty((k, d), dtype=torch.float64, device=device)
+        self.p0 = torch.empty((k, d), dtype=torch.float64, device=device)
+        self.omega = torch.empty((k,), dtype=torch.float64, device=device)
+        self.alpha = torch.empty((k, d), dtype=torch.complex128, device=device)
+        self.a0 = torch.empty((k,), dtype=torch.complex128, device=device)
+
+    def struct(self) -> "_lib.FwdBeams":
+        return _lib.FwdBeams(self.count, self.x0.data_ptr(), self.p0.data_ptr(),
+                             self.omega.data_ptr(), self.alpha.data_ptr(), self.a0.data_ptr())
+
+
+class StaticBeams:
+    """Device table of beams frozen at one time level (adjoint beams at t = 0)."""
+
+    def __init__(self, k: int, d: int, device):
+        import torch
+        self.count = int(k)
+        self.d = d
+        self.x = torch.empty((k, d), dtype=torch.float64, device=device)
+        self.p = torch.empty((k, d), dtype=torch.float64, device=device)
+        self.M = torch.empty((k, d, d), dtype=torch.complex128, device=device)
+        self.A = torch.empty((k,), dtype=torch.complex128, device=device)
+        self.omega = torch.empty((k,), dtype=torch.float64, device=device)
+
+    def struct(self) -> "_lib.StaticBeams":
+        return _lib.StaticBeams(self.count, self.x.data_ptr(), self.p.data_ptr(),
+                                self.M.data_ptr(), self.A.data_ptr(), self.omega.data_ptr())
+
+    @staticmethod
+    def concat(tables):
+        import torch
+        t0 = tables[0]
+        out = StaticBeams.__new__(StaticBeams)
+        out.count = sum(t.count for t in tables)
+        out.d = t0.d
+        for f in ("x", "p", "M", "A", "omega"):
+            setattr(out, f, torch.cat([getattr(t, f) for t in tables], dim=0).contiguous())
+        return out
+
+
+def topn_device(coeffs, k: int):
+    """top_n on the GPU: (idx int64, values complex128), ascending (|c|, index)."""
+    import torch
+    n = coeffs.numel()
+    k = min(int(k), n)
+    if k <= 0:
+        raise ValueError("top_n threshold must be a positive integer.")
+    idx = torch.empty(k, dtype=torch.int64, device=coeffs.device)
+    val = torch.empty(k, dtype=torch.complex128, device=coeffs.device)
+    ws = _lib.workspace(_lib.load().bh_topn_workspace_size(n, k), coeffs.device)
+    _lib.call("bh_topn", _lib.ptr(coeffs), n, k, _lib.ptr(idx), _lib.ptr(val), _lib.ptr(ws),
+              ws.numel(), _lib.stream_handle())
+    return idx, val
+
+
+def detector_plane(positions, domain: Domain):
+    """Normal axis, normal coordinate and inward sign of a planar detector
+    (tr_solver_utils.py:245-250 find_constant_columns; 356-371 inward sign)."""
+    pos = np.asarray(positions, dtype=float)
+    const = np.all(pos == pos[0], axis=0)
+    axes = np.flatnonzero(const)
+    normal_axis = int(axes[0]) if axes.size else 0
+    normal_value = float(pos[0, normal_axis]) if axes.size else 0.0
+    dx = np.asarray(domain.dx, dtype=float)
+    size = domain.grid_size
+    gmax = size[normal_axis] - dx[normal_axis]
+    close = lambda a, b: abs(a - b) <= 1e-8 + 1e-5 * abs(b)
+    if close(normal_value, 0.0):
+        inward = 1.0
+    elif close(normal_value, gmax):
+        inward = -1.0
+    else:
+        inward = float(np.sign(gmax / 2.0 - normal_value))
+    if inward == 0.0:
+        inward = 1.0
+    return normal_axis, normal_value, inward
+
+
+class MSGBSolver:
+    """Multiscale Gaussian Beam solver (msgb_solver.py:356-534)."""
+
+    def __init__(self, thr: Union[int, float], thr_strat: str, batch_size: int, input_type: str,
+                 ode_solver, sum_method: str, tr_ode_solver=None, sharding=None,
+                 ode_config=None, adjoint_relative_guard: float = 5e-2,
+                 precision: str = "fp32_split"):
+        if thr_strat not in _THR_STRATS:
+            raise ValueError(f"thr_strat must be one of {', '.join(sorted(_THR_STRATS))}; "
+                             f"got {thr_strat!r}.")
+        if isinstance(thr, bool) or not isinstance(thr, Real):
+            raise ValueError(f"thr must be a finite real scalar; got {thr!r}.")
+        if not math.isfinite(float(thr)):
+            raise ValueError(f"thr must be finite; got {thr!r}.")
+        if thr_strat == "top_n":
+            if not isinstance(thr, Integral) or int(thr) <= 0:
+                raise ValueError("top_n threshold must be a positive integer.")
+            thr = int(thr)
+        elif thr_strat == "percentile":
+            if not 0 <= float(thr) <= 100:
+                raise ValueError("percentile threshold must lie in [0, 100].")
+            thr = float(thr)
+        elif thr_strat in {"hard_reassign", "perc_max_abs"}:
+            if not 0 <= float(thr) <= 1:
+                raise ValueError(f"{thr_strat} threshold must lie in [0, 1].")
+            thr = float(thr)
+        else:
+            if float(thr) < 0:
+                raise ValueError(f"{thr_strat} threshold must be non-negative.")
+            thr = float(thr)
+        if input_type not in {"spatial", "fourier"}:
+            raise ValueError(f"input_type must be 'spatial' or 'fourier'; got {input_type!r}.")
+        if batch_size <= 0:
+            raise ValueError(f"batch_size must be positive; got {batch_size}.")
+        if not 0.0 <= adjoint_relative_guard < 1.0:
+            raise ValueError("adjoint_relative_guard must lie in [0, 1); got "
+                             f"{adjoint_relative_guard}.")
+        if sum_method not in _SUM_METHODS:
+            raise ValueError(f"sum_method must be one of {', '.join(sorted(_SUM_METHODS))}; "
+                             f"got {sum_method!r}.")
+        if precision not in _lib.PRECISION:
+            raise ValueError(f"precision must be 'fp32_split' or 'fp64'; got {precision!r}.")
+        self.thr = thr
+        self.thr_strat = thr_strat
+        self.batch_size = batch_size
+        self.input_type = input_type
+        self.ode_solver = ode_solver
+        self.tr_ode_solver = ode_solver if tr_ode_solver is None else tr_ode_solver
+        self.sharding = sharding
+        self.ode_config = ode_config if ode_config is not None else gb_solvers.SolverConfig.from_precision()
+        self.adjoint_relative_guard = float(adjoint_relative_guard)
+        self.use_real = "real" in sum_method
+        self.aggregate_method = ("scan" if "scan" in sum_method else
+                                 "vmap" if "vmap" in sum_method else "all")
+        self.precision = precision
+
+    # ------------------------------------------------------------------
+    def _effective_top_n(self, wpt: MSWPT, *, half_frame: bool = False):
+        """msgb_solver.py:536-549."""
+        if self.thr_strat != "top_n":
+            return self.thr
+        cap = wpt.half_frame_capacity if half_frame else wpt.total_coeffs
+        return min(int(self.thr), int(cap))
+
+    def _check_supported(self):
+        if self.thr_strat != "top_n":
+            raise NotImplementedError(f"thr_strat={self.thr_strat!r} is not on the GPU path; "
+                                      "use 'top_n'.")
+        if not self.use_real:
+            raise NotImplementedError("complex beam summation is not on the GPU path; use *_real.")
+        if self.sharding is not None:
+            raise NotImplementedError("multi-device sharding is not part of this single-GPU build")
+        if self.ode_solver is not gb_solvers.solve_hom_diag:
+            raise NotImplementedError("only the closed-form solve_hom_diag propagator is fused "
+                                      "into the GPU summation (homogeneous media)")
+
+    @staticmethod
+    def _sensor_positions(sensors):
+        import torch
+        if isinstance(sensors, Sensor):
+            return np.asarray(sensors.positions, dtype=float)
+        if isinstance(sensors, (np.ndarray, torch.Tensor)):
+            return np.asarray(_np(sensors), dtype=float)
+        return None
+
+    # ------------------------------------------------------------------
+    def prepare_forward_beams(self, p0, domain: Domain, wpt: MSWPT):
+        """compute_coefficients('pos_only') -> top_n -> compute_forward_parameters
+        (msgb_solver.py:570-633); returns (ForwardBeams, idx, values)."""
+        cpos = wpt.forward(p0, self.input_type, scale=0.5, half_mask=True)
+        k = self._effective_top_n(wpt, half_frame=True)
+        idx, val = topn_device(cpos, k)
+        beams = ForwardBeams(idx.numel(), domain.ndim, wpt.device)
+        gs = _lib.dbl_array(domain.grid_size)
+        _lib.call("bh_forward_params", C.byref(wpt.tiling), gs, _lib.ptr(idx), _lib.ptr(val),
+                  idx.numel(), beams.struct(), _lib.stream_handle())
+        return beams, idx, val
+
+    def sum_forward(self, beams: ForwardBeams, domain: Domain, sensor_positions, ts, out=None):
+        """Mirrored real beam sum at the sensors, (Nt, S)."""
+        import torch
+        dev = beams.x0.device
+        d = domain.ndim
+        pos = torch.as_tensor(np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d),
+                              device=dev)
+        tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
+        nt, S = tsd.numel(), pos.shape[0]
+        dt = torch.float64 if self.precision == "fp64" else torch.float32
+        if out is None:
+            out = torch.empty((nt, S), dtype=dt, device=dev)
+        per = _lib.int32_array([int(p) for p in domain.periodic])
+        b = beams.struct()
+        _lib.call("bh_forward_sum", C.byref(b), d, 1, float(domain.c), _lib.ptr(tsd), nt,
+                  _lib.ptr(pos), S, _lib.dbl_array(domain.grid_size), per,
+                  _lib.PRECISION[self.precision], _lib.ptr(out), _lib.stream_handle())
+        return out
+
+    def forward(self, p0, domain: Domain, sensors, ts, wpt: MSWPT, *, dpdt=None):
+        """Sensor data (Nt, *sensors.shape[:-1]) (msgb_solver.py:858-899)."""
+        data, _ = self.forward_with_params(p0, domain, sensors, ts, wpt, dpdt=dpdt)
+        return data
+
+    def forward_with_params(self, p0, domain: Domain, sensors, ts, wpt: MSWPT, *, dpdt=None):
+        """msgb_solver.py:901-982; params = (p0s, M0s, x0s, omegas, a0s, modes)."""
+        import torch
+        if tuple(p0.shape) != domain.N or (dpdt is not None and tuple(dpdt.shape) != domain.N):
+            raise ValueError(f"p0 and dpdt must both have shape {domain.N}; got "
+                             f"{tuple(p0.shape)} and "
+                             f"{tuple(dpdt.shape) if dpdt is not None else tuple(p0.shape)}.")
+        if wpt.dyadic_decomp.N != domain.N:
+            raise ValueError("wpt grid shape must match domain.N.")
+        _validate_time_grid(ts, allow_singleton=True)
+        pos = self._sensor_positions(sensors)
+        if pos is None:
+            raise ValueError("Unsupported sensor type")
+        is_complex = (p0.is_complex() if isinstance(p0, torch.Tensor) else np.iscomplexobj(p0))
+        if is_complex:
+            raise NotImplementedError("complex p0 (msgb_solver.py:635-689) is not on the GPU path")
+        if dpdt is not None:
+            nz = (torch.count_nonzero(dpdt).item() if isinstance(dpdt, torch.Tensor)
+                  else int(np.count_nonzero(np.asarray(dpdt))))
+            if nz:
+                raise NotImplementedError("dpdt != 0 is not on the GPU path")
+        if not domain.is_homogeneous:
+            raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
+        self._check_supported()
+        beams, _, _ = self.prepare_forward_beams(p0, domain, wpt)
+        data = self.sum_forward(beams, domain, pos, ts)
+        out_shape = (data.shape[0],) + tuple(np.asarray(_np(sensors.positions) if isinstance(sensors, Sensor) else pos).shape[:-1])
+        data = data.reshape(out_shape)
+        return data, self._params_tuple(beams)
+
+    def _params_tuple(self, beams: ForwardBeams):
+        """Mirrored parameters in the reference's tuple order (batched for scan/vmap)."""
+        import torch
+        d = beams.d
+        M0 = torch.diag_embed(beams.alpha)
+        ones = torch.ones(beams.count, dtype=torch.float64, device=beams.x0.device)
+        dup = lambda t: torch.cat([t, t], dim=0)
+        params = (dup(beams.p0), dup(M0), dup(beams.x0), dup(beams.omega), dup(beams.a0),
+                  torch.cat([ones, -ones]))
+        if self.aggregate_method in ("scan", "vmap"):
+            params = batch_params(params, self.batch_size, zero_padded_args=(4,))
+        return params
+
+    # ------------------------------------------------------------------
+    def _validate_boundary_data(self, data, data_domain: Domain, sources, *, ts=None) -> None:
+        """msgb_solver.py:806-856."""
+        if tuple(data.shape) != data_domain.N:
+            raise ValueError(f"Boundary data must have shape {data_domain.N}; got {tuple(data.shape)}.")
+        if ts is not None:
+            _validate_time_grid(ts)
+            span = (data.shape[0] - 1) * data_domain.dx[0]
+            t = np.asarray(_np(ts), dtype=float)
+            if not np.isclose(float(t[-1] - t[0]), span, rtol=1e-6, atol=1e-12):
+                raise ValueError(f"ts spans {float(t[-1] - t[0])}, but data_domain spans {span}.")
+        if not isinstance(sources, Sensor):
+            raise ValueError("sources must be a Sensor on a regular planar grid.")
+        pos = np.asarray(sources.positions)
+        stds = np.std(pos, axis=0)
+        normal_axis = int(np.argmin(stds))
+        tol = 1e-7 * max(1.0, float(np.max(sources.domain.grid_size)))
+        if stds[normal_axis] > tol:
+            raise ValueError("sources must lie on an axis-aligned planar surface.")
+        counts = []
+        for ax in [a for a in range(pos.shape[1]) if a != normal_axis]:
+            vals = np.unique(pos[:, ax])
+            if vals.size > 2 and not np.allclose(np.diff(vals), np.diff(vals)[0], rtol=1e-6, atol=tol):
+                raise ValueError("sources must be uniformly spaced on the detector plane.")
+            counts.append(int(vals.size))
+        expected = tuple(counts) if counts else (1,)
+        actual = tuple(data.shape[1:]) or (1,)
+        if actual != expected:
+            raise ValueError("Boundary data detector axes do not match the source grid: "
+                             f"got {actual}, expected {expected}.")
+        if math.prod(expected) != pos.shape[0]:
+            raise ValueError("sources must form a complete Cartesian detector grid.")
+
+    def adjoint_source(self, data, dt: float, c: float, window=None):
+        """_form_adjoint_source (msgb_solver.py:68-120) on the GPU; returns (Nt, Ns...)."""
+        import torch
+        dev = data.device
+        x = data.to(torch.float64).contiguous()
+        nt = x.shape[0]
+        ns = x.numel() // nt
+        out = torch.empty_like(x)
+        csrc = torch.full((ns,), float(c), dtype=torch.float64, device=dev)
+        w = None
+        w2d = 0
+        if window is not None:
+            w = torch.as_tensor(_np(window), device=dev).to(torch.float64).contiguous()
+            w2d = int(w.ndim > 1)
+        ws = _lib.workspace(_lib.load().bh_adjoint_source_workspace_size(nt, ns), dev)
+        _lib.call("bh_adjoint_source", _lib.ptr(x), nt, ns, float(dt), _lib.ptr(w), w2d,
+                  _lib.ptr(csrc), _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream_handle())
+        return out
+
+    def prepare_adjoint_beams(self, data, sources: Sensor, data_domain: Domain, data_wpt: MSWPT,
+                              c: float, window=None, swap_axes: bool = False,
+                              plane=None) -> StaticBeams:
+        """adjoint source -> data MSWPT -> top_n -> compute_adj_parameters ->
+        solve_hom_TR to t = 0 (msgb_solver.py:1168-1233, 1347-1366)."""
+        src = self.adjoint_source(data, data_domain.dx[0], c, window)
+        coeffs = data_wpt.forward(src, self.input_type)
+        idx, val = topn_device(coeffs, self._effective_top_n(data_wpt))
+        if plane is None:
+            plane = detector_plane(sources.positions, sources.domain)
+        normal_axis, normal_value, inward = plane
+        beams = StaticBeams(idx.numel(), sources.domain.ndim, data_wpt.device)
+        _lib.call("bh_adjoint_params", C.byref(data_wpt.tiling),
+                  _lib.dbl_array(data_domain.grid_size), _lib.ptr(idx), _lib.ptr(val), idx.numel(),
+                  normal_axis, normal_value, inward, float(c), self.adjoint_relative_guard,
+                  int(swap_axes), beams.struct(), _lib.stream_handle())
+        return beams
+
+    def sum_static(self, beams: StaticBeams, points, scale: float, out=None):
+        """out[P] = scale * sum_b field_b(points) (compute_TR_result at t = 0)."""
+        import torch
+        dev = beams.x.device
+        d = beams.d
+        pts = torch.as_tensor(np.ascontiguousarray(points, dtype=np.float64).reshape(-1, d),
+                              device=dev) if not isinstance(points, torch.Tensor) else points
+        P = pts.shape[0]
+        dt = torch.float64 if self.precision == "fp64" else torch.float32
+        if out is None:
+            out = torch.empty(P, dtype=dt, device=dev)
+        ws = _lib.workspace(_lib.load().bh_static_sum_workspace_size(beams.count, d), dev)
+        b = beams.struct()
+        _lib.call("bh_static_sum", C.byref(b), d, _lib.ptr(pts), P, float(scale),
+                  _lib.PRECISION[self.precision], _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                  _lib.stream_handle())
+        return out
+
+    def adjoint(self, data, domain: Domain, sensors, sources: Sensor, ts, data_domain: Domain,
+                data_wpt: MSWPT, *, window=None):
+        """Principal-symbol approximation of P* data (msgb_solver.py:1235-1297)."""
+        q, _ = self.adjoint_with_params(data, domain, sensors, sources, ts, data_domain, data_wpt,
+                                        window=window)
+        return q
+
+    def adjoint_with_params(self, data, domain: Domain, sensors, sources: Sensor, ts,
+                            data_domain: Domain, data_wpt: MSWPT, *, window=None):
+        """msgb_solver.py:1299-1386."""
+        import torch
+        if any(domain.periodic):
+            raise ValueError("MSGBSolver.adjoint currently assumes non-periodic spatial "
+                             "boundaries in the reconstruction domain.")
+        pos = self._sensor_positions(sensors)
+        if pos is None:
+            raise ValueError("Unsupported sensor type for `sensors` in adjoint().")
+        self._validate_boundary_data(data, data_domain, sources, ts=ts)
+        if not (domain.is_homogeneous and sources.domain.is_homogeneous):
+            raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
+        if self.thr_strat != "top_n" or self.sharding is not None:
+            self._check_supported()
+        if not isinstance(data, torch.Tensor):
+            data = torch.as_tensor(np.asarray(data, dtype=np.float64), device=data_wpt.device)
+        c_src = float(sources.domain.c)
+        beams = self.prepare_adjoint_beams(data.to(data_wpt.device), sources, data_domain,
+                                           data_wpt, c_src, window)
+        q = self.sum_static(beams, pos, 1.0 / float(domain.c) ** 2)
+        shape = np.asarray(sensors.positions if isinstance(sensors, Sensor) else pos).shape[:-1]
+        return q.reshape(tuple(shape)), beams

@@ -1,0 +1,236 @@
+"""Benchmark: MSGB forward + adjoint at 512^2 on one MI355X.
+
+Workload (one step, inputs resident in HBM):
+  forward  p0 (512^2, 64 seeded wave packets) -> MSWPT (4 levels, boxes
+           4/8/16/32, rho 2) -> top_n 65536 -> 131072 mirrored beams ->
+           sensor data (Nt = 2414) x (2044 four-edge sensors), FP32-split
+  adjoint  the data split into its 4 faces (corners weighted 1/2), each
+           Fourier-cropped to 1024 time samples, data MSWPT (2 levels, 4/8,
+           aspect 2:1), top_n 65536 per face -> 262144 beams summed over
+           the 512^2 image, FP32-split.
+Dense work per step: 131072*2414*2044 + 4*65536*262144 beam-sample
+evaluations.  The FP32-split outputs are checked against the FP64 kernels on
+the same beams (rel-L2 and max-rel) outside the timed region.
+
+Prints one JSON line (see the metric contract in the repository docs).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("Gbeam-evals/s MSGB fwd+adj (512², err<1e-5 vs FP64); "
+          "recon s to 1e-3 resid")
+FLOP_PER_EVAL_2D = 25.0
+FP32_VALU_PEAK_TFLOPS = 157.3
+
+
+def build_problem(device):
+    import beamax_amd as bx
+    from beamax_amd.gb import gb_solvers
+    from beamax_amd.solvers.faces import FourEdgeAdjoint, four_edge_mask
+    from beamax_amd.workloads import packets_2d
+
+    N = (512, 512)
+    dx = (5e-5, 5e-5)
+    dom = bx.Domain(N=N, dx=dx, c=1500.0, cfl=0.3, periodic=(False, False))
+    dd = bx.DyadicDecomposition(4, N, (4, 8, 16, 32), (1, 1))
+    wpt = bx.MSWPT(dd, 2, "rectangular_mirror", device=device)
+    sensors = bx.Sensor(dom, binary_mask=four_edge_mask(N))
+    ts = dom.generate_time_domain()
+    import torch
+    p0 = torch.as_tensor(packets_2d(N, dx, 64, seed=0), device=device)
+    solvers = {}
+    for prec in ("fp32_split", "fp64"):
+        solvers[prec] = bx.MSGBSolver(65536, "top_n", 128, "spatial", gb_solvers.solve_hom_diag,
+                                      "scan_real", precision=prec)
+    adj = {prec: FourEdgeAdjoint(solvers[prec], dom, ts, 1024, device=device) for prec in solvers}
+    return dict(dom=dom, wpt=wpt, sensors=sensors, ts=ts, p0=p0, solvers=solvers, adj=adj)
+
+
+def run_step(P, prec, timers=None):
+    """One forward+adjoint pass; returns (data, image, beams_fwd, beams_adj)."""
+    import torch
+    s = P["solvers"][prec]
+    beams, _, _ = s.prepare_forward_beams(P["p0"], P["dom"], P["wpt"])
+    if timers is not None:
+        timers[0].record()
+    data = s.sum_forward(beams, P["dom"], P["sensors"].positions, P["ts"])
+    if timers is not None:
+        timers[1].record()
+    ab = P["adj"][prec].beams(data)
+    if timers is not None:
+        timers[2].record()
+    img = s.sum_static(ab, P["adj"][prec].points, 1.0 / P["dom"].c ** 2)
+    if timers is not None:
+        timers[3].record()
+    return data, img, beams, ab
+
+
+def cpu_baseline(P, beams, ab, budget_s=20.0):
+    """The oracle's C++/OpenMP FP64 port on the host cores, on a prefix of
+    beam batches (cost is linear in the beam count), extrapolated to the
+    full workload."""
+    from oracle import cpu as ocpu
+    ts = P["ts"]
+    S = P["sensors"].positions
+    n = lambda t: t.detach().cpu().numpy()
+    x0, p0, om, al, a0 = n(beams.x0), n(beams.p0), n(beams.omega), n(beams.alpha), n(beams.a0)
+    k = x0.shape[0]
+    dup = lambda v: np.concatenate([v, v])
+    mode = np.concatenate([np.ones(k), -np.ones(k)])
+    X, Pp, AL, A0, OM = dup(x0), dup(p0), dup(al), dup(a0), dup(om)
+    cores = ocpu.ncores()
+    ocpu.load()
+    nb = 0
+    t_fwd = 0.0
+    while t_fwd < budget_s * 0.75 and nb < 2 * k:
+        step = 128
+        t0 = time.perf_counter()
+        ocpu.forward_sum(X, Pp, AL, A0, OM, mode, 1500.0, ts, S, nthreads=cores,
+                         beam_begin=nb, beam_end=nb + step)
+        t_fwd += time.perf_counter() - t0
+        nb += step
+    fwd_rate = nb * ts.size * S.shape[0] / t_fwd
+    # adjoint: a prefix of beams over all pixels
+    pts = n(P["adj"]["fp64"].points)
+    nab = min(ab.count, 256)
+    t0 = time.perf_counter()
+    ocpu.static_sum(n(ab.x)[:nab], n(ab.p)[:nab], n(ab.M)[:nab], n(ab.A)[:nab], n(ab.omega)[:nab],
+                    pts, nthreads=cores)
+    t_adj = time.perf_counter() - t0
+    adj_rate = nab * pts.shape[0] / t_adj
+    fwd_evals = 2 * k * ts.size * S.shape[0]
+    adj_evals = ab.count * pts.shape[0]
+    total_t = fwd_evals / fwd_rate + adj_evals / adj_rate
+    return {
+        "value": (fwd_evals + adj_evals) / total_t / 1e9,
+        "unit": "Gbeam-evals/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle C++/OpenMP FP64 beam sums on {cores} host threads: forward "
+                   f"{nb} of {2 * k} beams x {ts.size} steps x {S.shape[0]} sensors "
+                   f"({t_fwd:.1f} s), adjoint {nab} of {ab.count} beams x {pts.shape[0]} pixels "
+                   f"({t_adj:.1f} s); extrapolated linearly in the beam count"),
+        "fwd_rate": fwd_rate / 1e9,
+        "adj_rate": adj_rate / 1e9,
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(this build runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    P = build_problem(dev)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        run_step(P, "fp32_split")
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        data, img, beams, ab = run_step(P, "fp32_split", ev[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    fwd_ms = np.median([ev[i][0].elapsed_time(ev[i][1]) for i in range(args.steps)])
+    adjp_ms = np.median([ev[i][1].elapsed_time(ev[i][2]) for i in range(args.steps)])
+    adj_ms = np.median([ev[i][2].elapsed_time(ev[i][3]) for i in range(args.steps)])
+    nt, S = P["ts"].size, P["sensors"].positions.shape[0]
+    npix = int(np.prod(P["dom"].N))
+    fwd_evals = 2 * beams.count * nt * S
+    adj_evals = ab.count * npix
+    total = fwd_evals + adj_evals
+    value = total / (ms_step * 1e-3) / 1e9
+    fwd_tflops = fwd_evals * FLOP_PER_EVAL_2D / (fwd_ms * 1e-3) / 1e12
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gbeam-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "phase_dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("2-D 512^2 MSGB forward+adjoint: dx 5e-5, c 1500, cfl 0.3, 4 levels "
+                         "(4,8,16,32) rho 2, 64 seeded packets, top_n 65536 -> 131072 beams x "
+                         f"{nt} steps x {S} four-edge sensors; adjoint 4 faces x 65536 beams "
+                         "(data 1024 x 512, 2 levels 4/8, aspect 2:1) on 512^2 pixels"),
+            "forward_evals": fwd_evals,
+            "adjoint_evals": adj_evals,
+            "parallelism": "single GPU",
+        },
+        "breakdown_ms": {"forward_sum": round(float(fwd_ms), 3),
+                         "adjoint_prep": round(float(adjp_ms), 3),
+                         "adjoint_sum": round(float(adj_ms), 3),
+                         "forward_prep_and_rest": round(float(ms_step - fwd_ms - adjp_ms - adj_ms), 3)},
+        "setup_s": round(setup_s, 3),
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_forward_sum (bh_forward_sum, FP32-split)",
+            "achieved": round(fwd_tflops, 3),
+            "peak": FP32_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(fwd_tflops / FP32_VALU_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "flop_per_eval": FLOP_PER_EVAL_2D,
+            "evals_per_launch": fwd_evals,
+            "launch_ms": round(float(fwd_ms), 3),
+        },
+    }
+    if not args.no_check:
+        d64, i64, b64, a64 = run_step(P, "fp64")
+        torch.cuda.synchronize()
+        from beamax_amd.utils import max_rel, rel_l2
+        result["error_vs_fp64"] = {
+            "forward_rel_l2": rel_l2(d64, data), "forward_max_rel": max_rel(d64, data),
+            "adjoint_rel_l2": rel_l2(i64, img), "adjoint_max_rel": max_rel(i64, img)}
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(P, beams, ab)
+        result["speedup_vs_cpu"] = round(value / result["cpu_baseline"]["value"], 2)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

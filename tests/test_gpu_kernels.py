@@ -1,0 +1,288 @@
+"""GPU parity of the individual HIP kernels against the FP64 oracle.
+
+Tolerances: integer/index work is compared exactly; FP64 kernels to ~1e-12
+relative (different but correctly-rounded op order); the FP32-split summation
+to rel-L2 <= 1e-5 against the FP64 kernel (the north-star gate).
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import beams as obeams
+from oracle import topn as otopn
+from oracle.decomposition import Tiling
+from oracle.transforms import MSWPT as OMSWPT
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, dev):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(x), device=dev)
+
+
+def _n(x):
+    return x.detach().cpu().numpy()
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm(a - b) / (np.linalg.norm(a) + 1e-300))
+
+
+# ---------------------------------------------------------------------------
+# FFT
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 4, 6, 12, 64, 77, 512, 1024, 1207, 2414])
+def test_fft_columns_match_numpy(gpu, n):
+    import torch
+    from beamax_amd import _lib
+    rng = np.random.default_rng(n)
+    ns = 7
+    x = rng.standard_normal((n, ns)) + 1j * rng.standard_normal((n, ns))
+    d = _t(x, gpu)
+    ws = _lib.workspace(_lib.load().bh_fft_workspace_size(n, ns), gpu)
+    _lib.call("bh_fft", _lib.ptr(d), n, ns, ns, 1, n * ns, ns, -1, 1.0, _lib.ptr(ws), ws.numel(),
+              _lib.stream_handle())
+    ref = np.fft.fft(x, axis=0)
+    assert rel(ref, _n(d)) < 1e-13
+    _lib.call("bh_fft", _lib.ptr(d), n, ns, ns, 1, n * ns, ns, 1, 1.0 / n, _lib.ptr(ws), ws.numel(),
+              _lib.stream_handle())
+    assert rel(x, _n(d)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [8, 96, 512])
+def test_fft_rows_match_numpy(gpu, n):
+    from beamax_amd import _lib
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((5, n)) + 1j * rng.standard_normal((5, n))
+    d = _t(x, gpu)
+    _lib.call("bh_fft", _lib.ptr(d), n, 5, 1, 0, n, 1, 1, 1.0, None, 0, _lib.stream_handle())
+    assert rel(np.fft.ifft(x, axis=1) * n, _n(d)) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# MSWPT
+# ---------------------------------------------------------------------------
+TILINGS = [
+    (3, (128, 128), (4, 8, 16), (1, 1)),
+    (2, (64, 32), (4, 8), (2, 1)),
+    (2, (32, 64), (4, 8), (1, 2)),
+    (2, (32, 32, 32), (4, 8), (1, 1, 1)),
+    (1, (64,), (8,), (1,)),
+]
+
+
+def _pair(cfg, windowing="rectangular_mirror"):
+    import beamax_amd as bx
+    lv, N, nb, asp = cfg
+    dd = bx.DyadicDecomposition(lv, N, nb, asp)
+    return bx.MSWPT(dd, 2, windowing), OMSWPT(Tiling(lv, N, nb, asp), 2, windowing)
+
+
+@pytest.mark.parametrize("cfg", TILINGS)
+@pytest.mark.parametrize("windowing", ["rectangular_mirror", "rectangular"])
+def test_mswpt_tables_match_oracle(gpu, cfg, windowing):
+    w, o = _pair(cfg, windowing)
+    assert rel(o.sum_gsquare, _n(w.sum_gsquare)) < 1e-14
+    packed = np.concatenate([g.ravel() for g in o.gpacked])
+    assert rel(packed, _n(w.gfilts_packed)) < 1e-14
+    assert np.array_equal(o.half_mask, w._half_mask)
+    assert tuple(o.coeffs_cumsum) == tuple(w.coeffs_cumsum)
+
+
+@pytest.mark.parametrize("cfg", TILINGS)
+def test_mswpt_forward_matches_oracle(gpu, cfg):
+    w, o = _pair(cfg)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(cfg[1])
+    assert rel(o.forward(x), _n(w.forward(_t(x, gpu)))) < 1e-12
+    z = x + 1j * rng.standard_normal(cfg[1])
+    assert rel(o.forward(z), _n(w.forward(_t(z, gpu)))) < 1e-12
+    F = np.fft.fftshift(np.fft.fftn(z, norm="ortho"))
+    assert rel(o.forward(F, "fourier"), _n(w.forward(_t(F, gpu), "fourier"))) < 1e-12
+    # compute_coefficients 'pos_only' epilogue
+    ref = 0.5 * o.forward(x) * o.half_mask
+    got = _n(w.forward(_t(x, gpu), scale=0.5, half_mask=True))
+    assert rel(ref, got) < 1e-12
+    assert np.all(got[o.half_mask == 0] == 0)
+
+
+@pytest.mark.parametrize("cfg", TILINGS)
+def test_mswpt_inverse_roundtrip_and_oracle(gpu, cfg):
+    w, o = _pair(cfg)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(cfg[1])
+    c = w.forward(_t(x, gpu))
+    back = _n(w.inverse(c))
+    assert np.max(np.abs(back.real - x)) < 1e-12 * max(1.0, np.max(np.abs(x)))
+    cr = rng.standard_normal(w.total_coeffs) + 1j * rng.standard_normal(w.total_coeffs)
+    assert rel(o.inverse(cr), _n(w.inverse(_t(cr, gpu)))) < 1e-12
+    assert rel(o.inverse(cr, "fourier"), _n(w.inverse(_t(cr, gpu), "fourier"))) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# top_n
+# ---------------------------------------------------------------------------
+def _tie_heavy(rng, n):
+    re = rng.integers(-3, 4, n).astype(float) * 0.25
+    im = rng.integers(-3, 4, n).astype(float) * 0.25
+    c = re + 1j * im
+    c[rng.random(n) < 0.3] = 0.0
+    return c
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (5, 5), (5, 3), (2047, 100), (2048, 2048), (5000, 4097),
+                                 (100000, 4096), (262144, 65536)])
+@pytest.mark.parametrize("kind", ["random", "ties"])
+def test_topn_matches_oracle(gpu, n, k, kind):
+    from beamax_amd.solvers.msgb_solver import topn_device
+    rng = np.random.default_rng(n + k)
+    c = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) if kind == "random" else _tie_heavy(rng, n)
+    idx, val = topn_device(_t(c, gpu), k)
+    ridx, rval = otopn.top_n(c, k)
+    assert np.array_equal(_n(idx), ridx)
+    assert np.array_equal(_n(val), rval)
+
+
+def test_topn_rejects_nonpositive_k(gpu):
+    from beamax_amd.solvers.msgb_solver import topn_device
+    with pytest.raises(ValueError, match="positive integer"):
+        topn_device(_t(np.ones(4, complex), gpu), 0)
+
+
+# ---------------------------------------------------------------------------
+# beam parameters and propagation
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("cfg", TILINGS[:4])
+def test_forward_params_match_oracle(gpu, cfg):
+    import beamax_amd as bx
+    from beamax_amd.solvers.msgb_solver import ForwardBeams
+    from beamax_amd import _lib
+    w, o = _pair(cfg)
+    N = cfg[1]
+    L = np.asarray(N) * 1e-4
+    rng = np.random.default_rng(5)
+    idx = np.sort(rng.choice(w.total_coeffs, 300, replace=False)).astype(np.int64)
+    vals = rng.standard_normal(300) + 1j * rng.standard_normal(300)
+    fb = ForwardBeams(300, len(N), gpu)
+    _lib.call("bh_forward_params", C.byref(w.tiling), _lib.dbl_array(L), _lib.ptr(_t(idx, gpu)),
+              _lib.ptr(_t(vals, gpu)), 300, fb.struct(), _lib.stream_handle())
+    p, al, x0, om, a0, _ = obeams.forward_parameters(o, idx, L, N)
+    assert rel(p, _n(fb.p0)) < 1e-14
+    assert rel(al, _n(fb.alpha)) < 1e-14
+    assert rel(x0, _n(fb.x0)) < 1e-14
+    assert rel(om, _n(fb.omega)) < 1e-14
+    assert rel(a0 * vals, _n(fb.a0)) < 1e-13
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_solve_hom_diag_matches_oracle(gpu, d):
+    from beamax_amd.gb import gb_solvers
+    rng = np.random.default_rng(d)
+    b = 33
+    x0 = rng.random((b, d)) * 0.01
+    xi = rng.standard_normal((b, d)) * 3000
+    p0 = 2 * np.pi * xi / np.linalg.norm(xi, axis=1, keepdims=True)
+    om = np.linalg.norm(xi, axis=1)
+    alpha = 1j * (rng.random((b, d)) + 0.2) * 2 * (np.pi * 1e3) ** 2 / om[:, None]
+    a0 = rng.standard_normal(b) + 1j * rng.standard_normal(b)
+    mode = np.where(rng.random(b) < 0.5, 1.0, -1.0)
+    ts = np.linspace(0, 2e-5, 17)
+    M0 = np.stack([np.diag(a) for a in alpha])
+    xt, pt, Mt, At = gb_solvers.solve_hom_diag(_t(x0, gpu), _t(p0, gpu), _t(M0, gpu), _t(a0, gpu),
+                                               _t(mode, gpu), _t(ts, gpu), 1500.0)
+    rx, rp, rM, rA = obeams.solve_hom_diag(x0, p0, alpha, a0, mode, ts, 1500.0)
+    assert rel(rx, _n(xt)) < 1e-14
+    assert rel(rM, _n(Mt)) < 1e-13
+    assert rel(rA, _n(At)[..., 0]) < 1e-13
+
+
+def _random_beams(rng, B, d, L, kmax):
+    x0 = rng.random((B, d)) * L
+    xi = rng.standard_normal((B, d))
+    xi = xi / np.linalg.norm(xi, axis=1, keepdims=True) * rng.uniform(0.1, 1.0, (B, 1)) * kmax
+    om = np.linalg.norm(xi, axis=1)
+    p0 = 2 * np.pi * xi / om[:, None]
+    sig = rng.uniform(0.3, 3.0, (B, d)) * kmax / 32
+    alpha = 2j * (np.pi * sig) ** 2 / om[:, None]
+    a0 = (rng.standard_normal(B) + 1j * rng.standard_normal(B)) * 1e-2
+    return x0, p0, om, alpha, a0
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_forward_sum_matches_oracle(gpu, d):
+    import torch
+    from beamax_amd import _lib
+    from beamax_amd.solvers.msgb_solver import ForwardBeams
+    from oracle import cpu
+    rng = np.random.default_rng(10 + d)
+    B, L, kmax = 200, 0.0064, 5000.0
+    x0, p0, om, alpha, a0 = _random_beams(rng, B, d, L, kmax)
+    fb = ForwardBeams(B, d, gpu)
+    for name, v in (("x0", x0), ("p0", p0), ("omega", om), ("alpha", alpha), ("a0", a0)):
+        getattr(fb, name).copy_(_t(v, gpu))
+    ts = np.arange(0, 6e-6, 2e-8)
+    sens = rng.random((300, d)) * L
+    mode = np.concatenate([np.ones(B), -np.ones(B)])
+    dup = lambda v: np.concatenate([v, v])
+    ref = cpu.forward_sum(dup(x0), dup(p0), dup(alpha), dup(a0), dup(om), mode, 1500.0, ts, sens)
+    outs = {}
+    for prec, dt in (("fp64", torch.float64), ("fp32_split", torch.float32)):
+        out = torch.empty((ts.size, 300), dtype=dt, device=gpu)
+        _lib.call("bh_forward_sum", C.byref(fb.struct()), d, 1, 1500.0, _lib.ptr(_t(ts, gpu)),
+                  ts.size, _lib.ptr(_t(sens, gpu)), 300, _lib.dbl_array([L] * d),
+                  _lib.int32_array([0] * d), _lib.PRECISION[prec], _lib.ptr(out),
+                  _lib.stream_handle())
+        outs[prec] = _n(out).astype(np.float64)
+    assert rel(ref, outs["fp64"]) < 1e-12
+    assert rel(ref, outs["fp32_split"]) < 1e-5
+
+
+def test_forward_sum_periodic_matches_oracle(gpu):
+    import torch
+    from beamax_amd import _lib
+    from beamax_amd.solvers.msgb_solver import ForwardBeams
+    rng = np.random.default_rng(7)
+    d, B, L, kmax = 2, 20, 0.0064, 3000.0
+    x0, p0, om, alpha, a0 = _random_beams(rng, B, d, L, kmax)
+    fb = ForwardBeams(B, d, gpu)
+    for name, v in (("x0", x0), ("p0", p0), ("omega", om), ("alpha", alpha), ("a0", a0)):
+        getattr(fb, name).copy_(_t(v, gpu))
+    ts = np.arange(0, 8e-6, 4e-7)
+    sens = rng.random((50, d)) * L
+    out = torch.empty((ts.size, 50), dtype=torch.float64, device=gpu)
+    _lib.call("bh_forward_sum", C.byref(fb.struct()), d, 0, 1500.0, _lib.ptr(_t(ts, gpu)), ts.size,
+              _lib.ptr(_t(sens, gpu)), 50, _lib.dbl_array([L, L]), _lib.int32_array([1, 1]),
+              0, _lib.ptr(out), _lib.stream_handle())
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p0, alpha, a0, np.ones(B), ts, 1500.0)
+    xt = np.mod(xt, L)
+    ref = obeams.real_beam_sum(xt, pt, Mt, At, om, sens, [L, L], [True, True])
+    assert rel(ref, _n(out)) < 1e-12
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_static_sum_matches_oracle(gpu, d):
+    import torch
+    from beamax_amd import _lib
+    from beamax_amd.solvers.msgb_solver import StaticBeams
+    from oracle import cpu
+    rng = np.random.default_rng(20 + d)
+    B, L, kmax = 150, 0.0064, 5000.0
+    x0, p0, om, alpha, a0 = _random_beams(rng, B, d, L, kmax)
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p0, alpha, a0, np.ones(B), np.array([3e-6]), 1500.0)
+    sb = StaticBeams(B, d, gpu)
+    sb.x.copy_(_t(xt[:, 0], gpu)); sb.p.copy_(_t(pt[:, 0], gpu)); sb.M.copy_(_t(Mt[:, 0], gpu))
+    sb.A.copy_(_t(At[:, 0], gpu)); sb.omega.copy_(_t(om, gpu))
+    pts = rng.random((777, d)) * L
+    ref = cpu.static_sum(xt[:, 0], pt[:, 0], Mt[:, 0], At[:, 0], om, pts) * 0.5
+    from beamax_amd.solvers.msgb_solver import MSGBSolver
+    from beamax_amd.gb import gb_solvers
+    for prec, tol in (("fp64", 1e-12), ("fp32_split", 1e-5)):
+        s = MSGBSolver(10, "top_n", 128, "spatial", gb_solvers.solve_hom_diag, "scan_real",
+                       precision=prec)
+        got = _n(s.sum_static(sb, pts, 0.5)).astype(np.float64)
+        assert rel(ref, got) < tol

@@ -1,0 +1,187 @@
+"""End-to-end GPU parity of MSGBSolver.forward / .adjoint against the oracle.
+
+Forward: the README configuration (128^2, 3 levels, 4096 beams, row-0 line
+sensor) -- FP64 GPU path vs the oracle to rel-L2 <= 1e-12 (same top_n index
+set, exact), FP32-split vs FP64 to <= 1e-5.
+Adjoint: single-face detectors in 2-D and 3-D, FP64 vs the oracle.
+"""
+
+import numpy as np
+import pytest
+
+import beamax_amd as bx
+from beamax_amd.gb import gb_solvers
+from oracle import adjoint as oadj
+from oracle import beams as obeams
+from oracle import cpu as ocpu
+from oracle import topn as otopn
+from oracle.decomposition import Tiling
+from oracle.transforms import MSWPT as OMSWPT
+
+pytestmark = pytest.mark.gpu
+
+
+def _n(x):
+    return x.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / (np.linalg.norm(a) + 1e-300))
+
+
+def readme_p0(decomp):
+    """Two-packet p0 of the reference README (README.md:54-77)."""
+    grid = decomp.fourier_meshgrid
+    hi = bx.transforms.compute_frames(decomp, 125, np.array([11, 6]), grid, 2, "none")
+    lo = bx.transforms.compute_frames(decomp, 44, np.array([11, 3]), grid, 2, "none")
+    p0 = bx.transforms.unitary_ifft_host(hi) + bx.transforms.unitary_ifft_host(lo)
+    p0 = p0 / np.max(np.abs(p0))
+    return np.ascontiguousarray(p0.T.real)
+
+
+def test_readme_forward_fp64_and_split(gpu):
+    N = (128, 128)
+    dom = bx.Domain(N=N, dx=(1e-4, 1e-4), c=1500.0, cfl=0.3, periodic=(False, False))
+    dd = bx.DyadicDecomposition(3, N, (4, 8, 16), (1, 1))
+    wpt = bx.MSWPT(dd, redundancy=2, windowing="rectangular_mirror")
+    p0 = readme_p0(dd)
+    ts = dom.generate_time_domain()
+    mask = np.zeros(N)
+    mask[0, :] = 1
+    sensors = bx.Sensor(domain=dom, binary_mask=mask)
+    outs = {}
+    for prec in ("fp64", "fp32_split"):
+        s = bx.MSGBSolver(thr=4096, thr_strat="top_n", batch_size=128, input_type="spatial",
+                          ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real",
+                          precision=prec)
+        data, params = s.forward_with_params(p0, dom, sensors, ts, wpt)
+        outs[prec] = _n(data).astype(np.float64)
+    assert outs["fp64"].shape == (604, 128)
+    # oracle: same pipeline in NumPy + C++ sum
+    ow = OMSWPT(Tiling(3, N, (4, 8, 16), (1, 1)), 2, "rectangular_mirror")
+    L = dom.grid_size
+    cpos = obeams.compute_coefficients_pos_only(ow, p0, 1500.0, L)
+    idx, val = otopn.top_n(cpos, 4096)
+    p, al, x0, om, a0, mode = obeams.forward_parameters(ow, idx, L, N)
+    dup = lambda v: np.concatenate([v, v])
+    ref = ocpu.forward_sum(dup(x0), dup(p), dup(al), dup(a0 * val), dup(om),
+                           np.concatenate([mode, -mode]), 1500.0, ts, sensors.positions)
+    assert rel(ref, outs["fp64"]) < 1e-12
+    assert rel(outs["fp64"], outs["fp32_split"]) < 1e-5
+    # reference's diagnostic tuple: (p0s, M0s, x0s, omegas, a0s, modes), batched by 128
+    assert params[0].shape == (64, 128, 2) and params[5].shape == (64, 128)
+
+
+def _line_data(rng, nt, ns, dt):
+    t = np.arange(nt) * dt
+    data = np.zeros((nt, ns))
+    for _ in range(6):
+        t0 = rng.uniform(0.3, 0.7) * t[-1]
+        f = rng.uniform(0.05, 0.3) / dt
+        w = rng.uniform(0.02, 0.08) * t[-1]
+        x0 = rng.uniform(0, ns)
+        sx = rng.uniform(3, 10)
+        data += (np.exp(-((t[:, None] - t0) / w) ** 2) * np.cos(2 * np.pi * f * (t[:, None] - t0))
+                 * np.exp(-((np.arange(ns)[None, :] - x0) / sx) ** 2))
+    return data
+
+
+@pytest.mark.parametrize("face", ["row0", "rowN"])
+def test_adjoint_2d_face_matches_oracle(gpu, face):
+    import torch
+    rng = np.random.default_rng(11)
+    n = 64
+    dom = bx.Domain(N=(n, n), dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    ts_full = dom.generate_time_domain()
+    nt = 2 * n
+    ts = np.linspace(ts_full[0], ts_full[-1], nt)
+    dt = ts[1] - ts[0]
+    row = 0 if face == "row0" else n - 1
+    mask = np.zeros((n, n))
+    mask[row, :] = 1
+    sources = bx.Sensor(dom, binary_mask=mask)
+    ddom = bx.Domain(N=(nt, n), dx=(dt, 1e-4), c=1500.0, periodic=(False, False))
+    dd = bx.DyadicDecomposition(2, (nt, n), (4, 8), (2, 1))
+    dw = bx.MSWPT(dd, 2, "rectangular_mirror")
+    data = _line_data(rng, nt, n, dt)
+    img = bx.Sensor(dom, binary_mask=np.ones((n, n)))
+    s = bx.MSGBSolver(thr=3000, thr_strat="top_n", batch_size=64, input_type="spatial",
+                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64")
+    q, beams = s.adjoint_with_params(torch.as_tensor(data, device=gpu), dom, img, sources, ts, ddom, dw)
+    ow = OMSWPT(Tiling(2, (nt, n), (4, 8), (2, 1)), 2, "rectangular_mirror")
+    qref, (idx, val, x, p, M, A, om) = oadj.adjoint_pipeline(
+        data, dt, ow, (nt, n), (dt, 1e-4), sources.positions, (n, n), (1e-4, 1e-4), 1500.0,
+        img.positions, 3000)
+    assert rel(x, _n(beams.x)) < 1e-13
+    assert rel(M, _n(beams.M)) < 1e-12
+    assert rel(A, _n(beams.A)) < 1e-12
+    assert rel(qref, _n(q).ravel()) < 1e-11
+    s32 = bx.MSGBSolver(thr=3000, thr_strat="top_n", batch_size=64, input_type="spatial",
+                        ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real")
+    q32 = s32.adjoint(torch.as_tensor(data, device=gpu), dom, img, sources, ts, ddom, dw)
+    assert rel(qref, _n(q32).ravel()) < 1e-5
+
+
+def test_adjoint_3d_face_matches_oracle(gpu):
+    import torch
+    rng = np.random.default_rng(12)
+    n = 32
+    dom = bx.Domain(N=(n, n, n), dx=(1e-4,) * 3, c=1500.0, periodic=(False,) * 3)
+    nt = 2 * n
+    ts = np.linspace(0, 3e-6, nt)
+    dt = ts[1] - ts[0]
+    mask = np.zeros((n, n, n))
+    mask[0] = 1
+    sources = bx.Sensor(dom, binary_mask=mask)
+    ddom = bx.Domain(N=(nt, n, n), dx=(dt, 1e-4, 1e-4), c=1500.0, periodic=(False,) * 3)
+    dd = bx.DyadicDecomposition(2, (nt, n, n), (4, 8), (2, 1, 1))
+    dw = bx.MSWPT(dd, 2, "rectangular_mirror")
+    base = _line_data(rng, nt, n, dt)
+    data = base[:, :, None] * np.exp(-((np.arange(n) - 16) / 6.0) ** 2)[None, None, :]
+    pts = rng.random((2000, 3)) * n * 1e-4
+    s = bx.MSGBSolver(thr=3000, thr_strat="top_n", batch_size=64, input_type="spatial",
+                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64")
+    q, beams = s.adjoint_with_params(torch.as_tensor(data, device=gpu), dom, pts, sources, ts,
+                                     ddom, dw)
+    ow = OMSWPT(Tiling(2, (nt, n, n), (4, 8), (2, 1, 1)), 2, "rectangular_mirror")
+    qref, (idx, val, x, p, M, A, om) = oadj.adjoint_pipeline(
+        data, dt, ow, (nt, n, n), (dt, 1e-4, 1e-4), sources.positions, (n, n, n), (1e-4,) * 3,
+        1500.0, pts, 3000)
+    assert rel(M, _n(beams.M)) < 1e-12
+    assert rel(A, _n(beams.A)) < 1e-12
+    assert rel(qref, _n(q)) < 1e-11
+
+
+def test_adjoint_source_matches_oracle(gpu):
+    import torch
+    rng = np.random.default_rng(3)
+    for nt in (128, 1207):
+        data = rng.standard_normal((nt, 9))
+        win = np.hanning(nt)
+        s = bx.MSGBSolver(thr=10, thr_strat="top_n", batch_size=64, input_type="spatial",
+                          ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real")
+        got = _n(s.adjoint_source(torch.as_tensor(data, device=gpu), 1e-8, 1500.0, win))
+        ref = oadj.adjoint_source(data, 1e-8, np.full(9, 1500.0), win)
+        assert rel(ref, got) < 1e-12
+
+
+def test_fourier_crop_matches_reference_recipe(gpu):
+    import torch
+    from beamax_amd.solvers.faces import FourEdgeAdjoint, four_edge_faces
+    rng = np.random.default_rng(5)
+    N = (32, 32)
+    faces = four_edge_faces(N)
+    S = 2 * 32 + 2 * 30
+    nt = 151
+    data = rng.standard_normal((nt, S))
+    dom = bx.Domain(N=N, dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    ts = np.arange(nt) * 1e-8
+    fa = FourEdgeAdjoint(bx.MSGBSolver(10, "top_n", 64, "spatial", gb_solvers.solve_hom_diag,
+                                       "scan_real"), dom, ts, 64)
+    for f in fa.faces:
+        got = _n(fa.crop(torch.as_tensor(data, device=gpu), f))
+        sub = data[:, f["cols"]] * f["weights"][None, :]
+        F = np.fft.fftshift(np.fft.fftn(sub, norm="ortho"))
+        mid = nt // 2
+        ref = np.fft.ifftn(np.fft.ifftshift(F[mid - 32: mid + 32]), norm="ortho").real
+        assert rel(ref, got) < 1e-12

@@ -1,0 +1,326 @@
+"""CPU pins of the FP64 oracle against the reference's own known answers,
+invariants and closed forms (no GPU).  Each test names the reference test or
+source it restates.
+"""
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+from oracle import adjoint as oadj
+from oracle import beams as obeams
+from oracle import cpu as ocpu
+from oracle import topn as otopn
+from oracle.decomposition import Tiling
+from oracle.transforms import (MSWPT, compute_frames, filter_idx, frame_phase, unitary_fft,
+                               unitary_ifft)
+
+
+# ---------------------------------------------------------------------------
+# tiling (tests/test_decomposition.py)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("N,lengths,retained", [((2048,), (32, 64, 128), (4, 6, 12)),
+                                                ((256, 256), (4, 8, 16), (16, 60, 240))])
+def test_locked_thesis_tiling(N, lengths, retained):
+    """test_decomposition.py:113-130."""
+    t = Tiling(3, N, (4, 8, 16), (1,) * len(N))
+    assert tuple(t.box_lengths) == lengths
+    assert tuple(int(x) for x in t.num_boxes) == retained
+
+
+@pytest.mark.parametrize("cfg", [(3, (128, 128)), (2, (64, 32, 32))])
+def test_centres_have_mirror_pairs(cfg):
+    """test_decomposition.py:184-206: each level is (-xi)-symmetric, the
+    second half of a level mirrors the first half in reverse order."""
+    lv, N = cfg
+    asp = (1,) * len(N) if len(set(N)) == 1 else (2, 1, 1)
+    t = Tiling(lv, N, (4, 8, 16)[:lv], asp)
+    for l in range(lv):
+        c = t.centres[t.boxes_cumsum[l]:t.boxes_cumsum[l + 1]]
+        assert np.array_equal(c, -c[::-1])
+        assert not np.any(np.all(c == 0, axis=1))
+
+
+def test_doubled_domain_doubles_centres():
+    """test_decomposition.py:209-223."""
+    a = Tiling(2, (128, 128), (4, 8), (1, 1))
+    b = Tiling(2, (256, 256), (4, 8), (1, 1))
+    assert np.array_equal(2 * a.centres, b.centres)
+
+
+# ---------------------------------------------------------------------------
+# filters and MSWPT (tests/test_ms_filters.py, test_transforms.py, test_frames.py)
+# ---------------------------------------------------------------------------
+CFGS = [(3, (128, 128), (4, 8, 16), (1, 1)), (2, (64, 32), (4, 8), (2, 1)),
+        (2, (32, 32, 32), (4, 8), (1, 1, 1)), (1, (64,), (8,), (1,))]
+
+
+@pytest.mark.parametrize("cfg", CFGS)
+def test_filters_bounded_and_dual_partition_of_unity(cfg):
+    """test_ms_filters.py:62-92: 0 <= g <= 1 and sum g*h = 1, h = g/sum g^2."""
+    w = MSWPT(Tiling(*cfg), 2, "rectangular_mirror")
+    tot = np.zeros(cfg[1])
+    for b in range(w.tiling.total_num_boxes):
+        g = filter_idx(b, w.tiling, 2, "rectangular_mirror")
+        assert g.min() >= 0 and g.max() <= 1
+        tot += g * (g / w.sum_gsquare)
+    assert np.max(np.abs(tot - 1)) < 1e-13
+
+
+@pytest.mark.parametrize("cfg", CFGS)
+@pytest.mark.parametrize("windowing", ["rectangular", "rectangular_mirror"])
+def test_inverse_of_forward_is_identity(cfg, windowing):
+    """test_transforms.py:107-134."""
+    w = MSWPT(Tiling(*cfg), 2, windowing)
+    x = np.random.default_rng(0).standard_normal(cfg[1])
+    c = w.forward(x)
+    assert np.max(np.abs(w.inverse(c).real - x)) < 1e-13
+    c2 = w.forward(w.inverse(c).real)
+    assert np.max(np.abs(w.inverse(c2) - w.inverse(c))) < 1e-13
+
+
+def test_forward_is_linear():
+    """test_transforms.py:173-216."""
+    w = MSWPT(Tiling(*CFGS[1]), 2, "rectangular_mirror")
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal((2,) + CFGS[1][1])
+    assert np.max(np.abs(w.forward(2 * x - 3 * y) - (2 * w.forward(x) - 3 * w.forward(y)))) < 1e-13
+
+
+@pytest.mark.parametrize("cfg", CFGS[:3])
+def test_single_coefficient_is_the_frame_atom(cfg):
+    """test_frames.py:359-449: inverse(e_j) in the Fourier domain == compute_frames."""
+    t = Tiling(*cfg)
+    w = MSWPT(t, 2, "rectangular_mirror")
+    rng = np.random.default_rng(2)
+    for j in rng.choice(w.total_coeffs, 4, replace=False):
+        e = np.zeros(w.total_coeffs, complex)
+        e[j] = 1.0
+        lvl, multi = obeams.decode_indices(w, [j])
+        box = multi[0, 0] + t.boxes_cumsum[lvl[0]]
+        atom = compute_frames(t, box, multi[1:, 0], 2, "rectangular_mirror")
+        assert np.max(np.abs(atom - w.inverse(e, "fourier"))) < 1e-13 * np.max(np.abs(atom))
+
+
+def test_isotropic_rho2_frame_phase_is_one():
+    """test_frames.py:452-467."""
+    t = Tiling(2, (64, 64), (4, 8), (1, 1))
+    w = MSWPT(t, 2, "rectangular")
+    lvl, multi = obeams.decode_indices(w, np.arange(w.total_coeffs))
+    boxes = multi[0] + t.boxes_cumsum[lvl]
+    assert np.array_equal(frame_phase(t, boxes, multi[1:].T, 2), np.ones(w.total_coeffs))
+
+
+@pytest.mark.parametrize("cfg", [(2, (128, 128), (4, 8), (1, 1)), (2, (64, 64, 64), (4, 8), (1, 1, 1))])
+def test_unwindowed_frame_is_gaussian_beam_at_t0(cfg):
+    """test_frames.py:242-353: ifft of an unwindowed frame == complex Gaussian
+    beam at t = 0 built from the forward beam parameters (periodic wrap)."""
+    lv, N, nb, asp = cfg
+    t = Tiling(lv, N, nb, asp)
+    w = MSWPT(t, 2, "rectangular")
+    d = len(N)
+    dx = 1e-4
+    L = np.asarray(N) * dx
+    grid = np.stack(np.meshgrid(*[np.arange(n) * dx for n in N], indexing="ij"), -1).reshape(-1, d)
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        j = int(rng.integers(w.coeffs_cumsum[1], w.total_coeffs))
+        lvl, multi = obeams.decode_indices(w, [j])
+        box = multi[0, 0] + t.boxes_cumsum[lvl[0]]
+        phi = unitary_ifft(compute_frames(t, box, multi[1:, 0], 2, "none"))
+        p, al, x0, om, a0, _ = obeams.forward_parameters(w, [j], L, N)
+        delta = grid - x0[0]
+        delta = delta - L * np.round(delta / L)
+        ph = delta @ p[0] + 0.5 * np.einsum("si,i,si->s", delta, al[0], delta)
+        gb = a0[0] * np.exp(1j * om[0] * ph)
+        assert np.max(np.abs(gb.reshape(N) - phi)) < 1e-13 * max(1.0, np.max(np.abs(phi)))
+
+
+# ---------------------------------------------------------------------------
+# closed-form propagation vs the Riccati ODE (tests/test_gb.py:155-299)
+# ---------------------------------------------------------------------------
+def _riccati_rhs(d, c0, p):
+    np_ = np.linalg.norm(p)
+    P = np.eye(d) - np.outer(p, p) / np_ ** 2
+
+    def f(t, y):
+        M = (y[:d * d] + 1j * y[d * d:2 * d * d]).reshape(d, d)
+        A = y[2 * d * d] + 1j * y[2 * d * d + 1]
+        dM = -(c0 / np_) * M @ P @ M
+        dA = -0.5 * A * (c0 / np_) * np.trace(P @ M)
+        return np.concatenate([dM.real.ravel(), dM.imag.ravel(), [dA.real, dA.imag]])
+    return f
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_hom_diag_matches_riccati_ode(d):
+    rng = np.random.default_rng(d)
+    for mode in (1.0, -1.0):
+        xi = rng.standard_normal(d) * 3000
+        om = np.linalg.norm(xi)
+        p = 2 * np.pi * xi / om
+        alpha = 1j * (rng.random(d) + 0.3) * 2 * (np.pi * 800) ** 2 / om
+        a0 = 0.7 - 0.2j
+        ts = np.linspace(0, 2e-5, 9)
+        xt, pt, Mt, At = obeams.solve_hom_diag(np.zeros((1, d)), p[None], alpha[None],
+                                               np.array([a0]), np.array([mode]), ts, 1500.0)
+        y0 = np.concatenate([np.diag(alpha).real.ravel(), np.diag(alpha).imag.ravel(),
+                             [a0.real, a0.imag]])
+        sol = solve_ivp(_riccati_rhs(d, 1500.0 * mode, p), (0, ts[-1]), y0, t_eval=ts,
+                        method="DOP853", rtol=1e-12, atol=1e-14)
+        M_ode = (sol.y[:d * d] + 1j * sol.y[d * d:2 * d * d]).T.reshape(-1, d, d)
+        A_ode = sol.y[2 * d * d] + 1j * sol.y[2 * d * d + 1]
+        assert np.max(np.abs(M_ode - Mt[0])) / np.max(np.abs(M_ode)) < 1e-9
+        assert np.max(np.abs(A_ode - At[0])) < 1e-9
+        assert np.allclose(xt[0], (1500.0 * mode) * p / np.linalg.norm(p) * ts[:, None], atol=1e-15)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_hom_TR_matches_ode_and_reverses_forward(d):
+    """test_gb.py:173-299: TR closed form == ODE from t_start to 0, and
+    forward-then-TR returns the initial state."""
+    rng = np.random.default_rng(10 + d)
+    xi = rng.standard_normal(d) * 3000
+    om = np.linalg.norm(xi)
+    p = 2 * np.pi * xi / om
+    alpha = 1j * (rng.random(d) + 0.3) * 2 * (np.pi * 800) ** 2 / om
+    a0 = 0.4 + 0.9j
+    T = 1.5e-5
+    xt, pt, Mt, At = obeams.solve_hom_diag(np.zeros((1, d)), p[None], alpha[None], np.array([a0]),
+                                           np.array([1.0]), np.array([0.0, T]), 1500.0)
+    x, _, M, A = obeams.solve_hom_TR(xt[:, 1], pt[:, 1], Mt[:, 1], At[:, 1], np.array([1.0]),
+                                     np.array([T]), 1500.0)
+    assert np.max(np.abs(M[0] - np.diag(alpha))) / np.max(np.abs(alpha)) < 1e-12
+    assert abs(A[0] - a0) < 1e-12
+    assert np.max(np.abs(x[0])) < 1e-15
+    y0 = np.concatenate([Mt[0, 1].real.ravel(), Mt[0, 1].imag.ravel(), [At[0, 1].real, At[0, 1].imag]])
+    sol = solve_ivp(_riccati_rhs(d, 1500.0, p), (T, 0.0), y0, method="DOP853", rtol=1e-12, atol=1e-14)
+    y = sol.y[:, -1]
+    assert np.max(np.abs((y[:d * d] + 1j * y[d * d:2 * d * d]).reshape(d, d) - M[0])) / np.max(np.abs(M[0])) < 1e-9
+
+
+def test_real_sum_is_gb_plus_conjugate():
+    """test_gb.py:507-590: the real beam sum equals GB + conj(GB)."""
+    rng = np.random.default_rng(4)
+    d, B = 2, 5
+    xi = rng.standard_normal((B, d)) * 2000
+    om = np.linalg.norm(xi, axis=1)
+    p = 2 * np.pi * xi / om[:, None]
+    alpha = 1j * (rng.random((B, d)) + 0.3) * 2 * (np.pi * 500) ** 2 / om[:, None]
+    a0 = rng.standard_normal(B) + 1j * rng.standard_normal(B)
+    x0 = rng.random((B, d)) * 0.005
+    ts = np.linspace(0, 3e-6, 7)
+    sens = rng.random((40, d)) * 0.005
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p, alpha, a0, np.ones(B), ts, 1500.0)
+    real = obeams.real_beam_sum(xt, pt, Mt, At, om, sens)
+    gb = obeams.complex_beam(xt, pt, Mt, At, om, sens).sum(0)
+    assert np.max(np.abs(real - 2 * gb.real)) < 1e-12 * np.max(np.abs(real))
+    bf = obeams.real_beam_sum_bruteforce(xt, pt, Mt, At, om, sens)
+    assert np.max(np.abs(real - bf)) < 1e-13
+
+
+def test_cpp_port_matches_numpy():
+    rng = np.random.default_rng(5)
+    d, B = 2, 40
+    xi = rng.standard_normal((B, d)) * 2000
+    om = np.linalg.norm(xi, axis=1)
+    p = 2 * np.pi * xi / om[:, None]
+    alpha = 1j * (rng.random((B, d)) + 0.3) * 2 * (np.pi * 500) ** 2 / om[:, None]
+    a0 = rng.standard_normal(B) + 1j * rng.standard_normal(B)
+    x0 = rng.random((B, d)) * 0.005
+    mode = np.where(rng.random(B) < 0.5, 1.0, -1.0)
+    ts = np.linspace(0, 3e-6, 11)
+    sens = rng.random((33, d)) * 0.005
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p, alpha, a0, mode, ts, 1500.0)
+    ref = obeams.real_beam_sum(xt, pt, Mt, At, om, sens)
+    got = ocpu.forward_sum(x0, p, alpha, a0, om, mode, 1500.0, ts, sens, nthreads=3)
+    assert np.max(np.abs(ref - got)) < 1e-13 * np.max(np.abs(ref))
+    st = ocpu.static_sum(xt[:, 2], pt[:, 2], Mt[:, 2], At[:, 2], om, sens)
+    assert np.max(np.abs(st - ref[2])) < 1e-13 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------------------
+# top_n (forward_solver_utils.py:65-89; test_msgb_solver_misc.py:352-389, 515-558)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,k", [(1, 1), (7, 7), (50, 3), (300, 100), (1000, 999)])
+def test_topn_forms_agree_with_bruteforce(n, k):
+    rng = np.random.default_rng(n)
+    c = (rng.integers(-2, 3, n) + 1j * rng.integers(-2, 3, n)) * 0.5
+    a, b = otopn.top_n(c, k), otopn.top_n_argpartition(c, k)
+    assert np.array_equal(a[0], b[0])
+    mag = otopn.magnitude(c)
+    order = sorted(range(n), key=lambda i: (mag[i], i))
+    assert list(a[0]) == order[n - k:]
+    assert np.all(np.diff(mag[a[0]]) >= 0)
+    assert len(set(a[0].tolist())) == k
+
+
+def test_topn_half_frame_cap_selects_no_masked_zero():
+    t = Tiling(2, (64, 64), (4, 8), (1, 1))
+    w = MSWPT(t, 2, "rectangular_mirror")
+    x = np.random.default_rng(6).standard_normal((64, 64))
+    cpos = obeams.compute_coefficients_pos_only(w, x, 1500.0, [6.4e-3, 6.4e-3])
+    cap = int(np.sum(np.diff(w.coeffs_cumsum) // 2))
+    idx, val = otopn.top_n(cpos, 10 ** 9 if False else cap)
+    assert np.all(w.half_mask[idx] == 1)
+
+
+# ---------------------------------------------------------------------------
+# adjoint (msgb_solver.py:68-127; adjoint_solver_utils.py; tr_solver_utils.py)
+# ---------------------------------------------------------------------------
+def test_adjoint_source_spectral_derivative_closed_form():
+    """test_msgb_solver_misc.py:452-512: -c^2 d/dt of an on-grid sinusoid."""
+    nt, dt = 256, 1e-7
+    t = np.arange(nt) * dt
+    f0 = 7 / (nt * dt)
+    data = np.stack([np.sin(2 * np.pi * f0 * t), np.cos(2 * np.pi * 3 * f0 * t)], 1)
+    src = oadj.adjoint_source(data, dt, np.array([1500.0, 2.0]))
+    ref = np.stack([-1500.0 ** 2 * 2 * np.pi * f0 * np.cos(2 * np.pi * f0 * t),
+                    -4.0 * (-2 * np.pi * 3 * f0) * np.sin(2 * np.pi * 3 * f0 * t)], 1)
+    assert np.max(np.abs(src - ref)) < 1e-9 * np.max(np.abs(ref))
+
+
+def test_principal_b_inverse_known_answers():
+    """test_adj_solver_utils.py:67-107."""
+    b = oadj.principal_b_inverse(np.array([3.0, -3.0]), np.zeros((2, 1)), np.array([2.0, 2.0]), 1e-6)
+    assert np.allclose(b[0], -1j / (4 * np.pi * 2.0 * 3.0))
+    assert np.allclose(b[1], np.conj(b[0]))
+    r = 5e-7
+    b = oadj.principal_b_inverse(np.array([1.0, 2.0, 1.0, 0.0]),
+                                 np.array([[1.0], [1.0], [np.sqrt(1 - r * r)], [0.0]]),
+                                 np.array([2.0, 2.0, 1.0, 1.0]), 1e-6)
+    assert np.allclose(b, 0.0)
+    tau = np.array([1.0, 1e6])
+    b = oadj.principal_b_inverse(tau, (tau * np.sqrt(1 - 1e-6))[:, None], np.ones(2), 1e-6)
+    assert np.all(np.abs(b) > 0) and np.allclose(b[0] / b[1], tau[1] / tau[0], rtol=1e-6)
+
+
+def _small_tr_case(row):
+    t = Tiling(1, (16, 16), (2,), (1, 1))
+    w = MSWPT(t, 2, "rectangular")
+    pos = np.stack([np.full(16, float(row)), np.arange(16.0)], 1)
+    return w, pos
+
+
+@pytest.mark.parametrize("row,sign", [(0, 1.0), (15, -1.0)])
+def test_tr_beams_point_inward(row, sign):
+    """test_tr_solver.py:113-133."""
+    w, pos = _small_tr_case(row)
+    pts, *_ = oadj.tr_parameters(w, np.arange(64), (16, 16), (1.0, 1.0), pos, (16, 16), (1.0, 1.0), 0.3)
+    n = pts[:, 0]
+    fin = np.abs(n) > 1e-8
+    assert np.any(fin) and np.all(sign * n[fin] >= 0)
+
+
+def test_tr_temporal_carrier_sets_tangential_phase():
+    """test_tr_solver.py:168-203: omega = |tau| and omega * p_tan = 2 pi k_tan."""
+    w, pos = _small_tr_case(0)
+    idx = np.arange(64)
+    pts, _, _, om, *_rest = oadj.tr_parameters(w, idx, (16, 16), (1.0, 1.0), pos, (16, 16), (1.0, 1.0), 0.3)
+    lvl, multi = obeams.decode_indices(w, idx)
+    ch = w.tiling.centres[multi[0] + w.tiling.boxes_cumsum[lvl]] / 16.0
+    ok = (np.abs(ch[:, 0]) > 1e-8) & (np.abs(ch[:, 1]) > 1e-8) & (np.abs(pts[:, 0]) != 0)
+    grazing_free = ok & (np.abs(pts[:, 0] - 1.0) > 0)
+    assert np.any(grazing_free)
+    assert np.allclose(om[grazing_free], np.abs(ch[grazing_free, 0]))
+    assert np.allclose(om[grazing_free] * pts[grazing_free, 1], 2 * np.pi * ch[grazing_free, 1])
