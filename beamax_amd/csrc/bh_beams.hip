@@ -69,7 +69,7 @@ __device__ inline cd frame_phase(const TilingDev& T, const int32_t* centre, int 
 }
 
 // forward_solver_utils.py:255-373 (compute_params, sign +1) times the coefficient
-__global__ void k_forward_params(TilingDev T, const int32_t* __restrict__ centres, double L0,
+__global__ __launch_bounds__(128) void k_forward_params(TilingDev T, const int32_t* __restrict__ centres, double L0,
                                  double L1, double L2, const long long* __restrict__ idx,
                                  const double2* __restrict__ vals, long long k,
                                  bh_fwd_beams out) {
@@ -77,7 +77,21 @@ __global__ void k_forward_params(TilingDev T, const int32_t* __restrict__ centre
   if (i >= k) return;
   const int d = T.ndim;
   const double L[3] = {L0, L1, L2};
-  const Decoded q = decode_flat(T, idx[i]);
+  const long long flat = idx[i];
+  if (flat < 0 || flat >= T.coeffs_cumsum[T.num_levels]) {
+    // index outside the coefficient vector: an inert beam (zero amplitude)
+    for (int a = 0; a < d; ++a) {
+      out.p0[i * d + a] = a == 0 ? bh::kTwoPi : 0.0;
+      out.x0[i * d + a] = 0.0;
+      out.alpha[2 * (i * d + a)] = 0.0;
+      out.alpha[2 * (i * d + a) + 1] = 1.0;
+    }
+    out.a0[2 * i] = 0.0;
+    out.a0[2 * i + 1] = 0.0;
+    out.omega[i] = 1.0;
+    return;
+  }
+  const Decoded q = decode_flat(T, flat);
   const int32_t* c = centres + q.box * d;
   double ch[3], n2 = 0.0;
   for (int a = 0; a < d; ++a) { ch[a] = (double)c[a] / L[a]; n2 += ch[a] * ch[a]; }
@@ -435,7 +449,7 @@ __global__ __launch_bounds__(256) void k_forward_sum(FwdSumArgs A) {
 
 // materialised trajectories (SolverFn): one thread per (beam, time)
 template <int D>
-__global__ void k_hom_diag(long long nb, const double* x0, const double* p0, const double* alpha,
+__global__ __launch_bounds__(128) void k_hom_diag(long long nb, const double* x0, const double* p0, const double* alpha,
                            const double* a0, const double* mode, double c, const double* ts,
                            long long nt, double* xt, double* pt, double* Mt, double* At) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -511,11 +525,27 @@ struct AdjArgs {
 };
 
 template <int D>
-__global__ void k_adjoint_params(AdjArgs A) {
+__global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= A.k) return;
   const TilingDev& T = A.T;
-  const Decoded q = decode_flat(T, A.idx[i]);
+  const long long flat = A.idx[i];
+  if (flat < 0 || flat >= T.coeffs_cumsum[T.num_levels]) {
+    // index outside the coefficient vector: an inert beam (zero amplitude)
+    for (int a = 0; a < D; ++a) {
+      A.out.x[i * D + a] = 0.0;
+      A.out.p[i * D + a] = 1.0;
+      for (int b = 0; b < D; ++b) {
+        A.out.M[2 * ((i * D + a) * D + b)] = 0.0;
+        A.out.M[2 * ((i * D + a) * D + b) + 1] = a == b ? 1.0 : 0.0;
+      }
+    }
+    A.out.A[2 * i] = 0.0;
+    A.out.A[2 * i + 1] = 0.0;
+    A.out.omega[i] = 1.0;
+    return;
+  }
+  const Decoded q = decode_flat(T, flat);
   const int32_t* cen = A.centres + q.box * D;
   double ch[D], bl[D], Ll[D], sig[D];
   for (int a = 0; a < D; ++a) {

@@ -395,6 +395,7 @@ class MSGBSolver:
                   _lib.dbl_array(data_domain.grid_size), _lib.ptr(idx), _lib.ptr(val), idx.numel(),
                   normal_axis, normal_value, inward, float(c), self.adjoint_relative_guard,
                   int(swap_axes), beams.struct(), _lib.stream_handle())
+        beams.idx, beams.val, beams.coeffs = idx, val, coeffs   # diagnostics
         return beams
 
     def sum_static(self, beams: StaticBeams, points, scale: float, out=None):

@@ -169,8 +169,9 @@ def test_forward_params_match_oracle(gpu, cfg):
     idx = np.sort(rng.choice(w.total_coeffs, 300, replace=False)).astype(np.int64)
     vals = rng.standard_normal(300) + 1j * rng.standard_normal(300)
     fb = ForwardBeams(300, len(N), gpu)
-    _lib.call("bh_forward_params", C.byref(w.tiling), _lib.dbl_array(L), _lib.ptr(_t(idx, gpu)),
-              _lib.ptr(_t(vals, gpu)), 300, fb.struct(), _lib.stream_handle())
+    idx_d, vals_d = _t(idx, gpu), _t(vals, gpu)   # keep alive across the launch
+    _lib.call("bh_forward_params", C.byref(w.tiling), _lib.dbl_array(L), _lib.ptr(idx_d),
+              _lib.ptr(vals_d), 300, fb.struct(), _lib.stream_handle())
     p, al, x0, om, a0, _ = obeams.forward_parameters(o, idx, L, N)
     assert rel(p, _n(fb.p0)) < 1e-14
     assert rel(al, _n(fb.alpha)) < 1e-14
@@ -231,10 +232,11 @@ def test_forward_sum_matches_oracle(gpu, d):
     dup = lambda v: np.concatenate([v, v])
     ref = cpu.forward_sum(dup(x0), dup(p0), dup(alpha), dup(a0), dup(om), mode, 1500.0, ts, sens)
     outs = {}
+    ts_d, sens_d = _t(ts, gpu), _t(sens, gpu)
     for prec, dt in (("fp64", torch.float64), ("fp32_split", torch.float32)):
         out = torch.empty((ts.size, 300), dtype=dt, device=gpu)
-        _lib.call("bh_forward_sum", C.byref(fb.struct()), d, 1, 1500.0, _lib.ptr(_t(ts, gpu)),
-                  ts.size, _lib.ptr(_t(sens, gpu)), 300, _lib.dbl_array([L] * d),
+        _lib.call("bh_forward_sum", C.byref(fb.struct()), d, 1, 1500.0, _lib.ptr(ts_d),
+                  ts.size, _lib.ptr(sens_d), 300, _lib.dbl_array([L] * d),
                   _lib.int32_array([0] * d), _lib.PRECISION[prec], _lib.ptr(out),
                   _lib.stream_handle())
         outs[prec] = _n(out).astype(np.float64)
@@ -255,8 +257,9 @@ def test_forward_sum_periodic_matches_oracle(gpu):
     ts = np.arange(0, 8e-6, 4e-7)
     sens = rng.random((50, d)) * L
     out = torch.empty((ts.size, 50), dtype=torch.float64, device=gpu)
-    _lib.call("bh_forward_sum", C.byref(fb.struct()), d, 0, 1500.0, _lib.ptr(_t(ts, gpu)), ts.size,
-              _lib.ptr(_t(sens, gpu)), 50, _lib.dbl_array([L, L]), _lib.int32_array([1, 1]),
+    ts_d, sens_d = _t(ts, gpu), _t(sens, gpu)
+    _lib.call("bh_forward_sum", C.byref(fb.struct()), d, 0, 1500.0, _lib.ptr(ts_d), ts.size,
+              _lib.ptr(sens_d), 50, _lib.dbl_array([L, L]), _lib.int32_array([1, 1]),
               0, _lib.ptr(out), _lib.stream_handle())
     xt, pt, Mt, At = obeams.solve_hom_diag(x0, p0, alpha, a0, np.ones(B), ts, 1500.0)
     xt = np.mod(xt, L)
