@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
+                    help="also run one plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -153,6 +155,12 @@ def main():
         sys.exit(2)
     import torch
     dev = torch.device("cuda", 0)
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe * 2 ** 20 // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.perf_counter()
     P = build_problem(dev)
     torch.cuda.synchronize()
