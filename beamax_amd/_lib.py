@@ -46,6 +46,13 @@ class StaticBeams(C.Structure):
                 ("M", C.c_void_p), ("A", C.c_void_p), ("omega", C.c_void_p)]
 
 
+class Segs(C.Structure):
+    _fields_ = [("nseg", C.c_int64), ("ngroups", C.c_int64), ("S", C.c_int64),
+                ("seg_p0", C.c_void_p), ("seg_v", C.c_void_p), ("seg_n", C.c_void_p),
+                ("seg_g0", C.c_void_p), ("grp_seg", C.c_void_p), ("pts", C.c_void_p),
+                ("col", C.c_void_p)]
+
+
 _P = C.c_void_p
 _I64 = C.c_int64
 _SZ = C.c_size_t
@@ -77,6 +84,10 @@ SIGNATURES = {
     "bh_adjoint_params": (C.c_int, [C.POINTER(Tiling), C.POINTER(C.c_double), _P, _P, _I64,
                                     C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_int, StaticBeams, _P]),
+    "bh_forward_sum_culled": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, C.c_double, _P,
+                                        _I64, C.POINTER(Segs), C.c_double, C.c_int, _P, _P, _P]),
+    "bh_static_sum_culled": (C.c_int, [C.POINTER(StaticBeams), C.c_int, C.POINTER(Segs),
+                                       C.c_double, C.c_double, C.c_int, _P, _P, _P, _SZ, _P]),
     "bh_static_sum_workspace_size": (_SZ, [_I64, C.c_int]),
     "bh_static_sum": (C.c_int, [C.POINTER(StaticBeams), C.c_int, _P, _I64, C.c_double, C.c_int,
                                 _P, _P, _SZ, _P]),

@@ -1,0 +1,245 @@
+// Beam state, evaluation records and the per-term evaluators shared by the
+// summation kernels (included by several translation units; everything is
+// internal to each).
+#pragma once
+
+#include "bh_internal.hpp"
+
+namespace {
+
+template <int D>
+struct Sym {
+  static constexpr int N = D * (D + 1) / 2;
+};
+
+
+// ---------------------------------------------------------------------------
+// closed-form propagation (gb_solvers.py:194-422)
+// ---------------------------------------------------------------------------
+template <int D>
+struct State {
+  double x[D];
+  double p[D];
+  cd M[D][D];
+  cd A;
+};
+
+template <int D>
+__device__ inline void hom_diag_state(const double* x0, const double* p0, const cd* alpha, cd a0,
+                                      double c0, double t, State<D>& S) {
+  double np2 = 0.0;
+  for (int a = 0; a < D; ++a) np2 += p0[a] * p0[a];
+  const double np = sqrt(np2);
+  for (int a = 0; a < D; ++a) {
+    S.x[a] = x0[a] + (c0 * (p0[a] / np)) * t;
+    S.p[a] = p0[a];
+  }
+  const double ct = c0 * t;
+  const double np3 = np * np * np;
+  cd Ai[D], Aiu[D], vAi[D];
+  cd scal = mkc(1.0);
+  for (int a = 0; a < D; ++a) {
+    const cd A = mkc(1.0) + (ct * alpha[a]) / np;
+    Ai[a] = mkc(1.0) / A;
+    const double u = (-ct * p0[a]) / np3;
+    Aiu[a] = Ai[a] * u;
+    vAi[a] = (p0[a] * alpha[a]) * Ai[a];
+    scal = scal + vAi[a] * u;
+  }
+  for (int a = 0; a < D; ++a)
+    for (int b = 0; b < D; ++b) {
+      cd y = -(Aiu[a] * vAi[b]) / scal;
+      if (a == b) y = Ai[a] + y;
+      S.M[a][b] = alpha[a] * y;
+    }
+  if constexpr (D == 3) {
+    const double n = np;
+    const cd a1 = alpha[0], a2 = alpha[1], a3 = alpha[2];
+    const double tsc = c0 * t / (n * n * n * n);
+    const cd t1 = (c0 * t) * (a2 * a3) + (a2 + a3) * n;
+    const cd t2 = (c0 * t) * (a1 * a3) + (a1 + a3) * n;
+    const cd t3 = (c0 * t) * (a1 * a2) + (a1 + a2) * n;
+    const cd den = mkc(1.0) + tsc * ((p0[0] * p0[0]) * t1 + (p0[1] * p0[1]) * t2 +
+                                     (p0[2] * p0[2]) * t3);
+    S.A = a0 / csqrt_(den);
+  } else if constexpr (D == 2) {
+    cd ap = mkc(0.0);
+    for (int a = 0; a < D; ++a) ap = ap + (p0[a] * p0[a]) * alpha[D - 1 - a];
+    ap = ap / np3;
+    S.A = a0 / csqrt_(mkc(1.0) + ct * ap);
+  } else {
+    S.A = a0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// evaluation records
+// ---------------------------------------------------------------------------
+template <int D>
+struct RecS {  // FP32-split
+  double x[D];
+  double k[D];
+  double qr[Sym<D>::N];
+  double ph0;
+  float qi[Sym<D>::N];
+  float amp;
+};
+
+template <int D>
+struct RecD {  // FP64
+  double x[D];
+  double p[D];
+  double mr[Sym<D>::N];
+  double mi[Sym<D>::N];
+  double w;
+  double amp;
+  double ang;
+};
+
+__device__ inline double safe_angle(cd A) {
+  const double re = (A.re == 0.0 && A.im == 0.0) ? 1e-12 : A.re;
+  return atan2(A.im, re);
+}
+
+template <int D>
+__device__ inline void make_rec(const State<D>& S, double w, RecS<D>& R) {
+  for (int a = 0; a < D; ++a) {
+    R.x[a] = S.x[a];
+    R.k[a] = w * S.p[a] / bh::kTwoPi;
+  }
+  const double sr = w / (2.0 * bh::kTwoPi);
+  const double si = 0.5 * w * bh::kLog2e;
+  int q = 0;
+  for (int a = 0; a < D; ++a) {
+    R.qr[q] = sr * S.M[a][a].re;
+    R.qi[q] = (float)(si * S.M[a][a].im);
+    ++q;
+  }
+  for (int a = 0; a < D; ++a)
+    for (int b = a + 1; b < D; ++b) {
+      R.qr[q] = sr * (S.M[a][b].re + S.M[b][a].re);
+      R.qi[q] = (float)(si * (S.M[a][b].im + S.M[b][a].im));
+      ++q;
+    }
+  R.amp = (float)(2.0 * cabs_(S.A));
+  R.ph0 = safe_angle(S.A) / bh::kTwoPi;
+}
+
+template <int D>
+__device__ inline void make_rec(const State<D>& S, double w, RecD<D>& R) {
+  for (int a = 0; a < D; ++a) {
+    R.x[a] = S.x[a];
+    R.p[a] = S.p[a];
+  }
+  int q = 0;
+  for (int a = 0; a < D; ++a) {
+    R.mr[q] = S.M[a][a].re;
+    R.mi[q] = S.M[a][a].im;
+    ++q;
+  }
+  for (int a = 0; a < D; ++a)
+    for (int b = a + 1; b < D; ++b) {
+      R.mr[q] = S.M[a][b].re + S.M[b][a].re;
+      R.mi[q] = S.M[a][b].im + S.M[b][a].im;
+      ++q;
+    }
+  R.w = w;
+  R.amp = 2.0 * cabs_(S.A);
+  R.ang = safe_angle(S.A);
+}
+
+template <int D>
+__device__ inline void zero_rec(RecS<D>& R) {
+  for (int a = 0; a < D; ++a) { R.x[a] = 0.0; R.k[a] = 0.0; }
+  for (int q = 0; q < Sym<D>::N; ++q) { R.qr[q] = 0.0; R.qi[q] = 0.f; }
+  R.ph0 = 0.0;
+  R.amp = 0.f;
+}
+template <int D>
+__device__ inline void zero_rec(RecD<D>& R) {
+  for (int a = 0; a < D; ++a) { R.x[a] = 0.0; R.p[a] = 0.0; }
+  for (int q = 0; q < Sym<D>::N; ++q) { R.mr[q] = 0.0; R.mi[q] = 0.0; }
+  R.w = 0.0;
+  R.amp = 0.0;
+  R.ang = 0.0;
+}
+
+// delta = s - x with the optional minimum-image wrap (gb/core.py:16-36, 290-294)
+template <int D, bool PER>
+__device__ inline void make_delta(const double* s, const double* x, const double* dom,
+                                  const int* per, double* dl) {
+  for (int a = 0; a < D; ++a) {
+    double dv = s[a] - x[a];
+    if (PER) {
+      if (per[a]) dv = dv - dom[a] * rint(dv / dom[a]);
+    }
+    dl[a] = dv;
+  }
+}
+
+template <int D>
+__device__ inline float eval_split(const RecS<D>& R, const double* dl) {
+  // cycles = sum_a dl_a (k_a + sum_b Q_ab dl_b) + ph0, Q symmetric
+  double cyc = R.ph0;
+  float ex;
+  if constexpr (D == 1) {
+    cyc = fma(dl[0], fma(R.qr[0], dl[0], R.k[0]), cyc);
+    const float f0 = (float)dl[0];
+    ex = f0 * (R.qi[0] * f0);
+  } else if constexpr (D == 2) {
+    const double w0 = fma(R.qr[0], dl[0], fma(R.qr[2], dl[1], R.k[0]));
+    const double w1 = fma(R.qr[1], dl[1], R.k[1]);
+    cyc = fma(dl[0], w0, cyc);
+    cyc = fma(dl[1], w1, cyc);
+    const float f0 = (float)dl[0], f1 = (float)dl[1];
+    const float e0 = fmaf(R.qi[0], f0, R.qi[2] * f1);
+    ex = fmaf(f0, e0, f1 * (R.qi[1] * f1));
+  } else {
+    // qr layout: 00 11 22 01 02 12
+    const double w0 = fma(R.qr[0], dl[0], fma(R.qr[3], dl[1], fma(R.qr[4], dl[2], R.k[0])));
+    const double w1 = fma(R.qr[1], dl[1], fma(R.qr[5], dl[2], R.k[1]));
+    const double w2 = fma(R.qr[2], dl[2], R.k[2]);
+    cyc = fma(dl[0], w0, cyc);
+    cyc = fma(dl[1], w1, cyc);
+    cyc = fma(dl[2], w2, cyc);
+    const float f0 = (float)dl[0], f1 = (float)dl[1], f2 = (float)dl[2];
+    const float e0 = fmaf(R.qi[0], f0, fmaf(R.qi[3], f1, R.qi[4] * f2));
+    const float e1 = fmaf(R.qi[1], f1, R.qi[5] * f2);
+    ex = fmaf(f0, e0, fmaf(f1, e1, f2 * (R.qi[2] * f2)));
+  }
+  const float fr = (float)(cyc - rint(cyc));
+  return R.amp * __builtin_amdgcn_cosf(fr) * __builtin_amdgcn_exp2f(-ex);
+}
+
+template <int D>
+__device__ inline double eval_f64(const RecD<D>& R, const double* dl) {
+  double lin = 0.0;
+  for (int a = 0; a < D; ++a) lin += R.p[a] * dl[a];
+  double q = 0.0, chi = 0.0;
+  int idx = 0;
+  for (int a = 0; a < D; ++a) {
+    q += R.mr[idx] * dl[a] * dl[a];
+    chi += R.mi[idx] * dl[a] * dl[a];
+    ++idx;
+  }
+  for (int a = 0; a < D; ++a)
+    for (int b = a + 1; b < D; ++b) {
+      q += R.mr[idx] * dl[a] * dl[b];
+      chi += R.mi[idx] * dl[a] * dl[b];
+      ++idx;
+    }
+  const double ph = R.w * (lin + 0.5 * q) + R.ang;
+  return R.amp * cos(ph) * exp(-R.w * (0.5 * chi));
+}
+
+template <int D, bool F64>
+struct RecOf {
+  using type = RecS<D>;
+};
+template <int D>
+struct RecOf<D, true> {
+  using type = RecD<D>;
+};
+
+
+}  // namespace

@@ -1,0 +1,385 @@
+// Culled real beam summations over segment-decomposed points.
+//
+// Workgroup = 4 waves; it owns GPB = 4*SPT groups of 64 points (wave w owns
+// local groups w, w+4, ...; lane l holds point l of each) and, in the forward
+// form, one time level.  Beams stream in chunks of 256: lane i of wave q
+// builds (forward: closed form at time t) or loads (static) the record of beam
+// chunk + 64q + i and solves, for every segment of the block, the index
+// interval on which the beam's envelope 2^-(delta^T Q delta) stays above
+// 2^-cull_log2 (Q = w log2(e) ImM / 2; along a segment the exponent is a
+// quadratic in the point index).  A wave ballot per group turns those
+// intervals into a 64-bit mask per (group, wave q); each wave then evaluates
+// exactly the active (record, group) pairs, in record order -- no atomics, so
+// the result is deterministic.  Every evaluated term is computed exactly as
+// in the dense kernels.
+#include "bh_beamcore.hpp"
+
+namespace {
+
+constexpr int kRecChunk = 256;
+
+struct SegDev {
+  long long nseg, ngroups, S;
+  const double* p0;
+  const double* v;
+  const int32_t* n;
+  const int32_t* g0;
+  const int32_t* grp_seg;
+  const double* pts;
+  const int32_t* col;
+};
+
+// log2-scaled envelope matrix (full, symmetric) from a record
+template <int D>
+__device__ inline void env_matrix(const RecS<D>& R, double (&Q)[D][D]) {
+  int q = 0;
+  for (int a = 0; a < D; ++a) Q[a][a] = (double)R.qi[q++];
+  for (int a = 0; a < D; ++a)
+    for (int b = a + 1; b < D; ++b) {
+      Q[a][b] = 0.5 * (double)R.qi[q];
+      Q[b][a] = Q[a][b];
+      ++q;
+    }
+}
+template <int D>
+__device__ inline void env_matrix(const RecD<D>& R, double (&Q)[D][D]) {
+  const double s = 0.5 * R.w * bh::kLog2e;
+  int q = 0;
+  for (int a = 0; a < D; ++a) Q[a][a] = s * R.mi[q++];
+  for (int a = 0; a < D; ++a)
+    for (int b = a + 1; b < D; ++b) {
+      Q[a][b] = 0.5 * s * R.mi[q];
+      Q[b][a] = Q[a][b];
+      ++q;
+    }
+}
+template <int D>
+__device__ inline bool rec_zero(const RecS<D>& R) { return !(R.amp > 0.f); }
+template <int D>
+__device__ inline bool rec_zero(const RecD<D>& R) { return !(R.amp > 0.0); }
+
+// index interval [lo, hi] of segment k where exponent <= E; empty -> lo > hi
+template <int D>
+__device__ inline void seg_interval(const SegDev& G, long long k, const double* x,
+                                    const double (&Q)[D][D], double E, int& lo, int& hi) {
+  const int n = G.n[k];
+  double d0[D], v[D], Qv[D];
+  for (int a = 0; a < D; ++a) {
+    d0[a] = G.p0[k * D + a] - x[a];
+    v[a] = G.v[k * D + a];
+  }
+  for (int a = 0; a < D; ++a) {
+    double s = 0.0;
+    for (int b = 0; b < D; ++b) s += Q[a][b] * v[b];
+    Qv[a] = s;
+  }
+  double A = 0.0, Bq = 0.0, C = 0.0;
+  for (int a = 0; a < D; ++a) {
+    A += v[a] * Qv[a];
+    Bq += d0[a] * Qv[a];
+    double s = 0.0;
+    for (int b = 0; b < D; ++b) s += Q[a][b] * d0[b];
+    C += d0[a] * s;
+  }
+  // exponent(u) = A u^2 + 2 Bq u + C on u in [0, n-1]
+  if (!(A > 1e-300)) {
+    if (C <= E || !(C == C)) { lo = 0; hi = n - 1; } else { lo = 1; hi = 0; }
+    return;
+  }
+  const double disc = Bq * Bq - A * (C - E);
+  if (!(disc >= 0.0)) {
+    if (disc == disc) { lo = 1; hi = 0; } else { lo = 0; hi = n - 1; }   // NaN: keep all
+    return;
+  }
+  const double r = sqrt(disc);
+  const double ulo = (-Bq - r) / A, uhi = (-Bq + r) / A;
+  const double flo = floor(ulo) - 1.0, fhi = ceil(uhi) + 1.0;
+  lo = flo < 0.0 ? 0 : (flo > (double)(n - 1) ? n : (int)flo);
+  hi = fhi > (double)(n - 1) ? n - 1 : (fhi < 0.0 ? -1 : (int)fhi);
+}
+
+struct CulledArgs {
+  // forward source
+  bh_fwd_beams b;
+  long long count, total;
+  double c;
+  const double* ts;
+  // static source
+  const void* recs;
+  // common
+  SegDev G;
+  double E;       // cull exponent (log2 units); <= 0: evaluate all pairs
+  double scale;
+  void* out;
+  unsigned long long* pairs;
+};
+
+template <int D, int SPT, bool F64, bool STATIC>
+__global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
+  using Rec = typename RecOf<D, F64>::type;
+  constexpr int GPB = 4 * SPT;
+  __shared__ Rec rec[kRecChunk];
+  __shared__ unsigned long long mask[GPB][4];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const long long gbase = (long long)blockIdx.x * GPB;
+  const long long t = STATIC ? 0 : (long long)blockIdx.y;
+  const double tv = STATIC ? 0.0 : A.ts[t];
+  const int ng = (int)((A.G.ngroups - gbase) < GPB ? (A.G.ngroups - gbase) : GPB);
+  // points of the owned groups
+  double pt[SPT][D];
+  int colv[SPT];
+  for (int i = 0; i < SPT; ++i) {
+    const int gl = wv + 4 * i;
+    colv[i] = -1;
+    for (int a = 0; a < D; ++a) pt[i][a] = 0.0;
+    if (gl < ng) {
+      const long long slot = (gbase + gl) * 64 + lane;
+      for (int a = 0; a < D; ++a) pt[i][a] = A.G.pts[slot * D + a];
+      colv[i] = A.G.col[slot];
+    }
+  }
+  double acc[SPT];
+  for (int i = 0; i < SPT; ++i) acc[i] = 0.0;
+  unsigned long long npairs = 0;
+  const long long total = STATIC ? A.count : A.total;
+  for (long long cb = 0; cb < total; cb += kRecChunk) {
+    __syncthreads();   // previous chunk's records and masks fully consumed
+    const long long bi = cb + threadIdx.x;
+    Rec& R = rec[threadIdx.x];
+    bool live = bi < total;
+    if (live) {
+      if constexpr (STATIC) {
+        R = ((const Rec*)A.recs)[bi];
+      } else {
+        const bool neg = bi >= A.count;
+        const long long j = neg ? bi - A.count : bi;
+        double x0[D], p0[D];
+        cd al[D];
+        for (int a = 0; a < D; ++a) {
+          x0[a] = A.b.x0[j * D + a];
+          p0[a] = A.b.p0[j * D + a];
+          al[a] = mkc(A.b.alpha[2 * (j * D + a)], A.b.alpha[2 * (j * D + a) + 1]);
+        }
+        State<D> St;
+        hom_diag_state<D>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]), neg ? -A.c : A.c,
+                          tv, St);
+        make_rec<D>(St, A.b.omega[j], R);
+      }
+      live = !rec_zero<D>(R);   // zero amplitude contributes exactly 0
+    } else {
+      zero_rec<D>(R);
+    }
+    // per-group activity masks by ballot (word = this wave's 64 records)
+    double Q[D][D];
+    if (live && A.E > 0.0) env_matrix<D>(R, Q);
+    long long cur_seg = -1;
+    int lo = 1, hi = 0;
+    for (int gl = 0; gl < ng; ++gl) {
+      const long long g = gbase + gl;
+      bool act = live;
+      if (live && A.E > 0.0) {
+        const long long k = A.G.grp_seg[g];
+        if (k != cur_seg) {
+          seg_interval<D>(A.G, k, R.x, Q, A.E, lo, hi);
+          cur_seg = k;
+        }
+        const int j0 = (int)((g - A.G.g0[k]) * 64);
+        act = (hi >= lo) && (hi >= j0) && (lo <= j0 + 63);
+      }
+      const unsigned long long m = __ballot(act);
+      if (lane == 0) mask[gl][wv] = m;
+      npairs += __popcll(m);
+    }
+    __syncthreads();
+    for (int i = 0; i < SPT; ++i) {
+      const int gl = wv + 4 * i;
+      if (gl >= ng) continue;
+      if constexpr (F64) {
+        double part = 0.0;
+        for (int q = 0; q < 4; ++q) {
+          unsigned long long m = mask[gl][q];
+          unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)m);
+          unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
+          m = ((unsigned long long)hi32 << 32) | lo32;
+          while (m) {
+            const int r = __builtin_ctzll(m);
+            m &= m - 1;
+            const Rec Rr = rec[q * 64 + r];
+            double dl[D];
+            for (int a = 0; a < D; ++a) dl[a] = pt[i][a] - Rr.x[a];
+            part += eval_f64<D>(Rr, dl);
+          }
+        }
+        acc[i] += part;
+      } else {
+        float part = 0.f;
+        for (int q = 0; q < 4; ++q) {
+          unsigned long long m = mask[gl][q];
+          unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)m);
+          unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
+          m = ((unsigned long long)hi32 << 32) | lo32;
+          while (m) {
+            const int r = __builtin_ctzll(m);
+            m &= m - 1;
+            const Rec Rr = rec[q * 64 + r];
+            double dl[D];
+            for (int a = 0; a < D; ++a) dl[a] = pt[i][a] - Rr.x[a];
+            part += eval_split<D>(Rr, dl);
+          }
+        }
+        acc[i] += (double)part;
+      }
+    }
+  }
+  if (A.pairs && lane == 0) atomicAdd(A.pairs, npairs);
+  for (int i = 0; i < SPT; ++i) {
+    if (colv[i] < 0) continue;
+    const long long o = STATIC ? colv[i] : t * A.G.S + colv[i];
+    if constexpr (F64) ((double*)A.out)[o] = A.scale * acc[i];
+    else ((float*)A.out)[o] = (float)(A.scale * acc[i]);
+  }
+}
+
+// static records (same as the dense static path)
+template <int D, bool F64>
+__global__ void k_static_records_c(bh_static_beams b, void* recs) {
+  using Rec = typename RecOf<D, F64>::type;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= b.count) return;
+  State<D> S;
+  for (int a = 0; a < D; ++a) {
+    S.x[a] = b.x[i * D + a];
+    S.p[a] = b.p[i * D + a];
+    for (int e = 0; e < D; ++e)
+      S.M[a][e] = mkc(b.M[2 * ((i * D + a) * D + e)], b.M[2 * ((i * D + a) * D + e) + 1]);
+  }
+  S.A = mkc(b.A[2 * i], b.A[2 * i + 1]);
+  make_rec<D>(S, b.omega[i], ((Rec*)recs)[i]);
+}
+
+int check_segments(const bh_segments* g, int ndim) {
+  if (!g) BH_FAIL(BH_ERR_ARG, "segments: null");
+  if (g->nseg <= 0 || g->ngroups <= 0 || g->S <= 0)
+    BH_FAIL(BH_ERR_ARG, "segments: nseg=%lld ngroups=%lld S=%lld", (long long)g->nseg,
+            (long long)g->ngroups, (long long)g->S);
+  if (!g->seg_p0 || !g->seg_v || !g->seg_n || !g->seg_g0 || !g->grp_seg || !g->pts || !g->col)
+    BH_FAIL(BH_ERR_ARG, "segments: null array");
+  if (ndim < 1 || ndim > 3) BH_FAIL(BH_ERR_ARG, "segments: ndim=%d", ndim);
+  return BH_OK;
+}
+
+SegDev to_segdev(const bh_segments* g) {
+  SegDev d;
+  d.nseg = g->nseg;
+  d.ngroups = g->ngroups;
+  d.S = g->S;
+  d.p0 = g->seg_p0;
+  d.v = g->seg_v;
+  d.n = g->seg_n;
+  d.g0 = g->seg_g0;
+  d.grp_seg = g->grp_seg;
+  d.pts = g->pts;
+  d.col = g->col;
+  return d;
+}
+
+template <int D, bool F64, bool STATIC>
+void launch_culled(const CulledArgs& a, long long ny, hipStream_t s) {
+  // groups per block: enough blocks to fill 256 CUs, at most 32 groups
+  const long long G = a.G.ngroups;
+  long long want = G * ny;
+  if (want >= 2048 && G >= 32) {
+    dim3 grid((unsigned)((G + 31) / 32), (unsigned)ny);
+    hipLaunchKernelGGL((k_culled_sum<D, 8, F64, STATIC>), grid, dim3(256), 0, s, a);
+  } else if (G >= 16 && (G / 16) * ny >= 512) {
+    dim3 grid((unsigned)((G + 15) / 16), (unsigned)ny);
+    hipLaunchKernelGGL((k_culled_sum<D, 4, F64, STATIC>), grid, dim3(256), 0, s, a);
+  } else if (G >= 8) {
+    dim3 grid((unsigned)((G + 7) / 8), (unsigned)ny);
+    hipLaunchKernelGGL((k_culled_sum<D, 2, F64, STATIC>), grid, dim3(256), 0, s, a);
+  } else {
+    dim3 grid((unsigned)((G + 3) / 4), (unsigned)ny);
+    hipLaunchKernelGGL((k_culled_sum<D, 1, F64, STATIC>), grid, dim3(256), 0, s, a);
+  }
+}
+
+}  // namespace
+
+extern "C" int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror, double c,
+                                     const double* ts, int64_t nt, const bh_segments* seg,
+                                     double cull_log2, int precision, void* out,
+                                     unsigned long long* pairs, bh_stream s) {
+  bh::clear_error();
+  if (int rc = check_segments(seg, ndim)) return rc;
+  if (!b || !ts || !out) BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: null pointer");
+  if (nt <= 0 || nt > 65535) BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: nt=%lld", (long long)nt);
+  if (precision != BH_PRECISION_FP64 && precision != BH_PRECISION_FP32_SPLIT)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: precision=%d", precision);
+  hipStream_t st = (hipStream_t)s;
+  const bool f64 = precision == BH_PRECISION_FP64;
+  if (b->count == 0) {
+    BH_HIP(hipMemsetAsync(out, 0, (size_t)nt * seg->S * (f64 ? 8 : 4), st));
+    return BH_OK;
+  }
+  if (!b->x0 || !b->p0 || !b->omega || !b->alpha || !b->a0)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: null beam array");
+  CulledArgs a{};
+  a.b = *b;
+  a.count = b->count;
+  a.total = b->count * (mirror ? 2 : 1);
+  a.c = c;
+  a.ts = ts;
+  a.recs = nullptr;
+  a.G = to_segdev(seg);
+  a.E = cull_log2;
+  a.scale = 1.0;
+  a.out = out;
+  a.pairs = pairs;
+  if (ndim == 1) { if (f64) launch_culled<1, true, false>(a, nt, st); else launch_culled<1, false, false>(a, nt, st); }
+  else if (ndim == 2) { if (f64) launch_culled<2, true, false>(a, nt, st); else launch_culled<2, false, false>(a, nt, st); }
+  else { if (f64) launch_culled<3, true, false>(a, nt, st); else launch_culled<3, false, false>(a, nt, st); }
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
+                                    double scale, double cull_log2, int precision, void* out,
+                                    unsigned long long* pairs, void* ws, size_t ws_bytes,
+                                    bh_stream s) {
+  bh::clear_error();
+  if (int rc = check_segments(seg, ndim)) return rc;
+  if (!b || !out) BH_FAIL(BH_ERR_ARG, "bh_static_sum_culled: null pointer");
+  if (precision != BH_PRECISION_FP64 && precision != BH_PRECISION_FP32_SPLIT)
+    BH_FAIL(BH_ERR_ARG, "bh_static_sum_culled: precision=%d", precision);
+  hipStream_t st = (hipStream_t)s;
+  const bool f64 = precision == BH_PRECISION_FP64;
+  if (b->count == 0) {
+    BH_HIP(hipMemsetAsync(out, 0, (size_t)seg->S * (f64 ? 8 : 4), st));
+    return BH_OK;
+  }
+  const size_t need = bh_static_sum_workspace_size(b->count, ndim);
+  if (!ws || ws_bytes < need)
+    BH_FAIL(BH_ERR_WORKSPACE, "bh_static_sum_culled: workspace %zu < %zu", ws_bytes, need);
+  CulledArgs a{};
+  a.count = b->count;
+  a.total = b->count;
+  a.recs = ws;
+  a.G = to_segdev(seg);
+  a.E = cull_log2;
+  a.scale = scale;
+  a.out = out;
+  a.pairs = pairs;
+  const unsigned gr = bh::grid1d(b->count, 128);
+#define BH_SC(D, F)                                                                           \
+  do {                                                                                        \
+    hipLaunchKernelGGL((k_static_records_c<D, F>), dim3(gr), dim3(128), 0, st, *b, ws);        \
+    launch_culled<D, F, true>(a, 1, st);                                                      \
+  } while (0)
+  if (ndim == 1) { if (f64) BH_SC(1, true); else BH_SC(1, false); }
+  else if (ndim == 2) { if (f64) BH_SC(2, true); else BH_SC(2, false); }
+  else { if (f64) BH_SC(3, true); else BH_SC(3, false); }
+#undef BH_SC
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}

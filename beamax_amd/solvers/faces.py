@@ -130,5 +130,5 @@ class FourEdgeAdjoint:
 
     def __call__(self, data):
         b = self.beams(data)
-        q = self.solver.sum_static(b, self.points, 1.0 / self.c ** 2)
+        q = self.solver.sum_static(b, self.points, 1.0 / self.c ** 2, dx=self.domain.dx)
         return q.reshape(self.domain.N)

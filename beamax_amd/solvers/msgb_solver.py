@@ -168,7 +168,7 @@ class MSGBSolver:
     def __init__(self, thr: Union[int, float], thr_strat: str, batch_size: int, input_type: str,
                  ode_solver, sum_method: str, tr_ode_solver=None, sharding=None,
                  ode_config=None, adjoint_relative_guard: float = 5e-2,
-                 precision: str = "fp32_split"):
+                 precision: str = "fp32_split", cull_log2: float = 40.0):
         if thr_strat not in _THR_STRATS:
             raise ValueError(f"thr_strat must be one of {', '.join(sorted(_THR_STRATS))}; "
                              f"got {thr_strat!r}.")
@@ -217,6 +217,12 @@ class MSGBSolver:
         self.aggregate_method = ("scan" if "scan" in sum_method else
                                  "vmap" if "vmap" in sum_method else "all")
         self.precision = precision
+        # envelope cut: a (beam, 64-point group) pair is skipped when every
+        # term is below 2^-cull_log2 of the beam's own 2|A|; 0 = dense sums
+        self.cull_log2 = float(cull_log2)
+        self._seg_cache = {}
+        # optional device counter (uint64 tensor) of evaluated (beam, 64-point) pairs
+        self.pair_counter = None
 
     # ------------------------------------------------------------------
     def _effective_top_n(self, wpt: MSWPT, *, half_frame: bool = False):
@@ -260,11 +266,35 @@ class MSGBSolver:
                   idx.numel(), beams.struct(), _lib.stream_handle())
         return beams, idx, val
 
+    def _segments(self, points, dx, device):
+        from ..segments import Segments
+        key = (points.shape, hash(np.ascontiguousarray(points, dtype=np.float64).tobytes()),
+               tuple(dx), str(device))
+        seg = self._seg_cache.get(key)
+        if seg is None:
+            seg = Segments(points, dx, device)
+            self._seg_cache[key] = seg
+        return seg
+
     def sum_forward(self, beams: ForwardBeams, domain: Domain, sensor_positions, ts, out=None):
         """Mirrored real beam sum at the sensors, (Nt, S)."""
         import torch
         dev = beams.x0.device
         d = domain.ndim
+        if self.cull_log2 > 0 and not any(domain.periodic):
+            pts = np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d)
+            seg = self._segments(pts, domain.dx, dev)
+            if seg.efficient:
+                tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
+                dt = torch.float64 if self.precision == "fp64" else torch.float32
+                if out is None:
+                    out = torch.empty((tsd.numel(), seg.S), dtype=dt, device=dev)
+                b = beams.struct()
+                ss = seg.struct()
+                _lib.call("bh_forward_sum_culled", C.byref(b), d, 1, float(domain.c), _lib.ptr(tsd),
+                          tsd.numel(), C.byref(ss), self.cull_log2, _lib.PRECISION[self.precision],
+                          _lib.ptr(out), _lib.ptr(self.pair_counter), _lib.stream_handle())
+                return out
         pos = torch.as_tensor(np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d),
                               device=dev)
         tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
@@ -398,11 +428,30 @@ class MSGBSolver:
         beams.idx, beams.val, beams.coeffs = idx, val, coeffs   # diagnostics
         return beams
 
-    def sum_static(self, beams: StaticBeams, points, scale: float, out=None):
-        """out[P] = scale * sum_b field_b(points) (compute_TR_result at t = 0)."""
+    def sum_static(self, beams: StaticBeams, points, scale: float, out=None, dx=None):
+        """out[P] = scale * sum_b field_b(points) (compute_TR_result at t = 0).
+
+        With ``dx`` (the grid spacing of the points) and culling on, the
+        points are segment-decomposed and the culled kernel is used."""
         import torch
         dev = beams.x.device
         d = beams.d
+        if self.cull_log2 > 0 and dx is not None:
+            pn = _np(points) if isinstance(points, torch.Tensor) else points
+            pn = np.ascontiguousarray(pn, dtype=np.float64).reshape(-1, d)
+            seg = self._segments(pn, dx, dev)
+            if seg.efficient:
+                dt = torch.float64 if self.precision == "fp64" else torch.float32
+                if out is None:
+                    out = torch.empty(seg.S, dtype=dt, device=dev)
+                ws = _lib.workspace(_lib.load().bh_static_sum_workspace_size(beams.count, d), dev)
+                b = beams.struct()
+                ss = seg.struct()
+                _lib.call("bh_static_sum_culled", C.byref(b), d, C.byref(ss), float(scale),
+                          self.cull_log2, _lib.PRECISION[self.precision], _lib.ptr(out),
+                          _lib.ptr(self.pair_counter), _lib.ptr(ws), ws.numel(),
+                          _lib.stream_handle())
+                return out
         pts = torch.as_tensor(np.ascontiguousarray(points, dtype=np.float64).reshape(-1, d),
                               device=dev) if not isinstance(points, torch.Tensor) else points
         P = pts.shape[0]
@@ -443,6 +492,6 @@ class MSGBSolver:
         c_src = float(sources.domain.c)
         beams = self.prepare_adjoint_beams(data.to(data_wpt.device), sources, data_domain,
                                            data_wpt, c_src, window)
-        q = self.sum_static(beams, pos, 1.0 / float(domain.c) ** 2)
+        q = self.sum_static(beams, pos, 1.0 / float(domain.c) ** 2, dx=domain.dx)
         shape = np.asarray(sensors.positions if isinstance(sensors, Sensor) else pos).shape[:-1]
         return q.reshape(tuple(shape)), beams

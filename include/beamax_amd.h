@@ -32,6 +32,8 @@
  *                       tr_solver_utils.py:205-435, gb_solvers.py:484-594)
  *   bh_fourier_crop_time  prepare_data_domain_for_msgb Fourier crop
  *                       (examples/reconstruction/2d_time_reversal_and_adjoint.py:111-165)
+ *   bh_forward_sum_culled, bh_static_sum_culled: the same sums with a certified
+ *                       envelope cut (no reference counterpart; same seams)
  *   bh_static_sum       compute_TR_result + compute_gaussian_beam_real_TR at
  *                       t = 0 + c^-2 weight (tr_solver_utils.py:677-750,
  *                       beamax/gb/core.py:332-425, msgb_solver.py:123-127)
@@ -193,6 +195,40 @@ size_t bh_static_sum_workspace_size(int64_t count, int ndim);
 int bh_static_sum(const bh_static_beams* b, int ndim, const double* points, int64_t P,
                   double scale, int precision, void* out, void* ws, size_t ws_bytes,
                   bh_stream s);
+
+/* Evaluation points decomposed into straight, equally spaced runs
+ * ("segments") packed into groups of 64 points (one point per lane).
+ * Segment k: points p0[k] + j*v[k], j < n[k]; its groups are g0[k] ...
+ * Group g holds pts[g*64 .. g*64+63] (d doubles each, padded slots repeat a
+ * valid point) and writes them to output columns col[g*64 + lane] (-1 =
+ * padding).  All arrays are device memory. */
+typedef struct {
+  int64_t nseg;
+  int64_t ngroups;
+  int64_t S;                 /* number of output columns */
+  const double* seg_p0;      /* nseg x d */
+  const double* seg_v;       /* nseg x d */
+  const int32_t* seg_n;      /* nseg */
+  const int32_t* seg_g0;     /* nseg */
+  const int32_t* grp_seg;    /* ngroups */
+  const double* pts;         /* ngroups*64 x d */
+  const int32_t* col;        /* ngroups*64 */
+} bh_segments;
+
+/* Culled forms of bh_forward_sum / bh_static_sum: a (beam, 64-point group)
+ * pair is evaluated only if some point of the group has envelope
+ * 2^-exponent >= 2^-cull_log2 relative to the beam's own 2|A| (exact closed
+ * form along each segment, widened by one point); every evaluated term is
+ * the same as in the dense form.  cull_log2 <= 0 evaluates every pair.
+ * Dropped terms are each below 2^-cull_log2 * 2|A_b| (in absolute value).
+ * `pairs` (nullable, device) is incremented by the number of evaluated
+ * (beam, group) pairs; each pair is 64 lane evaluations. */
+int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror, double c,
+                          const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
+                          int precision, void* out, unsigned long long* pairs, bh_stream s);
+int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
+                         double scale, double cull_log2, int precision, void* out,
+                         unsigned long long* pairs, void* ws, size_t ws_bytes, bh_stream s);
 
 #ifdef __cplusplus
 }

@@ -289,3 +289,94 @@ def test_static_sum_matches_oracle(gpu, d):
                        precision=prec)
         got = _n(s.sum_static(sb, pts, 0.5)).astype(np.float64)
         assert rel(ref, got) < tol
+
+
+# ---------------------------------------------------------------------------
+# culled sums over segment-decomposed points
+# ---------------------------------------------------------------------------
+def _grid_points(kind, n, d, dx):
+    if kind == "edges":
+        m = np.zeros((n,) * d)
+        m[0] = 1
+        m[-1] = 1
+        if d == 2:
+            m[:, 0] = 1
+            m[:, -1] = 1
+        return np.stack(np.nonzero(m), 1) * dx
+    if kind == "grid":
+        return np.stack(np.nonzero(np.ones((n,) * d)), 1) * dx
+    return np.stack(np.nonzero(np.ones((n,) * d)), 1)[::7] * dx
+
+
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("kind", ["edges", "grid"])
+def test_culled_forward_matches_dense(gpu, d, kind):
+    import torch
+    from beamax_amd import _lib
+    from beamax_amd.segments import Segments
+    from beamax_amd.solvers.msgb_solver import ForwardBeams
+    from oracle import cpu
+    rng = np.random.default_rng(30 + d)
+    n = 96 if d == 2 else 24
+    dx = 1e-4
+    L = n * dx
+    B = 300
+    x0, p0, om, alpha, a0 = _random_beams(rng, B, d, L, 0.5 / dx)
+    fb = ForwardBeams(B, d, gpu)
+    for name, v in (("x0", x0), ("p0", p0), ("omega", om), ("alpha", alpha), ("a0", a0)):
+        getattr(fb, name).copy_(_t(v, gpu))
+    pts = _grid_points(kind, n, d, dx)
+    ts = np.arange(0, 1.5 * L / 1500.0, 2e-8 * 5)
+    seg = Segments(pts, [dx] * d, gpu)
+    assert seg.on_grid and (d == 3 or seg.efficient)
+    dup = lambda v: np.concatenate([v, v])
+    ref = cpu.forward_sum(dup(x0), dup(p0), dup(alpha), dup(a0), dup(om),
+                          np.concatenate([np.ones(B), -np.ones(B)]), 1500.0, ts, pts)
+    ts_d = _t(ts, gpu)
+    for prec, E, tol in (("fp64", 0.0, 1e-12), ("fp64", 40.0, 1e-10), ("fp32_split", 40.0, 1e-5)):
+        out = torch.full((ts.size, pts.shape[0]), np.nan,
+                         dtype=torch.float64 if prec == "fp64" else torch.float32, device=gpu)
+        cnt = torch.zeros(1, dtype=torch.uint64, device=gpu)
+        ss = seg.struct()
+        _lib.call("bh_forward_sum_culled", C.byref(fb.struct()), d, 1, 1500.0, _lib.ptr(ts_d),
+                  ts.size, C.byref(ss), E, _lib.PRECISION[prec], _lib.ptr(out), _lib.ptr(cnt),
+                  _lib.stream_handle())
+        got = _n(out).astype(np.float64)
+        assert np.all(np.isfinite(got))
+        assert rel(ref, got) < tol
+        pairs = int(cnt.item())
+        dense_pairs = 2 * B * ts.size * seg.ngroups
+        assert 0 < pairs <= dense_pairs
+        if E == 0.0:
+            assert pairs == dense_pairs
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_culled_static_matches_dense(gpu, d):
+    import torch
+    from beamax_amd import _lib
+    from beamax_amd.segments import Segments
+    from beamax_amd.solvers.msgb_solver import StaticBeams
+    from oracle import cpu
+    rng = np.random.default_rng(40 + d)
+    n = 128 if d == 2 else 32
+    dx = 1e-4
+    L = n * dx
+    B = 400
+    x0, p0, om, alpha, a0 = _random_beams(rng, B, d, L, 0.5 / dx)
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p0, alpha, a0, np.ones(B), np.array([2e-6]), 1500.0)
+    sb = StaticBeams(B, d, gpu)
+    sb.x.copy_(_t(xt[:, 0], gpu)); sb.p.copy_(_t(pt[:, 0], gpu)); sb.M.copy_(_t(Mt[:, 0], gpu))
+    sb.A.copy_(_t(At[:, 0], gpu)); sb.omega.copy_(_t(om, gpu))
+    pts = _grid_points("grid", n, d, dx)
+    ref = cpu.static_sum(xt[:, 0], pt[:, 0], Mt[:, 0], At[:, 0], om, pts) * 2.0
+    seg = Segments(pts, [dx] * d, gpu)
+    for prec, E, tol in (("fp64", 0.0, 1e-12), ("fp64", 40.0, 1e-10), ("fp32_split", 40.0, 1e-5)):
+        out = torch.empty(pts.shape[0], dtype=torch.float64 if prec == "fp64" else torch.float32,
+                          device=gpu)
+        ws = _lib.workspace(_lib.load().bh_static_sum_workspace_size(B, d), gpu)
+        ss = seg.struct()
+        _lib.call("bh_static_sum_culled", C.byref(sb.struct()), d, C.byref(ss), 2.0, E,
+                  _lib.PRECISION[prec], _lib.ptr(out), None, _lib.ptr(ws), ws.numel(),
+                  _lib.stream_handle())
+        assert rel(ref, _n(out).astype(np.float64)) < tol

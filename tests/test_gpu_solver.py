@@ -50,12 +50,12 @@ def test_readme_forward_fp64_and_split(gpu):
     mask[0, :] = 1
     sensors = bx.Sensor(domain=dom, binary_mask=mask)
     outs = {}
-    for prec in ("fp64", "fp32_split"):
+    for prec, cull in (("fp64", 0.0), ("fp64", 40.0), ("fp32_split", 40.0), ("fp32_split", 0.0)):
         s = bx.MSGBSolver(thr=4096, thr_strat="top_n", batch_size=128, input_type="spatial",
                           ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real",
-                          precision=prec)
+                          precision=prec, cull_log2=cull)
         data, params = s.forward_with_params(p0, dom, sensors, ts, wpt)
-        outs[prec] = _n(data).astype(np.float64)
+        outs[prec if cull == 0 else prec + "_cull"] = _n(data).astype(np.float64)
     assert outs["fp64"].shape == (604, 128)
     # oracle: same pipeline in NumPy + C++ sum
     ow = OMSWPT(Tiling(3, N, (4, 8, 16), (1, 1)), 2, "rectangular_mirror")
@@ -68,6 +68,9 @@ def test_readme_forward_fp64_and_split(gpu):
                            np.concatenate([mode, -mode]), 1500.0, ts, sensors.positions)
     assert rel(ref, outs["fp64"]) < 1e-12
     assert rel(outs["fp64"], outs["fp32_split"]) < 1e-5
+    # culled sums: every dropped term is below 2^-40 of its beam's 2|A|
+    assert rel(outs["fp64"], outs["fp64_cull"]) < 1e-10
+    assert rel(outs["fp64"], outs["fp32_split_cull"]) < 1e-5
     # reference's diagnostic tuple: (p0s, M0s, x0s, omegas, a0s, modes), batched by 128
     assert params[0].shape == (64, 128, 2) and params[5].shape == (64, 128)
 
@@ -132,8 +135,13 @@ def test_adjoint_2d_face_matches_oracle(gpu, face):
     data = _line_data(rng, nt, n, dt)
     img = bx.Sensor(dom, binary_mask=np.ones((n, n)))
     s = bx.MSGBSolver(thr=3000, thr_strat="top_n", batch_size=64, input_type="spatial",
-                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64")
+                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64",
+                      cull_log2=0.0)
     q, beams = s.adjoint_with_params(torch.as_tensor(data, device=gpu), dom, img, sources, ts, ddom, dw)
+    sc = bx.MSGBSolver(thr=3000, thr_strat="top_n", batch_size=64, input_type="spatial",
+                       ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64")
+    qc = sc.adjoint(torch.as_tensor(data, device=gpu), dom, img, sources, ts, ddom, dw)
+    assert rel(_n(q), _n(qc)) < 1e-10
     ow = OMSWPT(Tiling(2, (nt, n), (4, 8), (2, 1)), 2, "rectangular_mirror")
     _check_adjoint(beams, q, ow, data, dt, (nt, n), (dt, 1e-4), sources, (n, n), (1e-4, 1e-4),
                    img.positions, 3000)
@@ -162,7 +170,8 @@ def test_adjoint_3d_face_matches_oracle(gpu):
     data = base[:, :, None] * np.exp(-((np.arange(n) - 16) / 6.0) ** 2)[None, None, :]
     pts = rng.random((2000, 3)) * n * 1e-4
     s = bx.MSGBSolver(thr=3000, thr_strat="top_n", batch_size=64, input_type="spatial",
-                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64")
+                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64",
+                      cull_log2=0.0)
     q, beams = s.adjoint_with_params(torch.as_tensor(data, device=gpu), dom, pts, sources, ts,
                                      ddom, dw)
     ow = OMSWPT(Tiling(2, (nt, n, n), (4, 8), (2, 1, 1)), 2, "rectangular_mirror")

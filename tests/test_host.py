@@ -1,0 +1,93 @@
+"""Host-side logic: point segmentation, validation messages, C-ABI exports."""
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import beamax_amd as bx
+from beamax_amd.segments import _runs
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.mark.parametrize("N", [(512, 512), (64, 48), (16, 16, 16)])
+def test_runs_cover_every_point_once(N):
+    from beamax_amd.solvers.faces import four_edge_mask
+    masks = [np.ones(N)]
+    if len(N) == 2:
+        masks.append(four_edge_mask(N))
+    else:
+        m = np.zeros(N)
+        m[0] = 1
+        masks.append(m)
+    for m in masks:
+        idx = np.stack(np.nonzero(m), 1)
+        runs = _runs(idx, 8)
+        members = np.concatenate([r[0] for r in runs])
+        assert np.array_equal(np.sort(members), np.arange(len(idx)))
+        for mem, axis in runs:
+            if len(mem) > 1:
+                step = np.diff(idx[mem], axis=0)
+                e = np.zeros(len(N), int)
+                e[axis] = 1
+                assert np.all(step == e)
+    # four edges decompose into exactly four straight runs
+    if len(N) == 2:
+        runs = _runs(np.stack(np.nonzero(four_edge_mask(N)), 1), 8)
+        assert len(runs) == 4
+
+
+def test_decomposition_validation_messages():
+    with pytest.raises(ValueError, match="at least 2 Fourier samples"):
+        bx.DyadicDecomposition(3, (64, 64), (4, 8, 16), (1, 1))
+    d = bx.DyadicDecomposition(3, (256, 256), (4, 8, 16), (1, 1))
+    assert tuple(d.box_lengths) == (4, 8, 16) and tuple(d.num_boxes_ndim) == (16, 60, 240)
+
+
+def test_solver_validation_messages():
+    from beamax_amd.gb import gb_solvers
+    mk = lambda **kw: bx.MSGBSolver(**{**dict(thr=10, thr_strat="top_n", batch_size=8,
+                                               input_type="spatial",
+                                               ode_solver=gb_solvers.solve_hom_diag,
+                                               sum_method="scan_real"), **kw})
+    with pytest.raises(ValueError, match="thr_strat"):
+        mk(thr_strat="nope")
+    with pytest.raises(ValueError, match="positive integer"):
+        mk(thr=0)
+    with pytest.raises(ValueError, match="input_type"):
+        mk(input_type="x")
+    with pytest.raises(ValueError, match="batch_size"):
+        mk(batch_size=0)
+    with pytest.raises(ValueError, match="sum_method"):
+        mk(sum_method="x")
+    with pytest.raises(ValueError, match="adjoint_relative_guard"):
+        mk(adjoint_relative_guard=1.5)
+    with pytest.raises(ValueError, match=r"\[0, 100\]"):
+        mk(thr=120.0, thr_strat="percentile")
+
+
+def test_sensor_mask_order_is_row_major():
+    dom = bx.Domain(N=(4, 5), dx=(1.0, 2.0), periodic=(False, False))
+    m = np.zeros((4, 5))
+    m[0, 2] = m[1, 0] = m[3, 4] = 1
+    s = bx.Sensor(dom, binary_mask=m)
+    assert np.array_equal(s.positions, [[0, 4.0], [1.0, 0.0], [3.0, 8.0]])
+    with pytest.raises(ValueError, match="binary values"):
+        bx.Sensor(dom, binary_mask=m * 2)
+
+
+def test_library_exports_every_declared_symbol():
+    """The C-ABI library loads and exports every function include/*.h declares
+    (no GPU call is made)."""
+    from beamax_amd import _lib
+    lib = C.CDLL(str(_lib.LIB_PATH))
+    text = "\n".join(p.read_text() for p in (ROOT / "include").glob("*.h"))
+    names = set(re.findall(r"\b(bh_[a-z0-9_]+)\s*\(", text))
+    assert names, "no declarations found"
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.SIGNATURES) <= names
+    assert lib.bh_version() == 1
