@@ -243,3 +243,60 @@ struct RecOf<D, true> {
 
 
 }  // namespace
+
+namespace {
+
+// Closed form of solve_hom_diag (gb_solvers.py:306-422) in the expanded
+// Sherman-Morrison form used by the culled sums: with s = c0 t / |p|,
+//   w_a = alpha_a / (1 + s alpha_a),  q_a = w_a p_a,
+//   kappa = (s / |p|^2) / (1 - (s / |p|^2) sum_j p_j^2 w_j),
+//   M = diag(w) + kappa q q^T
+// (identical to diag(alpha) (A + u v^T)^-1 with A = I + s diag(alpha),
+// u = -s p / |p|^2, v = p o alpha), and the reference's amplitude formulas.
+template <int D>
+__device__ inline void hom_diag_state_fast(const double* x0, const double* p0, const cd* alpha,
+                                           cd a0, double c0, double t, State<D>& S) {
+  double np2 = 0.0;
+  for (int a = 0; a < D; ++a) np2 += p0[a] * p0[a];
+  const double np = sqrt(np2);
+  const double inp = 1.0 / np;
+  const double s = (c0 * t) * inp;
+  const double sn = s / np2;
+  cd w[D], q[D];
+  cd den = mkc(1.0);
+  for (int a = 0; a < D; ++a) {
+    S.x[a] = x0[a] + (c0 * (p0[a] * inp)) * t;
+    S.p[a] = p0[a];
+    // w = alpha / (1 + s alpha)
+    const double br = 1.0 + s * alpha[a].re, bi = s * alpha[a].im;
+    const double r = 1.0 / (br * br + bi * bi);
+    w[a] = mkc((alpha[a].re * br + alpha[a].im * bi) * r, (alpha[a].im * br - alpha[a].re * bi) * r);
+    q[a] = w[a] * p0[a];
+    den = den - (sn * (p0[a] * p0[a])) * w[a];
+  }
+  const double dr = 1.0 / (den.re * den.re + den.im * den.im);
+  const cd kap = mkc(sn * den.re * dr, -sn * den.im * dr);
+  for (int a = 0; a < D; ++a)
+    for (int b = a; b < D; ++b) {
+      cd m = kap * (q[a] * q[b]);
+      if (a == b) m = m + w[a];
+      S.M[a][b] = m;
+      S.M[b][a] = m;
+    }
+  if constexpr (D == 3) {
+    const double tsc = c0 * t / (np2 * np2);
+    const cd a1 = alpha[0], a2 = alpha[1], a3 = alpha[2];
+    const cd t1 = (c0 * t) * (a2 * a3) + (a2 + a3) * np;
+    const cd t2 = (c0 * t) * (a1 * a3) + (a1 + a3) * np;
+    const cd t3 = (c0 * t) * (a1 * a2) + (a1 + a2) * np;
+    const cd dd = mkc(1.0) + tsc * ((p0[0] * p0[0]) * t1 + (p0[1] * p0[1]) * t2 + (p0[2] * p0[2]) * t3);
+    S.A = a0 / csqrt_(dd);
+  } else if constexpr (D == 2) {
+    const cd ap = ((p0[0] * p0[0]) * alpha[1] + (p0[1] * p0[1]) * alpha[0]) * (inp / np2);
+    S.A = a0 / csqrt_(mkc(1.0) + (c0 * t) * ap);
+  } else {
+    S.A = a0;
+  }
+}
+
+}  // namespace

@@ -12,6 +12,8 @@
 // exactly the active (record, group) pairs, in record order -- no atomics, so
 // the result is deterministic.  Every evaluated term is computed exactly as
 // in the dense kernels.
+#include <type_traits>
+
 #include "bh_beamcore.hpp"
 
 namespace {
@@ -112,6 +114,9 @@ struct CulledArgs {
   double scale;
   void* out;
   unsigned long long* pairs;
+  double* part;        // split > 1: partial sums [split][rows][S] (FP64)
+  int split;
+  long long rows;      // output rows (nt for the forward, 1 for the static form)
 };
 
 template <int D, int SPT, bool F64, bool STATIC>
@@ -120,30 +125,29 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
   constexpr int GPB = 4 * SPT;
   __shared__ Rec rec[kRecChunk];
   __shared__ unsigned long long mask[GPB][4];
+  __shared__ double accs[GPB][64];
+  __shared__ int gseg[GPB], gj0[GPB];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const long long gbase = (long long)blockIdx.x * GPB;
   const long long t = STATIC ? 0 : (long long)blockIdx.y;
   const double tv = STATIC ? 0.0 : A.ts[t];
   const int ng = (int)((A.G.ngroups - gbase) < GPB ? (A.G.ngroups - gbase) : GPB);
-  // points of the owned groups
-  double pt[SPT][D];
-  int colv[SPT];
-  for (int i = 0; i < SPT; ++i) {
-    const int gl = wv + 4 * i;
-    colv[i] = -1;
-    for (int a = 0; a < D; ++a) pt[i][a] = 0.0;
-    if (gl < ng) {
-      const long long slot = (gbase + gl) * 64 + lane;
-      for (int a = 0; a < D; ++a) pt[i][a] = A.G.pts[slot * D + a];
-      colv[i] = A.G.col[slot];
-    }
+  // accumulators and group tables in LDS; each group's point is reloaded
+  // (L1/L2 resident) when the wave evaluates that group
+  for (int i = 0; i < SPT; ++i) accs[wv + 4 * i][lane] = 0.0;
+  if (threadIdx.x < GPB && (int)threadIdx.x < ng) {
+    const long long g = gbase + threadIdx.x;
+    const int k = A.G.grp_seg[g];
+    gseg[threadIdx.x] = k;
+    gj0[threadIdx.x] = (int)((g - A.G.g0[k]) * 64);
   }
-  double acc[SPT];
-  for (int i = 0; i < SPT; ++i) acc[i] = 0.0;
   unsigned long long npairs = 0;
   const long long total = STATIC ? A.count : A.total;
-  for (long long cb = 0; cb < total; cb += kRecChunk) {
+  const long long nchunk = (total + kRecChunk - 1) / kRecChunk;
+  const long long c_lo = nchunk * blockIdx.z / A.split;
+  const long long c_hi = nchunk * (blockIdx.z + 1) / A.split;
+  for (long long cb = c_lo * kRecChunk; cb < c_hi * kRecChunk && cb < total; cb += kRecChunk) {
     __syncthreads();   // previous chunk's records and masks fully consumed
     const long long bi = cb + threadIdx.x;
     Rec& R = rec[threadIdx.x];
@@ -162,8 +166,8 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
           al[a] = mkc(A.b.alpha[2 * (j * D + a)], A.b.alpha[2 * (j * D + a) + 1]);
         }
         State<D> St;
-        hom_diag_state<D>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]), neg ? -A.c : A.c,
-                          tv, St);
+        hom_diag_state_fast<D>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]),
+                               neg ? -A.c : A.c, tv, St);
         make_rec<D>(St, A.b.omega[j], R);
       }
       live = !rec_zero<D>(R);   // zero amplitude contributes exactly 0
@@ -176,15 +180,14 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
     long long cur_seg = -1;
     int lo = 1, hi = 0;
     for (int gl = 0; gl < ng; ++gl) {
-      const long long g = gbase + gl;
       bool act = live;
       if (live && A.E > 0.0) {
-        const long long k = A.G.grp_seg[g];
+        const int k = gseg[gl];
         if (k != cur_seg) {
           seg_interval<D>(A.G, k, R.x, Q, A.E, lo, hi);
           cur_seg = k;
         }
-        const int j0 = (int)((g - A.G.g0[k]) * 64);
+        const int j0 = gj0[gl];
         act = (hi >= lo) && (hi >= j0) && (lo <= j0 + 63);
       }
       const unsigned long long m = __ballot(act);
@@ -195,50 +198,73 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
     for (int i = 0; i < SPT; ++i) {
       const int gl = wv + 4 * i;
       if (gl >= ng) continue;
-      if constexpr (F64) {
-        double part = 0.0;
-        for (int q = 0; q < 4; ++q) {
-          unsigned long long m = mask[gl][q];
-          unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)m);
-          unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
-          m = ((unsigned long long)hi32 << 32) | lo32;
-          while (m) {
-            const int r = __builtin_ctzll(m);
-            m &= m - 1;
-            const Rec Rr = rec[q * 64 + r];
-            double dl[D];
-            for (int a = 0; a < D; ++a) dl[a] = pt[i][a] - Rr.x[a];
-            part += eval_f64<D>(Rr, dl);
-          }
-        }
-        acc[i] += part;
-      } else {
-        float part = 0.f;
-        for (int q = 0; q < 4; ++q) {
-          unsigned long long m = mask[gl][q];
-          unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)m);
-          unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
-          m = ((unsigned long long)hi32 << 32) | lo32;
-          while (m) {
-            const int r = __builtin_ctzll(m);
-            m &= m - 1;
-            const Rec Rr = rec[q * 64 + r];
-            double dl[D];
-            for (int a = 0; a < D; ++a) dl[a] = pt[i][a] - Rr.x[a];
-            part += eval_split<D>(Rr, dl);
-          }
-        }
-        acc[i] += (double)part;
+      using Part = typename std::conditional<F64, double, float>::type;
+      Part part = 0;
+      double pti[D];
+      {
+        const long long slot = (gbase + gl) * 64 + lane;
+        for (int a = 0; a < D; ++a) pti[a] = A.G.pts[slot * D + a];
       }
+      for (int q = 0; q < 4; ++q) {
+        unsigned long long m = mask[gl][q];
+        const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)m);
+        const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
+        m = ((unsigned long long)hi32 << 32) | lo32;
+        while (m) {
+          const int r1 = __builtin_ctzll(m);
+          m &= m - 1;
+          const Rec R1 = rec[q * 64 + r1];
+          double d1[D];
+          for (int a = 0; a < D; ++a) d1[a] = pti[a] - R1.x[a];
+          if (m) {
+            const int r2 = __builtin_ctzll(m);
+            m &= m - 1;
+            const Rec R2 = rec[q * 64 + r2];
+            double d2[D];
+            for (int a = 0; a < D; ++a) d2[a] = pti[a] - R2.x[a];
+            if constexpr (F64) {
+              const double e1 = eval_f64<D>(R1, d1), e2 = eval_f64<D>(R2, d2);
+              part += e1;
+              part += e2;
+            } else {
+              const float e1 = eval_split<D>(R1, d1), e2 = eval_split<D>(R2, d2);
+              part += e1;
+              part += e2;
+            }
+          } else {
+            if constexpr (F64) part += eval_f64<D>(R1, d1);
+            else part += eval_split<D>(R1, d1);
+          }
+        }
+      }
+      accs[gl][lane] += (double)part;
     }
   }
   if (A.pairs && lane == 0) atomicAdd(A.pairs, npairs);
   for (int i = 0; i < SPT; ++i) {
-    if (colv[i] < 0) continue;
-    const long long o = STATIC ? colv[i] : t * A.G.S + colv[i];
-    if constexpr (F64) ((double*)A.out)[o] = A.scale * acc[i];
-    else ((float*)A.out)[o] = (float)(A.scale * acc[i]);
+    const int gl = wv + 4 * i;
+    if (gl >= ng) continue;
+    const int col = A.G.col[(gbase + gl) * 64 + lane];
+    if (col < 0) continue;
+    const long long o = STATIC ? col : t * A.G.S + col;
+    if (A.split > 1) {
+      A.part[(long long)blockIdx.z * A.rows * A.G.S + o] = accs[gl][lane];
+    } else {
+      if constexpr (F64) ((double*)A.out)[o] = A.scale * accs[gl][lane];
+      else ((float*)A.out)[o] = (float)(A.scale * accs[gl][lane]);
+    }
   }
+}
+
+// ordered sum of the split partials (deterministic)
+__global__ void k_combine(const double* __restrict__ part, int split, long long n, double scale,
+                          int f64, void* out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int z = 0; z < split; ++z) s += part[(long long)z * n + i];
+  if (f64) ((double*)out)[i] = scale * s;
+  else ((float*)out)[i] = (float)(scale * s);
 }
 
 // static records (same as the dense static path)
@@ -284,32 +310,71 @@ SegDev to_segdev(const bh_segments* g) {
   return d;
 }
 
-template <int D, bool F64, bool STATIC>
-void launch_culled(const CulledArgs& a, long long ny, hipStream_t s) {
-  // groups per block: enough blocks to fill 256 CUs, at most 32 groups
-  const long long G = a.G.ngroups;
-  long long want = G * ny;
-  if (want >= 2048 && G >= 32) {
-    dim3 grid((unsigned)((G + 31) / 32), (unsigned)ny);
-    hipLaunchKernelGGL((k_culled_sum<D, 8, F64, STATIC>), grid, dim3(256), 0, s, a);
-  } else if (G >= 16 && (G / 16) * ny >= 512) {
-    dim3 grid((unsigned)((G + 15) / 16), (unsigned)ny);
-    hipLaunchKernelGGL((k_culled_sum<D, 4, F64, STATIC>), grid, dim3(256), 0, s, a);
-  } else if (G >= 8) {
-    dim3 grid((unsigned)((G + 7) / 8), (unsigned)ny);
-    hipLaunchKernelGGL((k_culled_sum<D, 2, F64, STATIC>), grid, dim3(256), 0, s, a);
-  } else {
-    dim3 grid((unsigned)((G + 3) / 4), (unsigned)ny);
-    hipLaunchKernelGGL((k_culled_sum<D, 1, F64, STATIC>), grid, dim3(256), 0, s, a);
+// Launch shape: groups per block GPB in {32, 16, 8, 4} (fewer groups per
+// block = more blocks but more record work per group), then a beam split so
+// that the grid covers the chip several times over (blocks resident per CU
+// are limited to ~4 by LDS).
+struct Shape {
+  int gpb;
+  int split;
+};
+
+inline Shape culled_shape(long long G, long long ny, long long nchunk, bool have_part) {
+  const long long target = 4096;   // ~ 16 block rounds of 256 CUs
+  Shape sh{4, 1};
+  const int opts[4] = {32, 16, 8, 4};
+  for (int o : opts) {
+    if ((G + o - 1) / o * ny >= 1024 || o == 4) {
+      sh.gpb = o;
+      break;
+    }
   }
+  const long long blocks = (G + sh.gpb - 1) / sh.gpb * ny;
+  if (have_part && blocks < target) {
+    long long sp = (target + blocks - 1) / blocks;
+    if (sp > 8) sp = 8;
+    if (sp > nchunk) sp = nchunk;
+    sh.split = (int)(sp < 1 ? 1 : sp);
+  }
+  return sh;
+}
+
+template <int D, bool F64, bool STATIC>
+int launch_culled(CulledArgs a, long long ny, long long nchunk, void* part_ws, size_t part_bytes,
+                  hipStream_t s) {
+  const long long G = a.G.ngroups;
+  const size_t per_split = (size_t)ny * a.G.S * sizeof(double);
+  const Shape sh = culled_shape(G, ny, nchunk, part_ws != nullptr && part_bytes >= 2 * per_split);
+  a.split = sh.split;
+  if (sh.split > 1 && (size_t)sh.split * per_split > part_bytes)
+    a.split = (int)(part_bytes / per_split);
+  a.part = (double*)part_ws;
+  a.rows = ny;
+  dim3 grid((unsigned)((G + sh.gpb - 1) / sh.gpb), (unsigned)ny, (unsigned)a.split);
+  if (sh.gpb == 32) hipLaunchKernelGGL((k_culled_sum<D, 8, F64, STATIC>), grid, dim3(256), 0, s, a);
+  else if (sh.gpb == 16) hipLaunchKernelGGL((k_culled_sum<D, 4, F64, STATIC>), grid, dim3(256), 0, s, a);
+  else if (sh.gpb == 8) hipLaunchKernelGGL((k_culled_sum<D, 2, F64, STATIC>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_culled_sum<D, 1, F64, STATIC>), grid, dim3(256), 0, s, a);
+  if (a.split > 1) {
+    const long long n = ny * a.G.S;
+    hipLaunchKernelGGL(k_combine, dim3(bh::grid1d(n, 256)), dim3(256), 0, s, (const double*)part_ws,
+                       a.split, n, a.scale, (int)F64, a.out);
+  }
+  return BH_OK;
 }
 
 }  // namespace
 
+extern "C" size_t bh_forward_sum_culled_workspace_size(int64_t nt, int64_t S) {
+  if (nt <= 0 || S <= 0) return 0;
+  return (size_t)8 * nt * S * sizeof(double);
+}
+
 extern "C" int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror, double c,
                                      const double* ts, int64_t nt, const bh_segments* seg,
                                      double cull_log2, int precision, void* out,
-                                     unsigned long long* pairs, bh_stream s) {
+                                     unsigned long long* pairs, void* ws, size_t ws_bytes,
+                                     bh_stream s) {
   bh::clear_error();
   if (int rc = check_segments(seg, ndim)) return rc;
   if (!b || !ts || !out) BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: null pointer");
@@ -336,11 +401,18 @@ extern "C" int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror
   a.scale = 1.0;
   a.out = out;
   a.pairs = pairs;
-  if (ndim == 1) { if (f64) launch_culled<1, true, false>(a, nt, st); else launch_culled<1, false, false>(a, nt, st); }
-  else if (ndim == 2) { if (f64) launch_culled<2, true, false>(a, nt, st); else launch_culled<2, false, false>(a, nt, st); }
-  else { if (f64) launch_culled<3, true, false>(a, nt, st); else launch_culled<3, false, false>(a, nt, st); }
+  const long long nchunk = (a.total + kRecChunk - 1) / kRecChunk;
+  if (ndim == 1) { if (f64) launch_culled<1, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<1, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
+  else if (ndim == 2) { if (f64) launch_culled<2, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<2, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
+  else { if (f64) launch_culled<3, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<3, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
   BH_LAUNCH_CHECK();
   return BH_OK;
+}
+
+extern "C" size_t bh_static_sum_culled_workspace_size(int64_t count, int ndim, int64_t P) {
+  if (count <= 0 || P <= 0) return 0;
+  const size_t rec = (bh_static_sum_workspace_size(count, ndim) + 255) / 256 * 256;
+  return rec + (size_t)8 * P * sizeof(double);
 }
 
 extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
@@ -358,9 +430,9 @@ extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh
     BH_HIP(hipMemsetAsync(out, 0, (size_t)seg->S * (f64 ? 8 : 4), st));
     return BH_OK;
   }
-  const size_t need = bh_static_sum_workspace_size(b->count, ndim);
-  if (!ws || ws_bytes < need)
-    BH_FAIL(BH_ERR_WORKSPACE, "bh_static_sum_culled: workspace %zu < %zu", ws_bytes, need);
+  const size_t rec = (bh_static_sum_workspace_size(b->count, ndim) + 255) / 256 * 256;
+  if (!ws || ws_bytes < rec)
+    BH_FAIL(BH_ERR_WORKSPACE, "bh_static_sum_culled: workspace %zu < %zu", ws_bytes, rec);
   CulledArgs a{};
   a.count = b->count;
   a.total = b->count;
@@ -370,11 +442,14 @@ extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh
   a.scale = scale;
   a.out = out;
   a.pairs = pairs;
+  void* part = ws_bytes > rec ? (char*)ws + rec : nullptr;
+  const size_t part_bytes = ws_bytes > rec ? ws_bytes - rec : 0;
+  const long long nchunk = (a.count + kRecChunk - 1) / kRecChunk;
   const unsigned gr = bh::grid1d(b->count, 128);
 #define BH_SC(D, F)                                                                           \
   do {                                                                                        \
     hipLaunchKernelGGL((k_static_records_c<D, F>), dim3(gr), dim3(128), 0, st, *b, ws);        \
-    launch_culled<D, F, true>(a, 1, st);                                                      \
+    launch_culled<D, F, true>(a, 1, nchunk, part, part_bytes, st);                            \
   } while (0)
   if (ndim == 1) { if (f64) BH_SC(1, true); else BH_SC(1, false); }
   else if (ndim == 2) { if (f64) BH_SC(2, true); else BH_SC(2, false); }

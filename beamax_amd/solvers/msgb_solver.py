@@ -291,9 +291,12 @@ class MSGBSolver:
                     out = torch.empty((tsd.numel(), seg.S), dtype=dt, device=dev)
                 b = beams.struct()
                 ss = seg.struct()
+                ws = _lib.workspace(
+                    _lib.load().bh_forward_sum_culled_workspace_size(tsd.numel(), seg.S), dev)
                 _lib.call("bh_forward_sum_culled", C.byref(b), d, 1, float(domain.c), _lib.ptr(tsd),
                           tsd.numel(), C.byref(ss), self.cull_log2, _lib.PRECISION[self.precision],
-                          _lib.ptr(out), _lib.ptr(self.pair_counter), _lib.stream_handle())
+                          _lib.ptr(out), _lib.ptr(self.pair_counter), _lib.ptr(ws), ws.numel(),
+                          _lib.stream_handle())
                 return out
         pos = torch.as_tensor(np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d),
                               device=dev)
@@ -444,7 +447,8 @@ class MSGBSolver:
                 dt = torch.float64 if self.precision == "fp64" else torch.float32
                 if out is None:
                     out = torch.empty(seg.S, dtype=dt, device=dev)
-                ws = _lib.workspace(_lib.load().bh_static_sum_workspace_size(beams.count, d), dev)
+                ws = _lib.workspace(
+                    _lib.load().bh_static_sum_culled_workspace_size(beams.count, d, seg.S), dev)
                 b = beams.struct()
                 ss = seg.struct()
                 _lib.call("bh_static_sum_culled", C.byref(b), d, C.byref(ss), float(scale),

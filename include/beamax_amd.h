@@ -222,10 +222,18 @@ typedef struct {
  * the same as in the dense form.  cull_log2 <= 0 evaluates every pair.
  * Dropped terms are each below 2^-cull_log2 * 2|A_b| (in absolute value).
  * `pairs` (nullable, device) is incremented by the number of evaluated
- * (beam, group) pairs; each pair is 64 lane evaluations. */
+ * (beam, group) pairs; each pair is 64 lane evaluations.  Workspace: the
+ * forward form may run without (ws NULL); with the size returned by
+ * bh_forward_sum_culled_workspace_size the beam range is split over more
+ * workgroups and the partial sums are added in a fixed order.  The static
+ * form needs at least bh_static_sum_workspace_size bytes (records) and uses
+ * bh_static_sum_culled_workspace_size for the split. */
+size_t bh_forward_sum_culled_workspace_size(int64_t nt, int64_t S);
 int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror, double c,
                           const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
-                          int precision, void* out, unsigned long long* pairs, bh_stream s);
+                          int precision, void* out, unsigned long long* pairs, void* ws,
+                          size_t ws_bytes, bh_stream s);
+size_t bh_static_sum_culled_workspace_size(int64_t count, int ndim, int64_t P);
 int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
                          double scale, double cull_log2, int precision, void* out,
                          unsigned long long* pairs, void* ws, size_t ws_bytes, bh_stream s);

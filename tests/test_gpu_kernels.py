@@ -338,9 +338,12 @@ def test_culled_forward_matches_dense(gpu, d, kind):
                          dtype=torch.float64 if prec == "fp64" else torch.float32, device=gpu)
         cnt = torch.zeros(1, dtype=torch.uint64, device=gpu)
         ss = seg.struct()
+        # the first case runs without workspace (no beam split), the others split
+        ws = (None if E == 0.0 else
+              _lib.workspace(_lib.load().bh_forward_sum_culled_workspace_size(ts.size, seg.S), gpu))
         _lib.call("bh_forward_sum_culled", C.byref(fb.struct()), d, 1, 1500.0, _lib.ptr(ts_d),
                   ts.size, C.byref(ss), E, _lib.PRECISION[prec], _lib.ptr(out), _lib.ptr(cnt),
-                  _lib.stream_handle())
+                  _lib.ptr(ws), 0 if ws is None else ws.numel(), _lib.stream_handle())
         got = _n(out).astype(np.float64)
         assert np.all(np.isfinite(got))
         assert rel(ref, got) < tol
@@ -374,7 +377,7 @@ def test_culled_static_matches_dense(gpu, d):
     for prec, E, tol in (("fp64", 0.0, 1e-12), ("fp64", 40.0, 1e-10), ("fp32_split", 40.0, 1e-5)):
         out = torch.empty(pts.shape[0], dtype=torch.float64 if prec == "fp64" else torch.float32,
                           device=gpu)
-        ws = _lib.workspace(_lib.load().bh_static_sum_workspace_size(B, d), gpu)
+        ws = _lib.workspace(_lib.load().bh_static_sum_culled_workspace_size(B, d, seg.S), gpu)
         ss = seg.struct()
         _lib.call("bh_static_sum_culled", C.byref(sb.struct()), d, C.byref(ss), 2.0, E,
                   _lib.PRECISION[prec], _lib.ptr(out), None, _lib.ptr(ws), ws.numel(),
