@@ -82,7 +82,7 @@ struct RecS {  // FP32-split
   double qr[Sym<D>::N];
   double ph0;
   float qi[Sym<D>::N];
-  float amp;
+  float amp;   // 2|A|
 };
 
 template <int D>
@@ -177,15 +177,16 @@ __device__ inline void make_delta(const double* s, const double* x, const double
   }
 }
 
+// part += 2|A| cos(2 pi cyc) 2^-ex
 template <int D>
-__device__ inline float eval_split(const RecS<D>& R, const double* dl) {
+__device__ inline void eval_split(const RecS<D>& R, const double* dl, float& part) {
   // cycles = sum_a dl_a (k_a + sum_b Q_ab dl_b) + ph0, Q symmetric
   double cyc = R.ph0;
-  float ex;
+  float arg;
   if constexpr (D == 1) {
     cyc = fma(dl[0], fma(R.qr[0], dl[0], R.k[0]), cyc);
     const float f0 = (float)dl[0];
-    ex = f0 * (R.qi[0] * f0);
+    arg = -f0 * (R.qi[0] * f0);
   } else if constexpr (D == 2) {
     const double w0 = fma(R.qr[0], dl[0], fma(R.qr[2], dl[1], R.k[0]));
     const double w1 = fma(R.qr[1], dl[1], R.k[1]);
@@ -193,7 +194,7 @@ __device__ inline float eval_split(const RecS<D>& R, const double* dl) {
     cyc = fma(dl[1], w1, cyc);
     const float f0 = (float)dl[0], f1 = (float)dl[1];
     const float e0 = fmaf(R.qi[0], f0, R.qi[2] * f1);
-    ex = fmaf(f0, e0, f1 * (R.qi[1] * f1));
+    arg = fmaf(-f0, e0, -f1 * (R.qi[1] * f1));
   } else {
     // qr layout: 00 11 22 01 02 12
     const double w0 = fma(R.qr[0], dl[0], fma(R.qr[3], dl[1], fma(R.qr[4], dl[2], R.k[0])));
@@ -205,10 +206,11 @@ __device__ inline float eval_split(const RecS<D>& R, const double* dl) {
     const float f0 = (float)dl[0], f1 = (float)dl[1], f2 = (float)dl[2];
     const float e0 = fmaf(R.qi[0], f0, fmaf(R.qi[3], f1, R.qi[4] * f2));
     const float e1 = fmaf(R.qi[1], f1, R.qi[5] * f2);
-    ex = fmaf(f0, e0, fmaf(f1, e1, f2 * (R.qi[2] * f2)));
+    arg = fmaf(-f0, e0, fmaf(-f1, e1, -f2 * (R.qi[2] * f2)));
   }
-  const float fr = (float)(cyc - rint(cyc));
-  return R.amp * __builtin_amdgcn_cosf(fr) * __builtin_amdgcn_exp2f(-ex);
+  // fract(cyc) in [0, 1): the hardware cos takes revolutions
+  const float fr = (float)__builtin_amdgcn_fract(cyc);
+  part = fmaf(R.amp * __builtin_amdgcn_cosf(fr), __builtin_amdgcn_exp2f(arg), part);
 }
 
 template <int D>

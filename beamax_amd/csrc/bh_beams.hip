@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void k_forward_sum(FwdSumArgs A) {
           for (int i = 0; i < SPT; ++i) {
             double dl[D];
             make_delta<D, PER>(sp[i], R.x, A.dom, A.per, dl);
-            part[i] += eval_split<D>(R, dl);
+            eval_split<D>(R, dl, part[i]);
           }
         }
         for (int i = 0; i < SPT; ++i) acc[i] += (double)part[i];
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(256) void k_static_sum(const void* __restrict__ rec
         for (int i = 0; i < PPT; ++i) {
           double dl[D];
           for (int a = 0; a < D; ++a) dl[a] = xp[i][a] - R.x[a];
-          part[i] += eval_split<D>(R, dl);
+          eval_split<D>(R, dl, part[i]);
         }
       }
       for (int i = 0; i < PPT; ++i) acc[i] += (double)part[i];

@@ -227,13 +227,12 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
               part += e1;
               part += e2;
             } else {
-              const float e1 = eval_split<D>(R1, d1), e2 = eval_split<D>(R2, d2);
-              part += e1;
-              part += e2;
+              eval_split<D>(R1, d1, part);
+              eval_split<D>(R2, d2, part);
             }
           } else {
             if constexpr (F64) part += eval_f64<D>(R1, d1);
-            else part += eval_split<D>(R1, d1);
+            else eval_split<D>(R1, d1, part);
           }
         }
       }
