@@ -288,6 +288,7 @@ struct AdjArgs {
   double c;
   double guard;
   int swap;
+  int time_reversal;   // 1: compute_TR_parameters only (no B^-1, no exclusion)
   bh_static_beams out;
 };
 
@@ -397,9 +398,9 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
   const double gam = sqrt(fmax(rad, 0.0));
   const bool valid = (rad > 0.0) && (fabs(Om) > 0.0) && (gam > A.guard * fabs(Om));
   const double sgo = (Om > 0.0) ? 1.0 : ((Om < 0.0) ? -1.0 : 0.0);
-  const cd binv = valid ? mkc(0.0, sgo / (2.0 * c * gam)) : mkc(0.0);
+  const cd binv = A.time_reversal ? mkc(1.0) : (valid ? mkc(0.0, sgo / (2.0 * c * gam)) : mkc(0.0));
   if (bad) aT = mkc(0.0);
-  if (bad || !valid) {
+  if (bad || (!valid && !A.time_reversal)) {
     for (int a = 0; a < D; ++a) {
       pT[a] = 1.0;
       for (int b = 0; b < D; ++b) MT[a][b] = (a == b) ? mkc(0.0, 1.0) : mkc(0.0);
@@ -625,11 +626,32 @@ extern "C" int bh_solve_hom_diag(int64_t b, int ndim, const double* x0, const do
   return BH_OK;
 }
 
+static int boundary_params(const bh_tiling* t, const double* data_grid_size, const int64_t* idx,
+                           const double* vals, int64_t k, int normal_axis, double normal_value,
+                           double inward_sign, double c, double guard, int swap_axes, int tr,
+                           bh_static_beams out, bh_stream s);
+
 extern "C" int bh_adjoint_params(const bh_tiling* t, const double* data_grid_size,
                                  const int64_t* idx, const double* vals, int64_t k,
                                  int normal_axis, double normal_value, double inward_sign,
                                  double c, double guard, int swap_axes, bh_static_beams out,
                                  bh_stream s) {
+  return boundary_params(t, data_grid_size, idx, vals, k, normal_axis, normal_value, inward_sign,
+                         c, guard, swap_axes, 0, out, s);
+}
+
+extern "C" int bh_tr_params(const bh_tiling* t, const double* data_grid_size, const int64_t* idx,
+                            const double* vals, int64_t k, int normal_axis, double normal_value,
+                            double inward_sign, double c, int swap_axes, bh_static_beams out,
+                            bh_stream s) {
+  return boundary_params(t, data_grid_size, idx, vals, k, normal_axis, normal_value, inward_sign,
+                         c, 0.0, swap_axes, 1, out, s);
+}
+
+static int boundary_params(const bh_tiling* t, const double* data_grid_size, const int64_t* idx,
+                           const double* vals, int64_t k, int normal_axis, double normal_value,
+                           double inward_sign, double c, double guard, int swap_axes, int tr,
+                           bh_static_beams out, bh_stream s) {
   bh::clear_error();
   if (int rc = bh::check_tiling(t)) return rc;
   if (k < 0) BH_FAIL(BH_ERR_ARG, "bh_adjoint_params: k=%lld", (long long)k);
@@ -652,6 +674,7 @@ extern "C" int bh_adjoint_params(const bh_tiling* t, const double* data_grid_siz
   a.c = c;
   a.guard = guard;
   a.swap = swap_axes;
+  a.time_reversal = tr;
   a.out = out;
   const unsigned g = bh::grid1d(k, 128);
   hipStream_t st = (hipStream_t)s;

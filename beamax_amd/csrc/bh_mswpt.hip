@@ -313,6 +313,52 @@ size_t grid_fft_ws(const bh_tiling* t) {
 
 }  // namespace
 
+namespace {
+// cpos = 0.5 (a + i b / vg), vg = c |2 pi centre / L| of the coefficient's box
+// (forward_solver_utils.py:763-781); optional half mask
+__global__ void k_combine_velocity(TilingDev T, const int32_t* __restrict__ centres, double L0,
+                                   double L1, double L2, double c, const double2* __restrict__ a,
+                                   const double2* __restrict__ b, int half,
+                                   double2* __restrict__ out) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long total = T.coeffs_cumsum[T.num_levels];
+  if (e >= total) return;
+  const int lvl = find_level(T.coeffs_cumsum, T.num_levels, e);
+  long long P = 1;
+  for (int x = 0; x < T.ndim; ++x) P *= T.support[lvl][x];
+  const long long box = T.boxes_cumsum[lvl] + (e - T.coeffs_cumsum[lvl]) / P;
+  const double L[3] = {L0, L1, L2};
+  double n2 = 0.0;
+  for (int x = 0; x < T.ndim; ++x) {
+    const double p = 2.0 * bh::kPi * ((double)centres[box * T.ndim + x] / L[x]);
+    n2 += p * p;
+  }
+  const double vg = c * sqrt(n2);
+  const double2 av = a[e], bv = b[e];
+  double2 r = make_double2(0.5 * (av.x + (-bv.y) / vg), 0.5 * (av.y + bv.x / vg));
+  const long long seg = T.coeffs_cumsum[lvl + 1] - T.coeffs_cumsum[lvl];
+  if (half && (e - T.coeffs_cumsum[lvl]) >= seg / 2) r = make_double2(0.0, 0.0);
+  out[e] = r;
+}
+}  // namespace
+
+extern "C" int bh_mswpt_combine_velocity(const bh_tiling* t, const double* grid_size, double c,
+                                         const double* a, const double* b, int half_mask,
+                                         double* out, bh_stream s) {
+  bh::clear_error();
+  if (int rc = bh::check_tiling(t)) return rc;
+  if (!grid_size || !a || !b || !out) BH_FAIL(BH_ERR_ARG, "bh_mswpt_combine_velocity: null pointer");
+  if (!(c > 0.0)) BH_FAIL(BH_ERR_ARG, "bh_mswpt_combine_velocity: c must be positive");
+  const bh::TilingDev T = bh::to_dev(t);
+  const long long nc = t->coeffs_cumsum[t->num_levels];
+  hipLaunchKernelGGL(k_combine_velocity, dim3(bh::grid1d(nc, 256)), dim3(256), 0, (hipStream_t)s,
+                     T, t->centres, grid_size[0], t->ndim > 1 ? grid_size[1] : 1.0,
+                     t->ndim > 2 ? grid_size[2] : 1.0, c, (const double2*)a, (const double2*)b,
+                     half_mask, (double2*)out);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
 extern "C" int bh_mswpt_setup(const bh_tiling* t, double* sum_gsquare, double* gpacked,
                               bh_stream s) {
   bh::clear_error();

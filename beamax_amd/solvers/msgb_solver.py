@@ -254,10 +254,19 @@ class MSGBSolver:
         return None
 
     # ------------------------------------------------------------------
-    def prepare_forward_beams(self, p0, domain: Domain, wpt: MSWPT):
+    def prepare_forward_beams(self, p0, domain: Domain, wpt: MSWPT, dpdt=None):
         """compute_coefficients('pos_only') -> top_n -> compute_forward_parameters
         (msgb_solver.py:570-633); returns (ForwardBeams, idx, values)."""
-        cpos = wpt.forward(p0, self.input_type, scale=0.5, half_mask=True)
+        if dpdt is None:
+            cpos = wpt.forward(p0, self.input_type, scale=0.5, half_mask=True)
+        else:
+            import torch
+            a = wpt.forward(p0, self.input_type)
+            b = wpt.forward(dpdt, self.input_type)
+            cpos = torch.empty_like(a)
+            _lib.call("bh_mswpt_combine_velocity", C.byref(wpt.tiling),
+                      _lib.dbl_array(domain.grid_size), float(domain.c), _lib.ptr(a), _lib.ptr(b),
+                      1, _lib.ptr(cpos), _lib.stream_handle())
         k = self._effective_top_n(wpt, half_frame=True)
         idx, val = topn_device(cpos, k)
         beams = ForwardBeams(idx.numel(), domain.ndim, wpt.device)
@@ -336,16 +345,25 @@ class MSGBSolver:
         if dpdt is not None:
             nz = (torch.count_nonzero(dpdt).item() if isinstance(dpdt, torch.Tensor)
                   else int(np.count_nonzero(np.asarray(dpdt))))
-            if nz:
-                raise NotImplementedError("dpdt != 0 is not on the GPU path")
+            if nz == 0:
+                dpdt = None   # 0.5 * (a + 1j * 0 / vg) == 0.5 * a exactly
+            elif (dpdt.is_complex() if isinstance(dpdt, torch.Tensor) else np.iscomplexobj(dpdt)):
+                raise NotImplementedError("complex dpdt (msgb_solver.py:635-689) is not on the GPU path")
         if not domain.is_homogeneous:
             raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
         self._check_supported()
-        beams, _, _ = self.prepare_forward_beams(p0, domain, wpt)
+        beams, _, _ = self.prepare_forward_beams(p0, domain, wpt, dpdt=dpdt)
         data = self.sum_forward(beams, domain, pos, ts)
         out_shape = (data.shape[0],) + tuple(np.asarray(_np(sensors.positions) if isinstance(sensors, Sensor) else pos).shape[:-1])
         data = data.reshape(out_shape)
         return data, self._params_tuple(beams)
+
+    def solve_ivp(self, p0, dpdt, domain: Domain, wpt: MSWPT, sensors, ts):
+        """Wave-equation IVP with initial velocity (msgb_solver.py:1113-1147)."""
+        return self.forward(p0, domain, sensors, ts, wpt, dpdt=dpdt)
+
+    def solve_ivp_with_params(self, p0, dpdt, domain: Domain, wpt: MSWPT, sensors, ts):
+        return self.forward_with_params(p0, domain, sensors, ts, wpt, dpdt=dpdt)
 
     def _params_tuple(self, beams: ForwardBeams):
         """Mirrored parameters in the reference's tuple order (batched for scan/vmap)."""
@@ -468,6 +486,53 @@ class MSGBSolver:
                   _lib.PRECISION[self.precision], _lib.ptr(out), _lib.ptr(ws), ws.numel(),
                   _lib.stream_handle())
         return out
+
+    def prepare_tr_beams(self, data, sources: Sensor, data_domain: Domain, data_wpt: MSWPT,
+                         c: float, swap_axes: bool = False, plane=None) -> StaticBeams:
+        """compute_coefficients('both') (the cpos half, = 0.5 * MSWPT(data)) ->
+        top_n -> compute_TR_parameters -> solve_hom_TR to t = 0
+        (msgb_solver.py:691-746)."""
+        coeffs = data_wpt.forward(data, self.input_type, scale=0.5)
+        idx, val = topn_device(coeffs, self._effective_top_n(data_wpt))
+        if plane is None:
+            plane = detector_plane(sources.positions, sources.domain)
+        normal_axis, normal_value, inward = plane
+        beams = StaticBeams(idx.numel(), sources.domain.ndim, data_wpt.device)
+        _lib.call("bh_tr_params", C.byref(data_wpt.tiling),
+                  _lib.dbl_array(data_domain.grid_size), _lib.ptr(idx), _lib.ptr(val), idx.numel(),
+                  normal_axis, normal_value, inward, float(c), int(swap_axes), beams.struct(),
+                  _lib.stream_handle())
+        beams.idx, beams.val, beams.coeffs = idx, val, coeffs
+        return beams
+
+    def time_reversal(self, data, domain: Domain, sensors, sources: Sensor, ts,
+                      data_domain: Domain, data_wpt: MSWPT):
+        """MSGB time-reversal reconstruction, times 2 (msgb_solver.py:984-1111)."""
+        q, _ = self.time_reversal_with_params(data, domain, sensors, sources, ts, data_domain,
+                                              data_wpt)
+        return q
+
+    def time_reversal_with_params(self, data, domain: Domain, sensors, sources: Sensor, ts,
+                                  data_domain: Domain, data_wpt: MSWPT):
+        import torch
+        if any(domain.periodic):
+            raise ValueError("The MSGB time reversal solver only supports free space boundary "
+                             "conditions.")
+        pos = self._sensor_positions(sensors)
+        if pos is None:
+            raise ValueError("Unsupported sensor type")
+        self._validate_boundary_data(data, data_domain, sources, ts=ts)
+        if not sources.domain.is_homogeneous:
+            raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
+        if self.thr_strat != "top_n" or self.sharding is not None:
+            self._check_supported()
+        if not isinstance(data, torch.Tensor):
+            data = torch.as_tensor(np.asarray(data, dtype=np.float64), device=data_wpt.device)
+        beams = self.prepare_tr_beams(data.to(data_wpt.device).to(torch.float64), sources,
+                                      data_domain, data_wpt, float(sources.domain.c))
+        q = self.sum_static(beams, pos, 2.0, dx=domain.dx)
+        shape = np.asarray(sensors.positions if isinstance(sensors, Sensor) else pos).shape[:-1]
+        return q.reshape(tuple(shape)), beams
 
     def adjoint(self, data, domain: Domain, sensors, sources: Sensor, ts, data_domain: Domain,
                 data_wpt: MSWPT, *, window=None):

@@ -30,6 +30,10 @@
  *   bh_adjoint_params   compute_adj_parameters + compute_TR_parameters +
  *                       solve_hom_TR at t = 0 (adjoint_solver_utils.py:75-220,
  *                       tr_solver_utils.py:205-435, gb_solvers.py:484-594)
+ *   bh_tr_params        compute_TR_parameters + solve_hom_TR at t = 0 for
+ *                       MSGBSolver.time_reversal (msgb_solver.py:691-746, 984-1111)
+ *   bh_mswpt_combine_velocity  compute_coefficients with dpdt != 0
+ *                       (forward_solver_utils.py:755-786)
  *   bh_fourier_crop_time  prepare_data_domain_for_msgb Fourier crop
  *                       (examples/reconstruction/2d_time_reversal_and_adjoint.py:111-165)
  *   bh_forward_sum_culled, bh_static_sum_culled: the same sums with a certified
@@ -130,6 +134,12 @@ size_t bh_mswpt_workspace_size(const bh_tiling* t);
 int bh_mswpt_forward(const bh_tiling* t, const double* sum_gsquare, const double* gpacked,
                      const double* input, int input_kind, double scale, int half_mask,
                      double* coeffs, void* ws, size_t ws_bytes, bh_stream s);
+/* cpos = 0.5 (a + i b / vg) for the coefficients a = MSWPT(p0),
+ * b = MSWPT(dpdt); vg = c |2 pi centre / L| of each coefficient's box; with
+ * half_mask the second half of every level is zeroed ('pos_only'). */
+int bh_mswpt_combine_velocity(const bh_tiling* t, const double* grid_size, double c,
+                              const double* a, const double* b, int half_mask, double* out,
+                              bh_stream s);
 /* out[prod N] complex = MSWPT^-1(coeffs) in the spatial (output_fourier = 0)
  * or centred Fourier (1) domain. */
 int bh_mswpt_inverse(const bh_tiling* t, const double* gpacked, const double* coeffs,
@@ -189,6 +199,13 @@ int bh_adjoint_params(const bh_tiling* t, const double* data_grid_size, const in
                       const double* vals, int64_t k, int normal_axis, double normal_value,
                       double inward_sign, double c, double guard, int swap_axes,
                       bh_static_beams out, bh_stream s);
+
+/* Time-reversal beams: compute_TR_parameters + solve_hom_TR to t = 0
+ * (no B^-1 factor and no evanescent exclusion); same arguments as
+ * bh_adjoint_params without the guard. */
+int bh_tr_params(const bh_tiling* t, const double* data_grid_size, const int64_t* idx,
+                 const double* vals, int64_t k, int normal_axis, double normal_value,
+                 double inward_sign, double c, int swap_axes, bh_static_beams out, bh_stream s);
 
 /* out[P] = scale * sum_b field_b(points[p]) for beams frozen at one time. */
 size_t bh_static_sum_workspace_size(int64_t count, int ndim);

@@ -22,7 +22,9 @@ follows):
   ``solve_hom_diag`` / ``solve_hom_TR`` and the real beam sums
   (``beamax/gb/*.py``, ``forward_solver_utils.py``).  Pinned by frame atom ==
   Gaussian beam at t=0, closed form == numerically integrated Riccati ODE,
-  real sum == GB + conj(GB), and a brute-force double loop.
+  real sum == GB + conj(GB), a brute-force double loop, and (for the
+  initial-velocity coefficients) the reference's c+ == +-c- symmetries,
+  its 0.5i b / H known answer and its mirror-box conjugacy.
 * ``topn`` -- the ``top_n`` selection rule (``forward_solver_utils.py:65-89``)
   in two independent formulations (an emulation of ``jnp.argpartition`` built
   on ``lax.top_k`` tie semantics, and a lexicographic sort), checked against
@@ -31,7 +33,9 @@ follows):
   spatial Hessian conversion, principal B^-1 symbol and the TR sum
   (``msgb_solver.py``, ``tr_solver_utils.py``, ``adjoint_solver_utils.py``).
   Pinned by the reference's known answers for B^-1, the spectral-derivative
-  closed form, inward normals, tangential phase and ODE agreement.
+  closed form, inward normals, tangential phase and ODE agreement.  The
+  time-reversal pipeline composes these pinned pieces (0.5 MSWPT, top_n,
+  TR parameters, solve_hom_TR, TR sum, times 2) and adds no new arithmetic.
 * ``cpu_beamsum.cpp`` -- the same forward/adjoint real beam sums in C++17 with
   OpenMP, FP64, threads owning disjoint output columns.  Used as the CPU
   baseline (kind "port") and as a fast checker for larger cases; itself

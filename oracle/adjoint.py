@@ -188,3 +188,18 @@ def adjoint_pipeline(data, dt, data_wpt: MSWPT, data_N, data_dx, src_positions, 
     x, p, M, A = solve_hom_TR(xts, pts, Mts, ats, signum, t_start, c)
     q = tr_field_sum(x, p, M, A, om, np.asarray(points, float))
     return q / (c ** 2), (idx, val, x, p, M, A, om)
+
+
+def time_reversal_pipeline(data, data_wpt: MSWPT, data_N, data_dx, src_positions, src_N, src_dx,
+                           c, points, thr):
+    """MSGBSolver.time_reversal (msgb_solver.py:691-746, 984-1111) with solve_hom_TR:
+    cpos = 0.5 MSWPT(data) (dpdt = 0), top_n over all coefficients,
+    compute_TR_parameters, TR field at t = 0, times 2."""
+    from .topn import top_n
+    cpos = 0.5 * (data_wpt.forward(np.asarray(data, float)) + 1j * 0.0)
+    idx, val = top_n(cpos, min(int(thr), data_wpt.total_coeffs))
+    pts, Mts, xts, om, ats, signum, t_start, _, _ = tr_parameters(
+        data_wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c)
+    x, p, M, A = solve_hom_TR(xts, pts, Mts, ats * val, signum, t_start, c)
+    q = tr_field_sum(x, p, M, A, om, np.asarray(points, float))
+    return 2.0 * q, (idx, val, x, p, M, A, om)

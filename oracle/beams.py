@@ -67,6 +67,23 @@ def compute_coefficients_pos_only(wpt: MSWPT, p0, c, grid_size, input_type="spat
     return 0.5 * (a + 1j * np.zeros_like(a)) * wpt.half_mask
 
 
+def compute_coefficients(wpt: MSWPT, p0, dpdt, c, grid_size, mode="pos_only",
+                         input_type="spatial"):
+    """forward_solver_utils.py:716-786 in full: cpos = 0.5 (a + i b / vg),
+    cneg = 0.5 (a - i b / vg), vg = c |p_b| with p_b = 2 pi centre / L."""
+    a = wpt.forward(p0, input_type)
+    b = wpt.forward(dpdt, input_type)
+    t = wpt.tiling
+    lvl, multi = decode_indices(wpt, np.arange(wpt.total_coeffs))
+    box = multi[0] + t.boxes_cumsum[lvl]
+    pb = 2 * np.pi * t.centres[box] / np.asarray(grid_size, dtype=float)
+    vg = c * np.linalg.norm(pb, axis=-1)
+    cpos = 0.5 * (a + 1j * b / vg)
+    if mode == "pos_only":
+        return cpos * wpt.half_mask
+    return cpos, 0.5 * (a - 1j * b / vg)
+
+
 # ----------------------------------------------------------------------------
 # closed-form homogeneous propagation (gb_solvers.py)
 # ----------------------------------------------------------------------------

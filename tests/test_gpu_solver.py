@@ -212,3 +212,72 @@ def test_fourier_crop_matches_reference_recipe(gpu):
         mid = nt // 2
         ref = np.fft.ifftn(np.fft.ifftshift(F[mid - 32: mid + 32]), norm="ortho").real
         assert rel(ref, got) < 1e-12
+
+
+def test_time_reversal_matches_oracle(gpu):
+    import torch
+    rng = np.random.default_rng(21)
+    n = 64
+    dom = bx.Domain(N=(n, n), dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    nt = 2 * n
+    ts = np.linspace(0, dom.generate_time_domain()[-1], nt)
+    dt = ts[1] - ts[0]
+    mask = np.zeros((n, n))
+    mask[0, :] = 1
+    sources = bx.Sensor(dom, binary_mask=mask)
+    ddom = bx.Domain(N=(nt, n), dx=(dt, 1e-4), c=1500.0, periodic=(False, False))
+    dw = bx.MSWPT(bx.DyadicDecomposition(2, (nt, n), (4, 8), (2, 1)), 2, "rectangular_mirror")
+    data = _line_data(rng, nt, n, dt)
+    img = bx.Sensor(dom, binary_mask=np.ones((n, n)))
+    s = bx.MSGBSolver(thr=2500, thr_strat="top_n", batch_size=64, input_type="spatial",
+                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64",
+                      cull_log2=0.0)
+    q, beams = s.time_reversal_with_params(torch.as_tensor(data, device=gpu), dom, img, sources, ts,
+                                           ddom, dw)
+    ow = OMSWPT(Tiling(2, (nt, n), (4, 8), (2, 1)), 2, "rectangular_mirror")
+    # oracle on the GPU's own selection (conjugate-pair order at rank N is unpinned)
+    cref = 0.5 * ow.forward(data)
+    assert rel(cref, _n(beams.coeffs)) < 1e-12
+    ridx, _ = otopn.top_n(cref, 2500)
+    assert len(set(ridx.tolist()) ^ set(_n(beams.idx).tolist())) <= 2
+    gidx, gval = _n(beams.idx), _n(beams.val)
+    pts, Mts, xts, om, ats, signum, t_start, _, _ = oadj.tr_parameters(
+        ow, gidx, (nt, n), (dt, 1e-4), sources.positions, (n, n), (1e-4, 1e-4), 1500.0)
+    x, p, M, A = obeams.solve_hom_TR(xts, pts, Mts, ats * gval, signum, t_start, 1500.0)
+    assert rel(M, _n(beams.M)) < 1e-12 and rel(A, _n(beams.A)) < 1e-12
+    qref = 2.0 * oadj.tr_field_sum(x, p, M, A, om, img.positions)
+    assert rel(qref, _n(q).ravel()) < 1e-11
+    s32 = bx.MSGBSolver(thr=2500, thr_strat="top_n", batch_size=64, input_type="spatial",
+                        ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real")
+    q32 = s32.time_reversal(torch.as_tensor(data, device=gpu), dom, img, sources, ts, ddom, dw)
+    assert rel(qref, _n(q32).ravel()) < 1e-5
+
+
+def test_solve_ivp_with_velocity_matches_oracle(gpu):
+    N = (64, 64)
+    rng = np.random.default_rng(22)
+    dom = bx.Domain(N=N, dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    dd = bx.DyadicDecomposition(2, N, (4, 8), (1, 1))
+    wpt = bx.MSWPT(dd, 2, "rectangular_mirror")
+    p0 = rng.standard_normal(N)
+    dpdt = rng.standard_normal(N) * 1e7
+    ts = dom.generate_time_domain()[:80]
+    mask = np.zeros(N)
+    mask[0, :] = 1
+    sensors = bx.Sensor(dom, binary_mask=mask)
+    s = bx.MSGBSolver(thr=700, thr_strat="top_n", batch_size=64, input_type="spatial",
+                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64",
+                      cull_log2=0.0)
+    got = _n(s.solve_ivp(p0, dpdt, dom, wpt, sensors, ts))
+    ow = OMSWPT(Tiling(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror")
+    L = dom.grid_size
+    cpos = obeams.compute_coefficients(ow, p0, dpdt, 1500.0, L)
+    idx, val = otopn.top_n(cpos, 700)
+    p, al, x0, om, a0, mode = obeams.forward_parameters(ow, idx, L, N)
+    dup = lambda v: np.concatenate([v, v])
+    ref = ocpu.forward_sum(dup(x0), dup(p), dup(al), dup(a0 * val), dup(om),
+                           np.concatenate([mode, -mode]), 1500.0, ts, sensors.positions)
+    assert rel(ref, got) < 1e-12
+    # test_msgb_solver_misc.py:253-270: a zero velocity reproduces forward()
+    z = _n(s.solve_ivp(p0, np.zeros(N), dom, wpt, sensors, ts))
+    assert np.array_equal(z, _n(s.forward(p0, dom, sensors, ts, wpt)))

@@ -324,3 +324,58 @@ def test_tr_temporal_carrier_sets_tangential_phase():
     assert np.any(grazing_free)
     assert np.allclose(om[grazing_free], np.abs(ch[grazing_free, 0]))
     assert np.allclose(om[grazing_free] * pts[grazing_free, 1], 2 * np.pi * ch[grazing_free, 1])
+
+
+# ---------------------------------------------------------------------------
+# coefficients with an initial velocity (tests/solvers/test_solver_utils.py:24-293)
+# ---------------------------------------------------------------------------
+def _velocity_case(windowing="rectangular", N=(128, 64), seed=0):
+    t = Tiling(2, N, (4, 8), (1, 1))
+    w = MSWPT(t, 2, windowing)
+    L = np.asarray(N) * 1e-4
+    rng = np.random.default_rng(seed)
+    return w, L, rng
+
+
+def test_velocity_coefficients_split_by_symmetry():
+    """test_solver_utils.py:24-90: dpdt = 0 gives c+ == c-; p0 = 0 gives c+ == -c-."""
+    w, L, rng = _velocity_case()
+    p0 = rng.standard_normal(w.tiling.N)
+    cp, cn = obeams.compute_coefficients(w, p0, np.zeros_like(p0), 1500.0, L, mode="both")
+    assert np.array_equal(cp, cn)
+    cp, cn = obeams.compute_coefficients(w, np.zeros_like(p0), p0, 1500.0, L, mode="both")
+    assert np.allclose(cp, -cn, rtol=0, atol=0) and np.max(np.abs(cp)) > 0
+
+
+def test_velocity_coefficients_known_answer():
+    """test_solver_utils.py:185-228 (homogeneous speed 2): p0 = 0 and an arange
+    dpdt give c+ = 0.5i b / H with H = c |2 pi centre / L|."""
+    N, dx = (32, 16), np.array([0.1, 0.3])
+    t = Tiling(1, N, (4,), (2, 1))
+    for rho in (1, 2):
+        w = MSWPT(t, rho, "rectangular")
+        L = np.asarray(N) * dx
+        dpdt = np.arange(N[0] * N[1], dtype=float).reshape(N) / (N[0] * N[1])
+        cp, _ = obeams.compute_coefficients(w, np.zeros(N), dpdt, 2.0, L, mode="both")
+        b = w.forward(dpdt)
+        lvl, multi = obeams.decode_indices(w, np.arange(w.total_coeffs))
+        cen = t.centres[multi[0] + t.boxes_cumsum[lvl]] / L
+        H = 2.0 * np.linalg.norm(2 * np.pi * cen, axis=1)
+        assert np.allclose(cp, 0.5j * b / H, rtol=1e-11, atol=1e-11)
+
+
+def test_velocity_coefficients_conjugate_across_mirror_boxes():
+    """test_solver_utils.py:231-293: real p0, dpdt give c+[box] == conj(c-[mirror box])
+    at the same packet index (rectangular_mirror)."""
+    w, L, rng = _velocity_case("rectangular_mirror", seed=10)
+    p0 = rng.standard_normal(w.tiling.N)
+    dpdt = rng.standard_normal(w.tiling.N)
+    cp, cn = obeams.compute_coefficients(w, p0, dpdt, 1500.0, L, mode="both")
+    t = w.tiling
+    for lvl in range(t.num_levels):
+        shp = tuple(w.coeff_shapes[lvl])
+        a = cp[w.coeffs_cumsum[lvl]:w.coeffs_cumsum[lvl + 1]].reshape(shp)
+        b = cn[w.coeffs_cumsum[lvl]:w.coeffs_cumsum[lvl + 1]].reshape(shp)
+        nb = shp[0]
+        for box in range(nb):
+            assert np.allclose(a[box], np.conj(b[nb - 1 - box]), rtol=1e-10, atol=1e-12)
