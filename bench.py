@@ -33,6 +33,12 @@ METRIC = ("Gbeam-evals/s MSGB fwd+adj (512², err<1e-5 vs FP64); "
           "recon s to 1e-3 resid")
 FLOP_PER_EVAL_2D = 25.0
 FP32_VALU_PEAK_TFLOPS = 157.3
+# HBM bytes per forward launch of the culled FP32-split kernel on this
+# workload at cull_log2 = 40: rocprofv3 FETCH_SIZE (244467.1 KiB raw, x2.0)
+# + WRITE_SIZE (77707.3 KiB raw, x1.0), both scaled by what the counters read
+# for a 512 MiB copy (profiles/k_culled_sum_msgb2d_512.md).  Counters cannot
+# be read from inside the process, so other settings report null.
+MEASURED_FWD_TRAFFIC = {40.0: 244467.125 * 2.0 * 1024 + 77707.34375 * 1.0 * 1024}
 
 
 def build_problem(device, cull_log2=40.0):
@@ -247,7 +253,8 @@ def main():
             "frac": round(fwd_exec * FLOP_PER_EVAL_2D / (fwd_ms * 1e-3) / 1e12
                           / FP32_VALU_PEAK_TFLOPS, 4),
             "dense_equivalent_tflops": round(fwd_tflops, 3),
-            "traffic": None,
+            "traffic": MEASURED_FWD_TRAFFIC.get(float(args.cull_log2)),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "flop_per_eval": FLOP_PER_EVAL_2D,
             "evals_per_launch": fwd_exec,
             "note": ("achieved counts executed lane evaluations of the launch (64 per "
