@@ -95,6 +95,13 @@ SIGNATURES = {
     "bh_static_sum_culled_workspace_size": (_SZ, [_I64, C.c_int, _I64]),
     "bh_static_sum_culled": (C.c_int, [C.POINTER(StaticBeams), C.c_int, C.POINTER(Segs),
                                        C.c_double, C.c_double, C.c_int, _P, _P, _P, _SZ, _P]),
+    "bh_forward_transpose_workspace_size": (_SZ, [_I64, _I64, _I64]),
+    "bh_forward_transpose": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, C.c_double, _P, _I64,
+                                       C.POINTER(Segs), C.c_double, C.c_int, _P, C.c_int, _P, _P,
+                                       _SZ, _P]),
+    "bh_coeff_scatter": (C.c_int, [_P, _P, _I64, C.c_double, _I64, _P, _P]),
+    "bh_coeff_gather": (C.c_int, [_P, _I64, _P, _I64, _P, _P]),
+    "bh_mswpt_adjoint": (C.c_int, [C.POINTER(Tiling), _P, _P, _P, C.c_int, _P, _P, _SZ, _P]),
     "bh_static_sum_workspace_size": (_SZ, [_I64, C.c_int]),
     "bh_static_sum": (C.c_int, [C.POINTER(StaticBeams), C.c_int, _P, _I64, C.c_double, C.c_int,
                                 _P, _P, _SZ, _P]),

@@ -177,12 +177,12 @@ __device__ inline void make_delta(const double* s, const double* x, const double
   }
 }
 
-// part += 2|A| cos(2 pi cyc) 2^-ex
+// FP32-split phase of one term: fr = fract(cycles) with
+// cycles = sum_a dl_a (k_a + sum_b Q_ab dl_b) + ph0 formed in FP64 (Q
+// symmetric), and the envelope exponent arg (log2 units) in FP32.
 template <int D>
-__device__ inline void eval_split(const RecS<D>& R, const double* dl, float& part) {
-  // cycles = sum_a dl_a (k_a + sum_b Q_ab dl_b) + ph0, Q symmetric
+__device__ inline void split_phase(const RecS<D>& R, const double* dl, float& fr, float& arg) {
   double cyc = R.ph0;
-  float arg;
   if constexpr (D == 1) {
     cyc = fma(dl[0], fma(R.qr[0], dl[0], R.k[0]), cyc);
     const float f0 = (float)dl[0];
@@ -208,9 +208,28 @@ __device__ inline void eval_split(const RecS<D>& R, const double* dl, float& par
     const float e1 = fmaf(R.qi[1], f1, R.qi[5] * f2);
     arg = fmaf(-f0, e0, fmaf(-f1, e1, -f2 * (R.qi[2] * f2)));
   }
-  // fract(cyc) in [0, 1): the hardware cos takes revolutions
-  const float fr = (float)__builtin_amdgcn_fract(cyc);
+  // fract(cyc) in [0, 1): the hardware cos/sin take revolutions
+  fr = (float)__builtin_amdgcn_fract(cyc);
+}
+
+// part += 2|A| cos(2 pi cyc) 2^-ex
+template <int D>
+__device__ inline void eval_split(const RecS<D>& R, const double* dl, float& part) {
+  float fr, arg;
+  split_phase<D>(R, dl, fr, arg);
   part = fmaf(R.amp * __builtin_amdgcn_cosf(fr), __builtin_amdgcn_exp2f(arg), part);
+}
+
+// (pr, pi) += r 2|A| 2^-ex (cos, -sin)(2 pi cyc): r times the conjugate of the
+// complex term whose real part eval_split sums
+template <int D>
+__device__ inline void eval_split_conj(const RecS<D>& R, const double* dl, float r, float& pr,
+                                       float& pi) {
+  float fr, arg;
+  split_phase<D>(R, dl, fr, arg);
+  const float m = (r * R.amp) * __builtin_amdgcn_exp2f(arg);
+  pr = fmaf(m, __builtin_amdgcn_cosf(fr), pr);
+  pi = fmaf(-m, __builtin_amdgcn_sinf(fr), pi);
 }
 
 template <int D>
@@ -232,6 +251,32 @@ __device__ inline double eval_f64(const RecD<D>& R, const double* dl) {
     }
   const double ph = R.w * (lin + 0.5 * q) + R.ang;
   return R.amp * cos(ph) * exp(-R.w * (0.5 * chi));
+}
+
+template <int D>
+__device__ inline void eval_f64_conj(const RecD<D>& R, const double* dl, double r, double& pr,
+                                     double& pi) {
+  double lin = 0.0;
+  for (int a = 0; a < D; ++a) lin += R.p[a] * dl[a];
+  double q = 0.0, chi = 0.0;
+  int idx = 0;
+  for (int a = 0; a < D; ++a) {
+    q += R.mr[idx] * dl[a] * dl[a];
+    chi += R.mi[idx] * dl[a] * dl[a];
+    ++idx;
+  }
+  for (int a = 0; a < D; ++a)
+    for (int b = a + 1; b < D; ++b) {
+      q += R.mr[idx] * dl[a] * dl[b];
+      chi += R.mi[idx] * dl[a] * dl[b];
+      ++idx;
+    }
+  const double ph = R.w * (lin + 0.5 * q) + R.ang;
+  const double m = r * R.amp * exp(-R.w * (0.5 * chi));
+  double sn, cs;
+  sincos(ph, &sn, &cs);
+  pr += m * cs;
+  pi -= m * sn;
 }
 
 template <int D, bool F64>

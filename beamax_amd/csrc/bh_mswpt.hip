@@ -222,9 +222,12 @@ __global__ void k_scale_mask(TilingDev T, double2* __restrict__ c, double scale,
 
 // Synthesis gather (transforms.py:735-850): ft[m] = sum over boxes (box order)
 // of g[jj] * fftshift(fftn(c_b))[(jj + r) mod S], jj = (m - start_b) mod N < S.
+// With sg != nullptr the sum is divided by sum_g2[m]: the adjoint of the
+// analysis gather (which divides by it before windowing).
 __global__ void k_mswpt_synth(TilingDev T, const int32_t* __restrict__ centres,
                               const double2* __restrict__ Cf, const double* __restrict__ gp,
-                              double2* __restrict__ ft, long long total) {
+                              const double* __restrict__ sg, double2* __restrict__ ft,
+                              long long total) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= total) return;
   const int d = T.ndim;
@@ -262,7 +265,18 @@ __global__ void k_mswpt_synth(TilingDev T, const int32_t* __restrict__ centres,
       im += g * v.y;
     }
   }
+  if (sg) {
+    const double q = sg[i];
+    re /= q;
+    im /= q;
+  }
   ft[i] = make_double2(re, im);
+}
+
+__global__ void k_real_part(const double2* __restrict__ in, double* __restrict__ out,
+                            long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i].x;
 }
 
 // batched fftn over the box axes of every level, in place
@@ -453,7 +467,7 @@ extern "C" int bh_mswpt_inverse(const bh_tiling* t, const double* gpacked,
   if (int rc = level_ffts(T, C, -1, fws, fws_bytes, st)) return rc;
   double2* dst = output_fourier ? (double2*)out : G;
   hipLaunchKernelGGL(k_mswpt_synth, dim3(bh::grid1d(total, 256)), dim3(256), 0, st, T,
-                     t->centres, C, gpacked, dst, total);
+                     t->centres, C, gpacked, (const double*)nullptr, dst, total);
   BH_LAUNCH_CHECK();
   if (!output_fourier) {
     hipLaunchKernelGGL(k_shift_copy, dim3(bh::grid1d(total, 256)), dim3(256), 0, st, T, G,
@@ -461,5 +475,43 @@ extern "C" int bh_mswpt_inverse(const bh_tiling* t, const double* gpacked,
     BH_LAUNCH_CHECK();
     if (int rc = bh::fftn_ortho((double2*)out, t->ndim, t->N, +1, fws, fws_bytes, st)) return rc;
   }
+  return BH_OK;
+}
+
+extern "C" int bh_mswpt_adjoint(const bh_tiling* t, const double* sum_gsquare,
+                                const double* gpacked, const double* coeffs, int output_real,
+                                double* out, void* ws, size_t ws_bytes, bh_stream s) {
+  bh::clear_error();
+  if (int rc = bh::check_tiling(t)) return rc;
+  if (!sum_gsquare || !gpacked || !coeffs || !out || !ws)
+    BH_FAIL(BH_ERR_ARG, "bh_mswpt_adjoint: null pointer");
+  const size_t need = bh_mswpt_workspace_size(t);
+  if (ws_bytes < need) BH_FAIL(BH_ERR_WORKSPACE, "bh_mswpt_adjoint: workspace %zu < %zu",
+                               ws_bytes, need);
+  const bh::TilingDev T = bh::to_dev(t);
+  hipStream_t st = (hipStream_t)s;
+  const long long total = bh::prod(t->N, t->ndim);
+  const int64_t nc = t->coeffs_cumsum[t->num_levels];
+  const int64_t big = total > nc ? total : nc;
+  double2* C = (double2*)ws;
+  double2* G = C + big;
+  void* fws = (char*)ws + (size_t)(2 * big) * sizeof(double2);
+  const size_t fws_bytes = ws_bytes - (size_t)(2 * big) * sizeof(double2);
+  BH_HIP(hipMemcpyAsync(C, coeffs, nc * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  if (int rc = level_ffts(T, C, -1, fws, fws_bytes, st)) return rc;
+  hipLaunchKernelGGL(k_mswpt_synth, dim3(bh::grid1d(total, 256)), dim3(256), 0, st, T,
+                     t->centres, C, gpacked, sum_gsquare, G, total);
+  BH_LAUNCH_CHECK();
+  // G (centred) -> C (unshifted) -> unitary inverse FFT in place
+  hipLaunchKernelGGL(k_shift_copy, dim3(bh::grid1d(total, 256)), dim3(256), 0, st, T, G, C,
+                     total);
+  BH_LAUNCH_CHECK();
+  if (int rc = bh::fftn_ortho(C, t->ndim, t->N, +1, fws, fws_bytes, st)) return rc;
+  if (output_real) {
+    hipLaunchKernelGGL(k_real_part, dim3(bh::grid1d(total, 256)), dim3(256), 0, st, C, out, total);
+  } else {
+    BH_HIP(hipMemcpyAsync(out, C, total * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  }
+  BH_LAUNCH_CHECK();
   return BH_OK;
 }

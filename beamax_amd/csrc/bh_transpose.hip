@@ -1,0 +1,298 @@
+// Exact transpose of the frozen-index forward beam sum.
+//
+// With the beam set frozen, the real forward sum is linear in the beam
+// coefficients c_k:  u(t, s) = sum_k sum_{mode} 2 Re(c_k G_k(t, s)), where
+// G_k = A_k(t) exp(i w_k (p.delta + delta^T M delta / 2)) is the Gaussian beam
+// of unit coefficient (A_k(t) from the closed-form propagator with a0 of the
+// beam table).  Its transpose for the real inner products <u, r> and
+// Re<c, g> is
+//     g_k = sum_{mode} sum_{t, s} 2 r(t, s) conj(G_k(t, s)),
+// which this file evaluates.  Work unit: a tile of 64 time rows (one per
+// lane) x up to 32 point groups; a wave takes one beam at a time, builds the
+// 64 records of its tile (lane = time row), finds per row the groups inside
+// the certified envelope (the same closed-form run intervals as the culled
+// forward sum), then walks the active (row, group) pairs with the lanes over
+// the group's 64 points, and reduces its lanes once per (beam, tile).  Every
+// (tile, beam) slot is written exactly once and the slots are summed in a
+// fixed order, so the result is deterministic without atomics.
+#include <type_traits>
+
+#include "bh_segcull.hpp"
+
+namespace {
+
+constexpr int kTileT = 64;   // time rows per tile (one per lane)
+constexpr int kTileG = 32;   // 64-point groups per tile (bits of a mask word)
+
+struct TArgs {
+  bh_fwd_beams b;
+  long long count;
+  int modes;             // 2: mirrored pair (+c, -c) per beam
+  double c;
+  const double* ts;
+  long long nt;
+  SegDev G;
+  double E;              // cull exponent (log2 units); <= 0: every pair
+  const void* resid;     // (nt, S) row-major, float or double
+  int resid_f64;
+  double* slots;         // [ntt * ntg][count] complex
+  long long ntg;
+};
+
+__device__ inline void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int D, bool F64>
+__global__ __launch_bounds__(256) void k_fwd_transpose(TArgs A) {
+  using Rec = typename RecOf<D, F64>::type;
+  using Part = typename std::conditional<F64, double, float>::type;
+  __shared__ Rec rec[4][kTileT];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const long long t0 = (long long)blockIdx.y * kTileT;
+  const long long g0 = (long long)blockIdx.z * kTileG;
+  const long long ngl = A.G.ngroups - g0;
+  const int ng = (int)(ngl < kTileG ? ngl : kTileG);
+  const long long tl = t0 + lane;
+  const bool tvalid = tl < A.nt;
+  const double tv = tvalid ? A.ts[tl] : 0.0;
+  const long long tile = (long long)blockIdx.y * A.ntg + blockIdx.z;
+  const long long nw = (long long)gridDim.x * 4;
+  const long long S = A.G.S;
+  for (long long k = (long long)blockIdx.x * 4 + wv; k < A.count; k += nw) {
+    double x0[D], p0[D];
+    cd al[D];
+    for (int a = 0; a < D; ++a) {
+      x0[a] = A.b.x0[k * D + a];
+      p0[a] = A.b.p0[k * D + a];
+      al[a] = mkc(A.b.alpha[2 * (k * D + a)], A.b.alpha[2 * (k * D + a) + 1]);
+    }
+    const cd a0 = mkc(A.b.a0[2 * k], A.b.a0[2 * k + 1]);
+    const double w = A.b.omega[k];
+    double acc_re = 0.0, acc_im = 0.0;
+    for (int md = 0; md < A.modes; ++md) {
+      Rec R;
+      bool live = tvalid;
+      if (live) {
+        State<D> St;
+        hom_diag_state_fast<D>(x0, p0, al, a0, md ? -A.c : A.c, tv, St);
+        make_rec<D>(St, w, R);
+        live = !rec_zero<D>(R);
+      }
+      if (!live) zero_rec<D>(R);
+      // groups of this tile inside the envelope at this lane's time row
+      unsigned msk = 0;
+      if (live) {
+        double Q[D][D];
+        if (A.E > 0.0) env_matrix<D>(R, Q);
+        long long cur = -1;
+        int lo = 1, hi = 0;
+        for (int gl = 0; gl < ng; ++gl) {
+          bool act = true;
+          if (A.E > 0.0) {
+            const long long g = g0 + gl;
+            const int ks = A.G.grp_seg[g];
+            if (ks != cur) {
+              seg_interval<D>(A.G, ks, R.x, Q, A.E, lo, hi);
+              cur = ks;
+            }
+            const int j0 = (int)((g - A.G.g0[ks]) * 64);
+            act = (hi >= lo) && (hi >= j0) && (lo <= j0 + 63);
+          }
+          if (act) msk |= 1u << gl;
+        }
+      }
+      wave_lds_fence();   // the previous mode's rows are consumed
+      rec[wv][lane] = R;
+      wave_lds_fence();   // every lane's record is visible to the wave
+      for (int j = 0; j < kTileT; ++j) {
+        unsigned m = __builtin_amdgcn_readlane(msk, j);
+        if (!m) continue;
+        const Rec Rj = rec[wv][j];
+        const long long t = t0 + j;
+        Part pr = 0, pi = 0;
+        while (m) {
+          const int gl = __builtin_ctz(m);
+          m &= m - 1;
+          const long long slot = (g0 + gl) * 64 + lane;
+          double dl[D];
+          for (int a = 0; a < D; ++a) dl[a] = A.G.pts[slot * D + a] - Rj.x[a];
+          const int col = A.G.col[slot];
+          double r = 0.0;
+          if (col >= 0)
+            r = A.resid_f64 ? ((const double*)A.resid)[t * S + col]
+                            : (double)((const float*)A.resid)[t * S + col];
+          if constexpr (F64) eval_f64_conj<D>(Rj, dl, r, pr, pi);
+          else eval_split_conj<D>(Rj, dl, (float)r, pr, pi);
+        }
+        acc_re += (double)pr;
+        acc_im += (double)pi;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      acc_re += __shfl_xor(acc_re, off);
+      acc_im += __shfl_xor(acc_im, off);
+    }
+    if (lane == 0) {
+      A.slots[2 * (tile * A.count + k)] = acc_re;
+      A.slots[2 * (tile * A.count + k) + 1] = acc_im;
+    }
+  }
+}
+
+// grad[k] = sum over tiles (fixed order) of slots[tile][k]
+__global__ void k_slot_reduce(const double* __restrict__ slots, long long tiles, long long count,
+                              double* __restrict__ grad) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  double re = 0.0, im = 0.0;
+  for (long long q = 0; q < tiles; ++q) {
+    re += slots[2 * (q * count + k)];
+    im += slots[2 * (q * count + k) + 1];
+  }
+  grad[2 * k] = re;
+  grad[2 * k + 1] = im;
+}
+
+__global__ void k_fill_zero(double* __restrict__ p, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// out[idx[i]] = scale * vals[i]; an index outside [0, total) is skipped (the
+// same guard as the parameter kernels, which make such beams inert)
+__global__ void k_scatter(const int64_t* __restrict__ idx, const double* __restrict__ vals,
+                          long long k, double scale, long long total, double* __restrict__ out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const int64_t j = idx[i];
+  if (j < 0 || j >= total) return;
+  out[2 * j] = scale * vals[2 * i];
+  out[2 * j + 1] = scale * vals[2 * i + 1];
+}
+
+// vals[i] = coeffs[idx[i]] (0 for an index outside [0, total))
+__global__ void k_gather(const double* __restrict__ coeffs, long long total,
+                         const int64_t* __restrict__ idx, long long k, double* __restrict__ vals) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const int64_t j = idx[i];
+  const bool ok = j >= 0 && j < total;
+  vals[2 * i] = ok ? coeffs[2 * j] : 0.0;
+  vals[2 * i + 1] = ok ? coeffs[2 * j + 1] : 0.0;
+}
+
+struct TShape {
+  long long ntt, ntg, gx;
+};
+
+inline TShape transpose_shape(long long count, long long nt, long long ngroups) {
+  TShape s;
+  s.ntt = (nt + kTileT - 1) / kTileT;
+  s.ntg = (ngroups + kTileG - 1) / kTileG;
+  const long long tiles = s.ntt * s.ntg;
+  long long gx = (4096 + tiles - 1) / tiles;       // ~16 block rounds of 256 CUs
+  const long long maxgx = (count + 3) / 4;          // at least one beam per wave
+  if (gx > maxgx) gx = maxgx;
+  if (gx > 65535) gx = 65535;
+  s.gx = gx < 1 ? 1 : gx;
+  return s;
+}
+
+}  // namespace
+
+extern "C" size_t bh_forward_transpose_workspace_size(int64_t count, int64_t nt,
+                                                      int64_t ngroups) {
+  if (count <= 0 || nt <= 0 || ngroups <= 0) return 0;
+  const TShape sh = transpose_shape(count, nt, ngroups);
+  return (size_t)sh.ntt * sh.ntg * count * 2 * sizeof(double);
+}
+
+extern "C" int bh_forward_transpose(const bh_fwd_beams* b, int ndim, int mirror, double c,
+                                    const double* ts, int64_t nt, const bh_segments* seg,
+                                    double cull_log2, int precision, const void* resid,
+                                    int resid_f64, double* grad, void* ws, size_t ws_bytes,
+                                    bh_stream s) {
+  bh::clear_error();
+  if (int rc = check_segments(seg, ndim)) return rc;
+  if (!b || !ts || !resid || !grad) BH_FAIL(BH_ERR_ARG, "bh_forward_transpose: null pointer");
+  if (nt <= 0 || nt > 65535 * (int64_t)kTileT)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_transpose: nt=%lld", (long long)nt);
+  if (precision != BH_PRECISION_FP64 && precision != BH_PRECISION_FP32_SPLIT)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_transpose: precision=%d", precision);
+  hipStream_t st = (hipStream_t)s;
+  if (b->count == 0) return BH_OK;
+  if (!b->x0 || !b->p0 || !b->omega || !b->alpha || !b->a0)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_transpose: null beam array");
+  const TShape sh = transpose_shape(b->count, nt, seg->ngroups);
+  if (sh.ntg > 65535) BH_FAIL(BH_ERR_ARG, "bh_forward_transpose: %lld point groups",
+                              (long long)seg->ngroups);
+  const size_t need = bh_forward_transpose_workspace_size(b->count, nt, seg->ngroups);
+  if (!ws || ws_bytes < need)
+    BH_FAIL(BH_ERR_WORKSPACE, "bh_forward_transpose: workspace %zu < %zu", ws_bytes, need);
+  TArgs a{};
+  a.b = *b;
+  a.count = b->count;
+  a.modes = mirror ? 2 : 1;
+  a.c = c;
+  a.ts = ts;
+  a.nt = nt;
+  a.G = to_segdev(seg);
+  a.E = cull_log2;
+  a.resid = resid;
+  a.resid_f64 = resid_f64;
+  a.slots = (double*)ws;
+  a.ntg = sh.ntg;
+  dim3 grid((unsigned)sh.gx, (unsigned)sh.ntt, (unsigned)sh.ntg);
+  const bool f64 = precision == BH_PRECISION_FP64;
+  if (ndim == 1) {
+    if (f64) hipLaunchKernelGGL((k_fwd_transpose<1, true>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_fwd_transpose<1, false>), grid, dim3(256), 0, st, a);
+  } else if (ndim == 2) {
+    if (f64) hipLaunchKernelGGL((k_fwd_transpose<2, true>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_fwd_transpose<2, false>), grid, dim3(256), 0, st, a);
+  } else {
+    if (f64) hipLaunchKernelGGL((k_fwd_transpose<3, true>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_fwd_transpose<3, false>), grid, dim3(256), 0, st, a);
+  }
+  BH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_slot_reduce, dim3(bh::grid1d(b->count, 256)), dim3(256), 0, st,
+                     (const double*)ws, sh.ntt * sh.ntg, (long long)b->count, grad);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_coeff_scatter(const int64_t* idx, const double* vals, int64_t k, double scale,
+                                int64_t total, double* coeffs, bh_stream s) {
+  bh::clear_error();
+  if (total <= 0 || k < 0 || k > total)
+    BH_FAIL(BH_ERR_ARG, "bh_coeff_scatter: k=%lld total=%lld", (long long)k, (long long)total);
+  if (!coeffs || (k > 0 && (!idx || !vals)))
+    BH_FAIL(BH_ERR_ARG, "bh_coeff_scatter: null pointer");
+  hipStream_t st = (hipStream_t)s;
+  hipLaunchKernelGGL(k_fill_zero, dim3(bh::grid1d(2 * total, 256)), dim3(256), 0, st, coeffs,
+                     (long long)(2 * total));
+  BH_LAUNCH_CHECK();
+  if (k > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(bh::grid1d(k, 256)), dim3(256), 0, st, idx, vals,
+                       (long long)k, scale, (long long)total, coeffs);
+    BH_LAUNCH_CHECK();
+  }
+  return BH_OK;
+}
+
+extern "C" int bh_coeff_gather(const double* coeffs, int64_t total, const int64_t* idx,
+                               int64_t k, double* vals, bh_stream s) {
+  bh::clear_error();
+  if (total <= 0 || k < 0) BH_FAIL(BH_ERR_ARG, "bh_coeff_gather: k=%lld total=%lld",
+                                   (long long)k, (long long)total);
+  if (k == 0) return BH_OK;
+  if (!coeffs || !idx || !vals) BH_FAIL(BH_ERR_ARG, "bh_coeff_gather: null pointer");
+  hipLaunchKernelGGL(k_gather, dim3(bh::grid1d(k, 256)), dim3(256), 0, (hipStream_t)s, coeffs,
+                     (long long)total, idx, (long long)k, vals);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}

@@ -254,3 +254,28 @@ class MSWPT:
                   _lib.ptr(c), int(output_type == "fourier"), _lib.ptr(out), _lib.ptr(ws),
                   ws.numel(), _lib.stream_handle())
         return out
+
+    def adjoint(self, coeffs, *, real: bool = True, out=None, ws=None):
+        """W^H coeffs, the adjoint of ``forward`` (spatial input, no scale or mask).
+
+        ``forward`` divides the spectrum by the sum of squared filters before
+        windowing, so W^H is the synthesis of ``inverse`` divided by that sum
+        before the inverse FFT (W^H W != I; W^-1 = ``inverse``).  ``real``
+        returns the real part, the transpose for real spatial input.
+        """
+        import torch
+        if self._tiling is None:
+            raise NotImplementedError("the adjoint with windowing='none' is not built on the GPU")
+        c = self._as_device(coeffs, torch.complex128)
+        if c.numel() != self.total_coeffs:
+            raise ValueError(f"coeffs must have {self.total_coeffs} entries; got {c.numel()}.")
+        if out is None:
+            out = torch.empty(self.dyadic_decomp.N,
+                              dtype=torch.float64 if real else torch.complex128,
+                              device=self.device)
+        if ws is None:
+            ws = _lib.workspace(self._ws_bytes, self.device)
+        _lib.call("bh_mswpt_adjoint", C.byref(self._tiling), _lib.ptr(self.sum_gsquare),
+                  _lib.ptr(self.gfilts_packed), _lib.ptr(c), int(real), _lib.ptr(out),
+                  _lib.ptr(ws), ws.numel(), _lib.stream_handle())
+        return out

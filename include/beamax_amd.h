@@ -38,6 +38,11 @@
  *                       (examples/reconstruction/2d_time_reversal_and_adjoint.py:111-165)
  *   bh_forward_sum_culled, bh_static_sum_culled: the same sums with a certified
  *                       envelope cut (no reference counterpart; same seams)
+ *   bh_forward_transpose, bh_coeff_scatter, bh_coeff_gather, bh_mswpt_adjoint: the exact
+ *                       transpose of the frozen-index forward (beam sum ->
+ *                       coefficients -> MSWPT analysis); the reference gets
+ *                       it only through JAX autodiff of forward
+ *                       (msgb_solver.py:858-982), it has no entry point of its own
  *   bh_static_sum       compute_TR_result + compute_gaussian_beam_real_TR at
  *                       t = 0 + c^-2 weight (tr_solver_utils.py:677-750,
  *                       beamax/gb/core.py:332-425, msgb_solver.py:123-127)
@@ -254,6 +259,38 @@ size_t bh_static_sum_culled_workspace_size(int64_t count, int ndim, int64_t P);
 int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
                          double scale, double cull_log2, int precision, void* out,
                          unsigned long long* pairs, void* ws, size_t ws_bytes, bh_stream s);
+
+/* Exact transpose of the frozen-index forward beam sum.  For the beam table b
+ * with unit coefficients (a0 without the MSWPT coefficient) and a residual
+ * resid (nt, S) (float, or double if resid_f64), grad[k] (complex) =
+ * sum_{mode} sum_{t,s} 2 resid(t,s) conj(G_k(t,s)), G_k the complex Gaussian
+ * beam of the forward sum (both mirrored modes when mirror != 0), so that
+ * <forward(c), resid> = Re sum_k conj(grad[k]) c[k].  Terms outside the
+ * envelope bound of bh_forward_sum_culled (cull_log2) are skipped the same
+ * way; cull_log2 <= 0 evaluates every term.  The workspace holds one partial
+ * per (beam, tile of 64 time rows x 32 point groups); the partials are added
+ * in a fixed order (deterministic). */
+size_t bh_forward_transpose_workspace_size(int64_t count, int64_t nt, int64_t ngroups);
+int bh_forward_transpose(const bh_fwd_beams* b, int ndim, int mirror, double c,
+                         const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
+                         int precision, const void* resid, int resid_f64, double* grad,
+                         void* ws, size_t ws_bytes, bh_stream s);
+/* coeffs[total] complex = 0, then coeffs[idx[i]] = scale * vals[i] (indices
+ * outside [0, total) are skipped): the transpose of gathering the selected
+ * coefficients. */
+int bh_coeff_scatter(const int64_t* idx, const double* vals, int64_t k, double scale,
+                     int64_t total, double* coeffs, bh_stream s);
+/* vals[i] = coeffs[idx[i]] (complex; 0 for an index outside [0, total)):
+ * the selected coefficients of a frozen index set. */
+int bh_coeff_gather(const double* coeffs, int64_t total, const int64_t* idx, int64_t k,
+                    double* vals, bh_stream s);
+/* Adjoint of bh_mswpt_forward (no scale, no mask): out = W^H coeffs, i.e.
+ * the synthesis of bh_mswpt_inverse divided by sum_g2 before the inverse
+ * FFT; output_real != 0 writes its real part (prod N doubles, the transpose
+ * for real spatial input), else prod N complex. */
+int bh_mswpt_adjoint(const bh_tiling* t, const double* sum_gsquare, const double* gpacked,
+                     const double* coeffs, int output_real, double* out, void* ws,
+                     size_t ws_bytes, bh_stream s);
 
 #ifdef __cplusplus
 }

@@ -194,6 +194,46 @@ def complex_beam(xt, pt, Mt, At, omega, sensors):
     return At[:, :, None] * np.exp(1j * omega[:, None, None] * ph)
 
 
+def forward_transpose_sum(x0, p0, alpha, a0, omega, mode, c, ts, sensors, resid):
+    """Transpose of the real beam sum in the beam coefficients: for beams with
+    unit-coefficient amplitudes a0, g_b = sum_{t,s} 2 r(t,s) conj(G_b(t,s)),
+    G_b = complex_beam of ``solve_hom_diag``, so that
+    <real_beam_sum(a0 * c), r> = Re sum_b conj(g_b) c_b.  The reference has no
+    such function; it is what JAX autodiff of ``compute_gaussian_beam_real``
+    (gb/core.py:194-329) in the amplitudes returns."""
+    xt, pt, Mt, At = solve_hom_diag(x0, p0, alpha, a0, mode, np.asarray(ts), c)
+    sensors = np.asarray(sensors)
+    r = np.asarray(resid, float)
+    g = np.empty(len(omega), dtype=complex)
+    for b in range(len(omega)):   # one beam at a time keeps memory at (Nt, S)
+        G = complex_beam(xt[b:b + 1], pt[b:b + 1], Mt[b:b + 1], At[b:b + 1], omega[b:b + 1],
+                         sensors)[0]
+        g[b] = 2.0 * np.sum(r * np.conj(G))
+    return g
+
+
+def frozen_forward(p0, wpt: MSWPT, idx, grid_size, N, c, ts, sensors):
+    """The forward pipeline of ``forward_pipeline`` on a frozen index set."""
+    cpos = compute_coefficients_pos_only(wpt, p0, c, grid_size)
+    p, alpha, x0, om, a0, mode = forward_parameters(wpt, idx, grid_size, N)
+    dup = lambda v: np.concatenate([v, v])
+    xt, pt, Mt, At = solve_hom_diag(dup(x0), dup(p), dup(alpha), dup(a0 * cpos[idx]),
+                                    np.concatenate([mode, -mode]), np.asarray(ts), c)
+    return real_beam_sum(xt, pt, Mt, At, dup(om), np.asarray(sensors))
+
+
+def frozen_transpose(r, wpt: MSWPT, idx, grid_size, N, c, ts, sensors):
+    """F^T r = Re W^H (0.5 * scatter_idx(g)), g summed over the mirrored pair."""
+    p, alpha, x0, om, a0, mode = forward_parameters(wpt, idx, grid_size, N)
+    dup = lambda v: np.concatenate([v, v])
+    g = forward_transpose_sum(dup(x0), dup(p), dup(alpha), dup(a0), dup(om),
+                              np.concatenate([mode, -mode]), c, ts, sensors, r)
+    k = len(idx)
+    z = np.zeros(wpt.total_coeffs, dtype=complex)
+    z[np.asarray(idx)] = 0.5 * (g[:k] + g[k:])
+    return wpt.adjoint(z).real
+
+
 def forward_pipeline(p0, wpt: MSWPT, grid_size, N, c, ts, sensors, thr):
     """MSGBSolver.forward for real p0, top_n, solve_hom_diag, scan_real
     (msgb_solver.py:570-633, 901-982).  Returns (data (Nt,S), params)."""

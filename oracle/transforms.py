@@ -215,3 +215,13 @@ class MSWPT:
                                     indexing="ij")
                 np.add.at(ft, tuple(grids), contrib)
         return unitary_ifft(ft) if output_type == "spatial" else ft
+
+    def adjoint(self, coeffs):
+        """W^H: the adjoint of ``forward`` for the inner product Re<., .>.
+
+        ``forward`` is unitary_fft, division by sum_gsquare, a gather of each
+        box's patch, the window, a roll and a unitary inverse box FFT; W^H is
+        the reverse chain of adjoints, i.e. the synthesis of ``inverse`` in the
+        Fourier domain divided by sum_gsquare, then unitary_ifft.
+        """
+        return unitary_ifft(self.inverse(coeffs, output_type="fourier") / self.sum_gsquare)

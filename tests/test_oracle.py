@@ -379,3 +379,69 @@ def test_velocity_coefficients_conjugate_across_mirror_boxes():
         nb = shp[0]
         for box in range(nb):
             assert np.allclose(a[box], np.conj(b[nb - 1 - box]), rtol=1e-10, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# transposes (no reference counterpart: JAX autodiff there); pinned by the
+# defining identity <A x, y> = <x, A^T y> on random inputs
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("windowing", ["rectangular", "rectangular_mirror"])
+def test_mswpt_adjoint_dot_product(windowing):
+    t = Tiling(2, (64, 32), (4, 8), (2, 1))
+    w = MSWPT(t, 2, windowing)
+    rng = np.random.default_rng(31)
+    p = rng.standard_normal(t.N)
+    c = rng.standard_normal(w.total_coeffs) + 1j * rng.standard_normal(w.total_coeffs)
+    lhs = np.real(np.vdot(c, w.forward(p)))
+    rhs = np.vdot(p, w.adjoint(c).real).real
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    pc = p + 1j * rng.standard_normal(t.N)
+    lhs = np.real(np.vdot(c, w.forward(pc)))
+    rhs = np.real(np.vdot(w.adjoint(c), pc))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    # W^H is not W^-1 (the frame is not tight): a dropped sum_gsquare would pass
+    # the inverse test, so check the two differ
+    assert np.max(np.abs(w.adjoint(c) - w.inverse(c))) > 1e-3 * np.max(np.abs(w.inverse(c)))
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_beam_sum_transpose_dot_product(d):
+    rng = np.random.default_rng(32 + d)
+    B = 24
+    xi = rng.standard_normal((B, d)) * 2000
+    om = np.linalg.norm(xi, axis=1)
+    p = 2 * np.pi * xi / om[:, None]
+    alpha = 1j * (rng.random((B, d)) + 0.3) * 2 * (np.pi * 500) ** 2 / om[:, None]
+    a0 = rng.standard_normal(B) + 1j * rng.standard_normal(B)
+    x0 = rng.random((B, d)) * 0.005
+    mode = np.where(rng.random(B) < 0.5, 1.0, -1.0)
+    ts = np.linspace(0, 3e-6, 7)
+    sens = rng.random((29, d)) * 0.005
+    c = rng.standard_normal(B) + 1j * rng.standard_normal(B)
+    r = rng.standard_normal((ts.size, sens.shape[0]))
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p, alpha, a0 * c, mode, ts, 1500.0)
+    u = obeams.real_beam_sum(xt, pt, Mt, At, om, sens)
+    g = obeams.forward_transpose_sum(x0, p, alpha, a0, om, mode, 1500.0, ts, sens, r)
+    lhs = np.sum(u * r)
+    rhs = np.real(np.vdot(g, c))
+    assert abs(lhs - rhs) <= 1e-12 * np.sum(np.abs(u * r))
+
+
+def test_frozen_forward_transpose_dot_product():
+    N = (32, 32)
+    t = Tiling(2, N, (4, 8), (1, 1))
+    w = MSWPT(t, 2, "rectangular_mirror")
+    L = np.asarray(N) * 1e-4
+    rng = np.random.default_rng(34)
+    p0 = rng.standard_normal(N)
+    cpos = obeams.compute_coefficients_pos_only(w, p0, 1500.0, L)
+    idx, _ = otopn.top_n(cpos, 120)
+    ts = np.arange(0, 40) * 0.3e-4 / 1500.0
+    grid = np.stack(np.meshgrid(*[np.arange(n) * 1e-4 for n in N], indexing="ij"), -1)
+    sens = np.concatenate([grid[0], grid[-1]], axis=0)
+    q = rng.standard_normal(N)
+    r = rng.standard_normal((ts.size, sens.shape[0]))
+    Fq = obeams.frozen_forward(q, w, idx, L, N, 1500.0, ts, sens)
+    FTr = obeams.frozen_transpose(r, w, idx, L, N, 1500.0, ts, sens)
+    lhs, rhs = np.sum(Fq * r), np.sum(q * FTr)
+    assert abs(lhs - rhs) <= 1e-11 * np.sum(np.abs(Fq * r))
