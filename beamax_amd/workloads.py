@@ -29,3 +29,32 @@ def packets_2d(N, dx, n_packets=64, seed=0, sigma_cells=(4.0, 16.0), k_frac=(0.1
         ph = 2 * np.pi * kmag[j] * (np.cos(th[j]) * rx + np.sin(th[j]) * ry)
         p0 += amp[j] * env * np.cos(ph)
     return p0
+
+
+def phantom_2d(N, dx, seed=0, n_ellipses=(3, 7), edge_cells=1.5, n_packets=2):
+    """Ellipses (centres in the inner 60%, semi-axes 4-25% of the grid, random
+    orientation, amplitude U(0, 1), logistic edge of width ``edge_cells``) plus
+    ``n_packets`` packets of ``packets_2d`` scaled to amplitude 0.3."""
+    rng = np.random.default_rng(seed)
+    n0, n1 = N
+    L = np.asarray(N, float) * np.asarray(dx, float)
+    x = np.arange(n0)[:, None] * dx[0]
+    y = np.arange(n1)[None, :] * dx[1]
+    p0 = np.zeros(N)
+    m = int(rng.integers(n_ellipses[0], n_ellipses[1] + 1))
+    for _ in range(m):
+        c = rng.uniform(0.2, 0.8, 2) * L
+        ax = rng.uniform(0.04, 0.25, 2) * L
+        th = rng.uniform(0.0, np.pi)
+        amp = rng.uniform(0.0, 1.0)
+        rx, ry = x - c[0], y - c[1]
+        u = (np.cos(th) * rx + np.sin(th) * ry) / ax[0]
+        v = (-np.sin(th) * rx + np.cos(th) * ry) / ax[1]
+        # signed distance proxy in cells: (1 - rho) * min semi-axis / dx
+        rho = np.sqrt(u * u + v * v)
+        sd = (1.0 - rho) * min(ax) / min(dx)
+        p0 += amp / (1.0 + np.exp(-sd / edge_cells))
+    if n_packets:
+        p0 += 0.3 * packets_2d(N, dx, n_packets, seed=seed + 7919, sigma_cells=(3.0, 8.0),
+                               k_frac=(0.2, 0.6))
+    return p0

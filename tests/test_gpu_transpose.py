@@ -112,3 +112,35 @@ def test_frozen_operator_is_transpose_pair(gpu):
         FTr = _n(F.transpose(_t(r, gpu)))
         lhs, rhs = float(np.sum(Fq * r)), float(np.sum(q * FTr))
         assert abs(lhs - rhs) <= tol * np.sum(np.abs(Fq * r)), (prec, E, lhs, rhs)
+
+
+def test_cgnr_reconstruction_small(gpu):
+    import torch
+    import beamax_amd as bx
+    from beamax_amd.gb import gb_solvers
+    from beamax_amd.reconstruct import cgnr
+    from beamax_amd.solvers.faces import four_edge_mask
+    from beamax_amd.solvers.linear import FrozenForward
+    from beamax_amd.workloads import phantom_2d
+    N = (64, 64)
+    dx = (1e-4, 1e-4)
+    dom = bx.Domain(N=N, dx=dx, c=1500.0, periodic=(False, False))
+    wpt = bx.MSWPT(bx.DyadicDecomposition(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror",
+                   device=gpu)
+    sensors = bx.Sensor(dom, binary_mask=four_edge_mask(N))
+    ts = dom.generate_time_domain()
+    p = _t(phantom_2d(N, dx, seed=3), gpu)
+    for prec, E in (("fp64", 0.0), ("fp32_split", 40.0)):
+        s = bx.MSGBSolver(1500, "top_n", 128, "spatial", gb_solvers.solve_hom_diag, "scan_real",
+                          precision=prec, cull_log2=E)
+        F = FrozenForward.from_field(s, p, dom, sensors, ts, wpt)
+        d = F.apply(p).to(torch.float64)
+        x, info = cgnr(F, d, tol=1e-3, maxiter=200)
+        assert info["converged"], info["residuals"][-5:]
+        g0 = F.transpose(d)
+        g = F.transpose(d - F.apply(x).to(torch.float64))
+        res = float(g.norm() / g0.norm())
+        assert res <= 1.05e-3, res     # recomputed outside the solver
+        assert abs(res - info["residuals"][-1]) <= 1e-2 * res + 1e-6
+        # consistent data: the fitted data matches the measured data closely
+        assert float((F.apply(x).to(torch.float64) - d).norm() / d.norm()) < 0.05
