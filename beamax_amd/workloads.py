@@ -58,3 +58,23 @@ def phantom_2d(N, dx, seed=0, n_ellipses=(3, 7), edge_cells=1.5, n_packets=2):
         p0 += 0.3 * packets_2d(N, dx, n_packets, seed=seed + 7919, sigma_cells=(3.0, 8.0),
                                k_frac=(0.2, 0.6))
     return p0
+
+
+def packets_3d(N, dx, n_packets=32, seed=0, sigma_cells=(3.0, 10.0), k_frac=(0.1, 0.7)):
+    rng = np.random.default_rng(seed)
+    L = np.asarray(N, float) * np.asarray(dx, float)
+    c = rng.uniform(0.1, 0.9, (n_packets, 3)) * L
+    sig = rng.uniform(sigma_cells[0], sigma_cells[1], n_packets) * min(dx)
+    kmag = rng.uniform(k_frac[0], k_frac[1], n_packets) / (2.0 * min(dx))
+    dirs = rng.standard_normal((n_packets, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    amp = rng.standard_normal(n_packets)
+    ax = [np.arange(N[a]) * dx[a] for a in range(3)]
+    p0 = np.zeros(N)
+    for j in range(n_packets):
+        r = [ax[a] - c[j, a] for a in range(3)]
+        rx, ry, rz = r[0][:, None, None], r[1][None, :, None], r[2][None, None, :]
+        env = np.exp(-(rx * rx + ry * ry + rz * rz) / (2 * sig[j] ** 2))
+        ph = 2 * np.pi * kmag[j] * (dirs[j, 0] * rx + dirs[j, 1] * ry + dirs[j, 2] * rz)
+        p0 += amp[j] * env * np.cos(ph)
+    return p0

@@ -32,6 +32,8 @@ if ROOT not in sys.path:
 METRIC = ("Gbeam-evals/s MSGB fwd+adj (512², err<1e-5 vs FP64); "
           "recon s to 1e-3 resid")
 FLOP_PER_EVAL_2D = 25.0
+FLOP_PER_EVAL_3D = 43.0
+METRIC_3D = "Gbeam-evals/s MSGB fwd+adj (128^3 3-D, err<1e-5 vs FP64)"
 FP32_VALU_PEAK_TFLOPS = 157.3
 # HBM bytes per forward launch of the culled FP32-split kernel on this
 # workload at cull_log2 = 40: rocprofv3 FETCH_SIZE (244467.1 KiB raw, x2.0)
@@ -84,6 +86,78 @@ def run_step(P, prec, timers=None):
         timers[2].record()
         s.pair_counter = P["counters"]["adj"]
     img = s.sum_static(ab, P["adj"][prec].points, 1.0 / P["dom"].c ** 2, dx=P["dom"].dx)
+    if timers is not None:
+        timers[3].record()
+    s.pair_counter = None
+    return data, img, beams, ab
+
+
+def build_problem_3d(device, cull_log2=40.0):
+    """128^3, dx 1e-4, c 1500, cfl 0.3, 3 levels (4,8,16) rho 2, 32 seeded
+    packets; planar detector on the axis-0 face (128 x 128 = 16384 sensors;
+    the face the reference's TR parameters take as the normal, the z = 0 face
+    of the cube up to relabelling the axes); top_n 262144.  Adjoint: the face
+    data Fourier-cropped to 256 time samples, data MSWPT 2 levels (4,8) with
+    aspect (2,1,1), top_n 262144, summed over the 128^3 voxels."""
+    import torch
+    import beamax_amd as bx
+    from beamax_amd import _lib
+    from beamax_amd.gb import gb_solvers
+    from beamax_amd.workloads import packets_3d
+    N = (128, 128, 128)
+    dx = (1e-4,) * 3
+    dom = bx.Domain(N=N, dx=dx, c=1500.0, cfl=0.3, periodic=(False,) * 3)
+    wpt = bx.MSWPT(bx.DyadicDecomposition(3, N, (4, 8, 16), (1, 1, 1)), 2, "rectangular_mirror",
+                   device=device)
+    mask = np.zeros(N)
+    mask[0] = 1
+    sensors = bx.Sensor(dom, binary_mask=mask)
+    ts = dom.generate_time_domain()
+    p0 = torch.as_tensor(packets_3d(N, dx, 32, seed=0), device=device)
+    nt_d = 256
+    span = float(ts[-1] - ts[0])
+    ddom = bx.Domain(N=(nt_d, 128, 128), dx=(span / (nt_d - 1), 1e-4, 1e-4), c=1500.0,
+                     periodic=(False,) * 3)
+    dw = bx.MSWPT(bx.DyadicDecomposition(2, (nt_d, 128, 128), (4, 8), (2, 1, 1)), 2,
+                  "rectangular_mirror", device=device)
+    solvers = {prec: bx.MSGBSolver(262144, "top_n", 128, "spatial", gb_solvers.solve_hom_diag,
+                                   "scan_real", precision=prec, cull_log2=cull_log2)
+               for prec in ("fp32_split", "fp64")}
+    counters = {k: torch.zeros(1, dtype=torch.uint64, device=device) for k in ("fwd", "adj")}
+    S = sensors.positions.shape[0]
+    cols = torch.arange(S, dtype=torch.int64, device=device)
+    wts = torch.ones(S, dtype=torch.float64, device=device)
+    crop_ws = _lib.workspace(_lib.load().bh_fourier_crop_time_workspace_size(ts.size, nt_d, S),
+                             device)
+    points = np.ascontiguousarray(dom.grid.reshape(-1, 3))
+    return dict(dom=dom, wpt=wpt, sensors=sensors, ts=ts, p0=p0, solvers=solvers, ddom=ddom,
+                dw=dw, counters=counters, cols=cols, wts=wts, crop_ws=crop_ws, points=points,
+                nt_d=nt_d, dim=3)
+
+
+def run_step_3d(P, prec, timers=None):
+    import torch
+    from beamax_amd import _lib
+    s = P["solvers"][prec]
+    beams, _, _ = s.prepare_forward_beams(P["p0"], P["dom"], P["wpt"])
+    if timers is not None:
+        timers[0].record()
+        s.pair_counter = P["counters"]["fwd"]
+    data = s.sum_forward(beams, P["dom"], P["sensors"].positions, P["ts"])
+    if timers is not None:
+        timers[1].record()
+    S = data.shape[1]
+    x = data.to(torch.float64).contiguous()
+    face = torch.empty((P["nt_d"], S), dtype=torch.float64, device=x.device)
+    _lib.call("bh_fourier_crop_time", _lib.ptr(x), x.shape[0], S, _lib.ptr(P["cols"]),
+              _lib.ptr(P["wts"]), S, P["nt_d"], _lib.ptr(face), _lib.ptr(P["crop_ws"]),
+              P["crop_ws"].numel(), _lib.stream_handle())
+    ab = s.prepare_adjoint_beams(face.reshape(P["nt_d"], 128, 128), P["sensors"], P["ddom"],
+                                 P["dw"], float(P["dom"].c))
+    if timers is not None:
+        timers[2].record()
+        s.pair_counter = P["counters"]["adj"]
+    img = s.sum_static(ab, P["points"], 1.0 / P["dom"].c ** 2, dx=P["dom"].dx)
     if timers is not None:
         timers[3].record()
     s.pair_counter = None
@@ -150,7 +224,7 @@ def recon_leg(device, tol=1e-3, seed=0):
     }
 
 
-def cpu_baseline(P, beams, ab, budget_s=20.0):
+def cpu_baseline(P, beams, ab, pts, budget_s=20.0):
     """The oracle's C++/OpenMP FP64 port on the host cores, on a prefix of
     beam batches (cost is linear in the beam count), extrapolated to the
     full workload."""
@@ -171,12 +245,11 @@ def cpu_baseline(P, beams, ab, budget_s=20.0):
         step = 128
         t0 = time.perf_counter()
         ocpu.forward_sum(X, Pp, AL, A0, OM, mode, 1500.0, ts, S, nthreads=cores,
-                         beam_begin=nb, beam_end=nb + step)
+                         beam_begin=nb, beam_end=min(nb + step, 2 * k))
         t_fwd += time.perf_counter() - t0
         nb += step
     fwd_rate = nb * ts.size * S.shape[0] / t_fwd
     # adjoint: a prefix of beams over all pixels
-    pts = n(P["adj"]["fp64"].points)
     nab = min(ab.count, 256)
     t0 = time.perf_counter()
     ocpu.static_sum(n(ab.x)[:nab], n(ab.p)[:nab], n(ab.M)[:nab], n(ab.A)[:nab], n(ab.omega)[:nab],
@@ -222,6 +295,8 @@ def main():
     ap.add_argument("--cull-log2", type=float, default=40.0,
                     help="envelope cut exponent of the summations (0 = dense sums)")
     ap.add_argument("--no-recon", action="store_true", help="skip the reconstruction leg")
+    ap.add_argument("--workload", choices=("2d", "3d"), default="2d",
+                    help="2d: the 512^2 metric workload (default); 3d: 128^3 forward+adjoint")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
                     help="also run one plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -237,12 +312,15 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    three = args.workload == "3d"
+    step_fn = run_step_3d if three else run_step
+    flop = FLOP_PER_EVAL_3D if three else FLOP_PER_EVAL_2D
     t0 = time.perf_counter()
-    P = build_problem(dev, args.cull_log2)
+    P = build_problem_3d(dev, args.cull_log2) if three else build_problem(dev, args.cull_log2)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     for _ in range(args.warmup):
-        run_step(P, "fp32_split")
+        step_fn(P, "fp32_split")
     torch.cuda.synchronize()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     for c in P["counters"].values():
@@ -250,7 +328,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        data, img, beams, ab = run_step(P, "fp32_split", ev[i])
+        data, img, beams, ab = step_fn(P, "fp32_split", ev[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
@@ -271,9 +349,19 @@ def main():
     else:
         fwd_exec, adj_exec = fwd_evals, adj_evals
     exec_evals = fwd_exec + adj_exec
-    fwd_tflops = fwd_evals * FLOP_PER_EVAL_2D / (fwd_ms * 1e-3) / 1e12
+    fwd_tflops = fwd_evals * flop / (fwd_ms * 1e-3) / 1e12
+    if three:
+        workload = ("3-D 128^3 MSGB forward+adjoint: dx 1e-4, c 1500, cfl 0.3, 3 levels "
+                    "(4,8,16) rho 2, 32 seeded packets, top_n 262144 -> 524288 beams x "
+                    f"{nt} steps x {S} sensors on the axis-0 face; adjoint: face data cropped "
+                    "to 256 steps, 2 levels 4/8 aspect 2:1:1, top_n 262144 on 128^3 voxels")
+    else:
+        workload = ("2-D 512^2 MSGB forward+adjoint: dx 5e-5, c 1500, cfl 0.3, 4 levels "
+                    "(4,8,16,32) rho 2, 64 seeded packets, top_n 65536 -> 131072 beams x "
+                    f"{nt} steps x {S} four-edge sensors; adjoint 4 faces x 65536 beams "
+                    "(data 1024 x 512, 2 levels 4/8, aspect 2:1) on 512^2 pixels")
     result = {
-        "metric": METRIC,
+        "metric": METRIC_3D if three else METRIC,
         "value": round(value, 3),
         "unit": "Gbeam-evals/s",
         "n_gpus": 1,
@@ -287,10 +375,7 @@ def main():
         "phase_dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": ("2-D 512^2 MSGB forward+adjoint: dx 5e-5, c 1500, cfl 0.3, 4 levels "
-                         "(4,8,16,32) rho 2, 64 seeded packets, top_n 65536 -> 131072 beams x "
-                         f"{nt} steps x {S} four-edge sensors; adjoint 4 faces x 65536 beams "
-                         "(data 1024 x 512, 2 levels 4/8, aspect 2:1) on 512^2 pixels"),
+            "workload": workload,
             "forward_evals": fwd_evals,
             "adjoint_evals": adj_evals,
             "cull_log2": args.cull_log2,
@@ -308,15 +393,15 @@ def main():
             "bound": "valu",
             "kernel": ("k_culled_sum (bh_forward_sum_culled, FP32-split)" if args.cull_log2 > 0
                        else "k_forward_sum (bh_forward_sum, FP32-split)"),
-            "achieved": round(fwd_exec * FLOP_PER_EVAL_2D / (fwd_ms * 1e-3) / 1e12, 3),
+            "achieved": round(fwd_exec * flop / (fwd_ms * 1e-3) / 1e12, 3),
             "peak": FP32_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": round(fwd_exec * FLOP_PER_EVAL_2D / (fwd_ms * 1e-3) / 1e12
+            "frac": round(fwd_exec * flop / (fwd_ms * 1e-3) / 1e12
                           / FP32_VALU_PEAK_TFLOPS, 4),
             "dense_equivalent_tflops": round(fwd_tflops, 3),
-            "traffic": MEASURED_FWD_TRAFFIC.get(float(args.cull_log2)),
+            "traffic": None if three else MEASURED_FWD_TRAFFIC.get(float(args.cull_log2)),
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
-            "flop_per_eval": FLOP_PER_EVAL_2D,
+            "flop_per_eval": flop,
             "evals_per_launch": fwd_exec,
             "note": ("achieved counts executed lane evaluations of the launch (64 per "
                      "evaluated beam x 64-point group pair); dense_equivalent_tflops counts "
@@ -325,16 +410,20 @@ def main():
         },
     }
     if not args.no_check:
-        d64, i64, b64, a64 = run_step(P, "fp64")
+        d64, i64, b64, a64 = step_fn(P, "fp64")
         torch.cuda.synchronize()
         from beamax_amd.utils import max_rel, rel_l2
         result["error_vs_fp64"] = {
             "forward_rel_l2": rel_l2(d64, data), "forward_max_rel": max_rel(d64, data),
             "adjoint_rel_l2": rel_l2(i64, img), "adjoint_max_rel": max_rel(i64, img)}
-    if not args.no_recon:
+        if three:
+            result["error_vs_fp64"]["reference"] = ("FP64 kernels with the same envelope cut "
+                                                    "(dropped terms < 2^-40 of their beam)")
+    if not args.no_recon and not three:
         result["recon"] = recon_leg(dev)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(P, beams, ab)
+        pts = P["points"] if three else P["adj"]["fp64"].points.detach().cpu().numpy()
+        result["cpu_baseline"] = cpu_baseline(P, beams, ab, pts)
         result["speedup_vs_cpu"] = round(value / result["cpu_baseline"]["value"], 2)
     print(json.dumps(result))
 
