@@ -87,7 +87,11 @@ SIGNATURES = {
     "bh_tr_params": (C.c_int, [C.POINTER(Tiling), C.POINTER(C.c_double), _P, _P, _I64, C.c_int,
                                C.c_double, C.c_double, C.c_double, C.c_int, StaticBeams, _P]),
     "bh_mswpt_combine_velocity": (C.c_int, [C.POINTER(Tiling), C.POINTER(C.c_double), C.c_double,
-                                            _P, _P, C.c_int, _P, _P]),
+                                            _P, _P, C.c_int, _P, _P, _P]),
+    "bh_forward_sum_modes": (C.c_int, [C.POINTER(FwdBeams), C.c_int, _I64, C.c_int, C.c_double, _P,
+                                       _I64,
+                                       C.POINTER(Segs), C.c_double, C.c_int, C.c_int, _P, _P, _P,
+                                       _SZ, _P]),
     "bh_forward_sum_culled_workspace_size": (_SZ, [_I64, _I64]),
     "bh_forward_sum_culled": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, C.c_double, _P,
                                         _I64, C.POINTER(Segs), C.c_double, C.c_int, _P, _P, _P,

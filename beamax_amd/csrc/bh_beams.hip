@@ -545,6 +545,15 @@ void launch_fwd_spt(const FwdSumArgs& a, bool per, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+// zero fill in 4-byte words (a kernel rather than hipMemsetAsync, so the
+// call stays correct inside a captured graph)
+__global__ void k_zero4(void* out, long long n4) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n4) ((unsigned*)out)[i] = 0u;
+}
+}  // namespace
+
 extern "C" int bh_forward_params(const bh_tiling* t, const double* grid_size,
                                  const int64_t* idx, const double* vals, int64_t k,
                                  bh_fwd_beams out, bh_stream s) {
@@ -594,8 +603,9 @@ extern "C" int bh_forward_sum(const bh_fwd_beams* b, int ndim, int mirror, doubl
   a.out = out;
   hipStream_t st = (hipStream_t)s;
   if (a.total == 0) {
-    const size_t bytes = (size_t)nt * S * (precision == BH_PRECISION_FP64 ? 8 : 4);
-    BH_HIP(hipMemsetAsync(out, 0, bytes, st));
+    const long long n4 = (long long)nt * S * (precision == BH_PRECISION_FP64 ? 2 : 1);
+    hipLaunchKernelGGL(k_zero4, dim3(bh::grid1d(n4, 256)), dim3(256), 0, st, out, n4);
+    BH_LAUNCH_CHECK();
     return BH_OK;
   }
   if (!b->x0 || !b->p0 || !b->omega || !b->alpha || !b->a0)
@@ -706,7 +716,9 @@ extern "C" int bh_static_sum(const bh_static_beams* b, int ndim, const double* p
   hipStream_t st = (hipStream_t)s;
   const bool f64 = precision == BH_PRECISION_FP64;
   if (b->count == 0) {
-    BH_HIP(hipMemsetAsync(out, 0, (size_t)P * (f64 ? 8 : 4), st));
+    const long long n4 = (long long)P * (f64 ? 2 : 1);
+    hipLaunchKernelGGL(k_zero4, dim3(bh::grid1d(n4, 256)), dim3(256), 0, st, out, n4);
+    BH_LAUNCH_CHECK();
     return BH_OK;
   }
   const size_t need = bh_static_sum_workspace_size(b->count, ndim);

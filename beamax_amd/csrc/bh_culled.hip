@@ -21,9 +21,13 @@ namespace {
 constexpr int kRecChunk = 256;
 
 struct CulledArgs {
-  // forward source
+  // forward source: mirror != 0 -> beams [0, count) with mode +1 and their
+  // copies [count, 2 count) with mode -1; mirror == 0 -> beam j has mode +1
+  // for j < npos, -1 otherwise
   bh_fwd_beams b;
-  long long count, total;
+  long long count, total, npos;
+  int mirror;
+  double phase;   // added to every beam's arg(A) (cycles)
   double c;
   const double* ts;
   // static source
@@ -37,7 +41,14 @@ struct CulledArgs {
   double* part;        // split > 1: partial sums [split][rows][S] (FP64)
   int split;
   long long rows;      // output rows (nt for the forward, 1 for the static form)
+  int ostride;         // output element o goes to out[o * ostride + ooff]
+  int ooff;
 };
+
+template <int D>
+__device__ inline void shift_phase(RecS<D>& R, double cyc) { R.ph0 += cyc; }
+template <int D>
+__device__ inline void shift_phase(RecD<D>& R, double cyc) { R.ang += bh::kTwoPi * cyc; }
 
 template <int D, int SPT, bool F64, bool STATIC>
 __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
@@ -76,8 +87,8 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
       if constexpr (STATIC) {
         R = ((const Rec*)A.recs)[bi];
       } else {
-        const bool neg = bi >= A.count;
-        const long long j = neg ? bi - A.count : bi;
+        const bool neg = A.mirror ? bi >= A.count : bi >= A.npos;
+        const long long j = (A.mirror && neg) ? bi - A.count : bi;
         double x0[D], p0[D];
         cd al[D];
         for (int a = 0; a < D; ++a) {
@@ -89,6 +100,7 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
         hom_diag_state_fast<D>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]),
                                neg ? -A.c : A.c, tv, St);
         make_rec<D>(St, A.b.omega[j], R);
+        if (A.phase != 0.0) shift_phase<D>(R, A.phase);
       }
       live = !rec_zero<D>(R);   // zero amplitude contributes exactly 0
     } else {
@@ -169,21 +181,28 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
     if (A.split > 1) {
       A.part[(long long)blockIdx.z * A.rows * A.G.S + o] = accs[gl][lane];
     } else {
-      if constexpr (F64) ((double*)A.out)[o] = A.scale * accs[gl][lane];
-      else ((float*)A.out)[o] = (float)(A.scale * accs[gl][lane]);
+      const long long oo = o * A.ostride + A.ooff;
+      if constexpr (F64) ((double*)A.out)[oo] = A.scale * accs[gl][lane];
+      else ((float*)A.out)[oo] = (float)(A.scale * accs[gl][lane]);
     }
   }
 }
 
 // ordered sum of the split partials (deterministic)
 __global__ void k_combine(const double* __restrict__ part, int split, long long n, double scale,
-                          int f64, void* out) {
+                          int f64, void* out, int ostride, int ooff) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
   for (int z = 0; z < split; ++z) s += part[(long long)z * n + i];
-  if (f64) ((double*)out)[i] = scale * s;
-  else ((float*)out)[i] = (float)(scale * s);
+  const long long o = i * ostride + ooff;
+  if (f64) ((double*)out)[o] = scale * s;
+  else ((float*)out)[o] = (float)(scale * s);
+}
+
+__global__ void k_zero_out(void* out, long long nbytes4) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < nbytes4) ((unsigned*)out)[i] = 0u;
 }
 
 // static records (same as the dense static path)
@@ -251,7 +270,7 @@ int launch_culled(CulledArgs a, long long ny, long long nchunk, void* part_ws, s
   if (a.split > 1) {
     const long long n = ny * a.G.S;
     hipLaunchKernelGGL(k_combine, dim3(bh::grid1d(n, 256)), dim3(256), 0, s, (const double*)part_ws,
-                       a.split, n, a.scale, (int)F64, a.out);
+                       a.split, n, a.scale, (int)F64, a.out, a.ostride, a.ooff);
   }
   return BH_OK;
 }
@@ -263,43 +282,76 @@ extern "C" size_t bh_forward_sum_culled_workspace_size(int64_t nt, int64_t S) {
   return (size_t)8 * nt * S * sizeof(double);
 }
 
-extern "C" int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror, double c,
-                                     const double* ts, int64_t nt, const bh_segments* seg,
-                                     double cull_log2, int precision, void* out,
-                                     unsigned long long* pairs, void* ws, size_t ws_bytes,
-                                     bh_stream s) {
+namespace {
+int forward_culled(const char* name, const bh_fwd_beams* b, int ndim, long long npos, int mirror,
+                   double c, const double* ts, int64_t nt, const bh_segments* seg,
+                   double cull_log2, int precision, int complex_out, void* out,
+                   unsigned long long* pairs, void* ws, size_t ws_bytes, bh_stream s) {
   bh::clear_error();
   if (int rc = check_segments(seg, ndim)) return rc;
-  if (!b || !ts || !out) BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: null pointer");
-  if (nt <= 0 || nt > 65535) BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: nt=%lld", (long long)nt);
+  if (!b || !ts || !out) BH_FAIL(BH_ERR_ARG, "%s: null pointer", name);
+  if (nt <= 0 || nt > 65535) BH_FAIL(BH_ERR_ARG, "%s: nt=%lld", name, (long long)nt);
   if (precision != BH_PRECISION_FP64 && precision != BH_PRECISION_FP32_SPLIT)
-    BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: precision=%d", precision);
+    BH_FAIL(BH_ERR_ARG, "%s: precision=%d", name, precision);
+  if (!mirror && (npos < 0 || npos > b->count))
+    BH_FAIL(BH_ERR_ARG, "%s: npos=%lld outside [0, %lld]", name, npos, (long long)b->count);
   hipStream_t st = (hipStream_t)s;
   const bool f64 = precision == BH_PRECISION_FP64;
+  const int ostride = complex_out ? 2 : 1;
   if (b->count == 0) {
-    BH_HIP(hipMemsetAsync(out, 0, (size_t)nt * seg->S * (f64 ? 8 : 4), st));
+    const long long n4 = (long long)nt * seg->S * ostride * (f64 ? 2 : 1);
+    hipLaunchKernelGGL(k_zero_out, dim3(bh::grid1d(n4, 256)), dim3(256), 0, st, out, n4);
+    BH_LAUNCH_CHECK();
     return BH_OK;
   }
   if (!b->x0 || !b->p0 || !b->omega || !b->alpha || !b->a0)
-    BH_FAIL(BH_ERR_ARG, "bh_forward_sum_culled: null beam array");
+    BH_FAIL(BH_ERR_ARG, "%s: null beam array", name);
   CulledArgs a{};
   a.b = *b;
   a.count = b->count;
   a.total = b->count * (mirror ? 2 : 1);
+  a.npos = mirror ? b->count : npos;
+  a.mirror = mirror ? 1 : 0;
   a.c = c;
   a.ts = ts;
   a.recs = nullptr;
   a.G = to_segdev(seg);
   a.E = cull_log2;
-  a.scale = 1.0;
   a.out = out;
   a.pairs = pairs;
+  a.ostride = ostride;
   const long long nchunk = (a.total + kRecChunk - 1) / kRecChunk;
-  if (ndim == 1) { if (f64) launch_culled<1, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<1, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
-  else if (ndim == 2) { if (f64) launch_culled<2, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<2, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
-  else { if (f64) launch_culled<3, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<3, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
-  BH_LAUNCH_CHECK();
+  // real sum: 2 Re sum A G.  Complex sum: Re = half of it, Im = half of the
+  // real sum with every arg(A) lowered by a quarter cycle (cos(x - pi/2) = sin x)
+  for (int part = 0; part < (complex_out ? 2 : 1); ++part) {
+    a.scale = complex_out ? 0.5 : 1.0;
+    a.phase = part ? -0.25 : 0.0;
+    a.ooff = part;
+    if (ndim == 1) { if (f64) launch_culled<1, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<1, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
+    else if (ndim == 2) { if (f64) launch_culled<2, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<2, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
+    else { if (f64) launch_culled<3, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<3, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
+    BH_LAUNCH_CHECK();
+  }
   return BH_OK;
+}
+}  // namespace
+
+extern "C" int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror, double c,
+                                     const double* ts, int64_t nt, const bh_segments* seg,
+                                     double cull_log2, int precision, void* out,
+                                     unsigned long long* pairs, void* ws, size_t ws_bytes,
+                                     bh_stream s) {
+  return forward_culled("bh_forward_sum_culled", b, ndim, b ? (long long)b->count : 0, mirror, c,
+                        ts, nt, seg, cull_log2, precision, 0, out, pairs, ws, ws_bytes, s);
+}
+
+extern "C" int bh_forward_sum_modes(const bh_fwd_beams* b, int ndim, int64_t npos, int mirror,
+                                    double c, const double* ts, int64_t nt,
+                                    const bh_segments* seg, double cull_log2, int precision,
+                                    int complex_out, void* out, unsigned long long* pairs,
+                                    void* ws, size_t ws_bytes, bh_stream s) {
+  return forward_culled("bh_forward_sum_modes", b, ndim, npos, mirror, c, ts, nt, seg, cull_log2,
+                        precision, complex_out, out, pairs, ws, ws_bytes, s);
 }
 
 extern "C" size_t bh_static_sum_culled_workspace_size(int64_t count, int ndim, int64_t P) {
@@ -320,7 +372,9 @@ extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh
   hipStream_t st = (hipStream_t)s;
   const bool f64 = precision == BH_PRECISION_FP64;
   if (b->count == 0) {
-    BH_HIP(hipMemsetAsync(out, 0, (size_t)seg->S * (f64 ? 8 : 4), st));
+    const long long n4 = (long long)seg->S * (f64 ? 2 : 1);
+    hipLaunchKernelGGL(k_zero_out, dim3(bh::grid1d(n4, 256)), dim3(256), 0, st, out, n4);
+    BH_LAUNCH_CHECK();
     return BH_OK;
   }
   const size_t rec = (bh_static_sum_workspace_size(b->count, ndim) + 255) / 256 * 256;
@@ -329,6 +383,8 @@ extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh
   CulledArgs a{};
   a.count = b->count;
   a.total = b->count;
+  a.ostride = 1;
+  a.ooff = 0;
   a.recs = ws;
   a.G = to_segdev(seg);
   a.E = cull_log2;

@@ -328,12 +328,13 @@ size_t grid_fft_ws(const bh_tiling* t) {
 }  // namespace
 
 namespace {
-// cpos = 0.5 (a + i b / vg), vg = c |2 pi centre / L| of the coefficient's box
-// (forward_solver_utils.py:763-781); optional half mask
+// cpos = 0.5 (a + i b / vg), cneg = 0.5 (a - i b / vg), vg = c |2 pi centre / L|
+// of the coefficient's box (forward_solver_utils.py:763-785); b == nullptr
+// stands for b = 0; optional half mask on cpos ('pos_only')
 __global__ void k_combine_velocity(TilingDev T, const int32_t* __restrict__ centres, double L0,
                                    double L1, double L2, double c, const double2* __restrict__ a,
                                    const double2* __restrict__ b, int half,
-                                   double2* __restrict__ out) {
+                                   double2* __restrict__ pos, double2* __restrict__ neg) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const long long total = T.coeffs_cumsum[T.num_levels];
   if (e >= total) return;
@@ -348,27 +349,31 @@ __global__ void k_combine_velocity(TilingDev T, const int32_t* __restrict__ cent
     n2 += p * p;
   }
   const double vg = c * sqrt(n2);
-  const double2 av = a[e], bv = b[e];
-  double2 r = make_double2(0.5 * (av.x + (-bv.y) / vg), 0.5 * (av.y + bv.x / vg));
+  const double2 av = a[e];
+  const double2 bv = b ? b[e] : make_double2(0.0, 0.0);
+  // i b / vg = (-b.im / vg, b.re / vg)
+  const double ur = (-bv.y) / vg, ui = bv.x / vg;
+  double2 r = make_double2(0.5 * (av.x + ur), 0.5 * (av.y + ui));
   const long long seg = T.coeffs_cumsum[lvl + 1] - T.coeffs_cumsum[lvl];
   if (half && (e - T.coeffs_cumsum[lvl]) >= seg / 2) r = make_double2(0.0, 0.0);
-  out[e] = r;
+  pos[e] = r;
+  if (neg) neg[e] = make_double2(0.5 * (av.x - ur), 0.5 * (av.y - ui));
 }
 }  // namespace
 
 extern "C" int bh_mswpt_combine_velocity(const bh_tiling* t, const double* grid_size, double c,
                                          const double* a, const double* b, int half_mask,
-                                         double* out, bh_stream s) {
+                                         double* cpos, double* cneg, bh_stream s) {
   bh::clear_error();
   if (int rc = bh::check_tiling(t)) return rc;
-  if (!grid_size || !a || !b || !out) BH_FAIL(BH_ERR_ARG, "bh_mswpt_combine_velocity: null pointer");
+  if (!grid_size || !a || !cpos) BH_FAIL(BH_ERR_ARG, "bh_mswpt_combine_velocity: null pointer");
   if (!(c > 0.0)) BH_FAIL(BH_ERR_ARG, "bh_mswpt_combine_velocity: c must be positive");
   const bh::TilingDev T = bh::to_dev(t);
   const long long nc = t->coeffs_cumsum[t->num_levels];
   hipLaunchKernelGGL(k_combine_velocity, dim3(bh::grid1d(nc, 256)), dim3(256), 0, (hipStream_t)s,
                      T, t->centres, grid_size[0], t->ndim > 1 ? grid_size[1] : 1.0,
                      t->ndim > 2 ? grid_size[2] : 1.0, c, (const double2*)a, (const double2*)b,
-                     half_mask, (double2*)out);
+                     half_mask, (double2*)cpos, (double2*)cneg);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }

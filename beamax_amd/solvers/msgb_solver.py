@@ -90,9 +90,15 @@ class ForwardBeams:
         self.alpha = torch.empty((k, d), dtype=torch.complex128, device=device)
         self.a0 = torch.empty((k,), dtype=torch.complex128, device=device)
 
-    def struct(self) -> "_lib.FwdBeams":
-        return _lib.FwdBeams(self.count, self.x0.data_ptr(), self.p0.data_ptr(),
-                             self.omega.data_ptr(), self.alpha.data_ptr(), self.a0.data_ptr())
+    def struct(self, offset: int = 0, count=None) -> "_lib.FwdBeams":
+        """C view of rows [offset, offset + count) (all rows by default)."""
+        n = self.count - offset if count is None else int(count)
+        d = self.d
+        return _lib.FwdBeams(n, self.x0.data_ptr() + 8 * d * offset,
+                             self.p0.data_ptr() + 8 * d * offset,
+                             self.omega.data_ptr() + 8 * offset,
+                             self.alpha.data_ptr() + 16 * d * offset,
+                             self.a0.data_ptr() + 16 * offset)
 
 
 class StaticBeams:
@@ -232,12 +238,13 @@ class MSGBSolver:
         cap = wpt.half_frame_capacity if half_frame else wpt.total_coeffs
         return min(int(self.thr), int(cap))
 
-    def _check_supported(self):
+    def _check_supported(self, complex_sum_ok: bool = False):
         if self.thr_strat != "top_n":
             raise NotImplementedError(f"thr_strat={self.thr_strat!r} is not on the GPU path; "
                                       "use 'top_n'.")
-        if not self.use_real:
-            raise NotImplementedError("complex beam summation is not on the GPU path; use *_real.")
+        if not self.use_real and not complex_sum_ok:
+            raise NotImplementedError("complex beam summation is built for the forward sum only; "
+                                      "use *_real here.")
         if self.sharding is not None:
             raise NotImplementedError("multi-device sharding is not part of this single-GPU build")
         if self.ode_solver is not gb_solvers.solve_hom_diag:
@@ -266,7 +273,7 @@ class MSGBSolver:
             cpos = torch.empty_like(a)
             _lib.call("bh_mswpt_combine_velocity", C.byref(wpt.tiling),
                       _lib.dbl_array(domain.grid_size), float(domain.c), _lib.ptr(a), _lib.ptr(b),
-                      1, _lib.ptr(cpos), _lib.stream_handle())
+                      1, _lib.ptr(cpos), None, _lib.stream_handle())
         k = self._effective_top_n(wpt, half_frame=True)
         idx, val = topn_device(cpos, k)
         beams = ForwardBeams(idx.numel(), domain.ndim, wpt.device)
@@ -285,11 +292,38 @@ class MSGBSolver:
             self._seg_cache[key] = seg
         return seg
 
-    def sum_forward(self, beams: ForwardBeams, domain: Domain, sensor_positions, ts, out=None):
-        """Mirrored real beam sum at the sensors, (Nt, S)."""
+    def sum_forward(self, beams: ForwardBeams, domain: Domain, sensor_positions, ts, out=None,
+                    npos=None):
+        """Beam sum at the sensors, (Nt, S): real (2 Re sum, ``*_real`` sum
+        methods) or complex.  npos=None mirrors every beam (modes +1 and -1);
+        otherwise beams [0, npos) have mode +1 and the rest mode -1."""
         import torch
         dev = beams.x0.device
         d = domain.ndim
+        if npos is not None or not self.use_real:
+            if any(domain.periodic):
+                raise NotImplementedError("complex-input or complex-sum forward with periodic "
+                                          "boundaries is not on the GPU path")
+            pts = np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d)
+            seg = self._segments(pts, domain.dx, dev)
+            tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
+            f64 = self.precision == "fp64"
+            if self.use_real:
+                dt = torch.float64 if f64 else torch.float32
+            else:
+                dt = torch.complex128 if f64 else torch.complex64
+            if out is None:
+                out = torch.empty((tsd.numel(), seg.S), dtype=dt, device=dev)
+            b = beams.struct()
+            ss = seg.struct()
+            ws = _lib.workspace(
+                _lib.load().bh_forward_sum_culled_workspace_size(tsd.numel(), seg.S), dev)
+            _lib.call("bh_forward_sum_modes", C.byref(b), d, 0 if npos is None else int(npos),
+                      int(npos is None), float(domain.c), _lib.ptr(tsd), tsd.numel(), C.byref(ss),
+                      self.cull_log2, _lib.PRECISION[self.precision], int(not self.use_real),
+                      _lib.ptr(out), _lib.ptr(self.pair_counter), _lib.ptr(ws), ws.numel(),
+                      _lib.stream_handle())
+            return out
         if self.cull_log2 > 0 and not any(domain.periodic):
             pts = np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d)
             seg = self._segments(pts, domain.dx, dev)
@@ -339,24 +373,51 @@ class MSGBSolver:
         pos = self._sensor_positions(sensors)
         if pos is None:
             raise ValueError("Unsupported sensor type")
-        is_complex = (p0.is_complex() if isinstance(p0, torch.Tensor) else np.iscomplexobj(p0))
-        if is_complex:
-            raise NotImplementedError("complex p0 (msgb_solver.py:635-689) is not on the GPU path")
+        cplx = lambda x: x.is_complex() if isinstance(x, torch.Tensor) else np.iscomplexobj(x)
+        is_complex = cplx(p0) or (dpdt is not None and cplx(dpdt))
         if dpdt is not None:
             nz = (torch.count_nonzero(dpdt).item() if isinstance(dpdt, torch.Tensor)
                   else int(np.count_nonzero(np.asarray(dpdt))))
             if nz == 0:
-                dpdt = None   # 0.5 * (a + 1j * 0 / vg) == 0.5 * a exactly
-            elif (dpdt.is_complex() if isinstance(dpdt, torch.Tensor) else np.iscomplexobj(dpdt)):
-                raise NotImplementedError("complex dpdt (msgb_solver.py:635-689) is not on the GPU path")
+                dpdt = None   # 0.5 * (a +- 1j * 0 / vg) == 0.5 * a exactly
         if not domain.is_homogeneous:
             raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
-        self._check_supported()
+        self._check_supported(complex_sum_ok=True)
+        out_shape = tuple(np.asarray(_np(sensors.positions) if isinstance(sensors, Sensor)
+                                     else pos).shape[:-1])
+        if is_complex:
+            beams, npos = self.prepare_forward_beams_complex(p0, domain, wpt, dpdt=dpdt)
+            data = self.sum_forward(beams, domain, pos, ts, npos=npos)
+            return (data.reshape((data.shape[0],) + out_shape),
+                    self._params_tuple(beams, npos=npos))
         beams, _, _ = self.prepare_forward_beams(p0, domain, wpt, dpdt=dpdt)
         data = self.sum_forward(beams, domain, pos, ts)
-        out_shape = (data.shape[0],) + tuple(np.asarray(_np(sensors.positions) if isinstance(sensors, Sensor) else pos).shape[:-1])
-        data = data.reshape(out_shape)
+        data = data.reshape((data.shape[0],) + out_shape)
         return data, self._params_tuple(beams)
+
+    def prepare_forward_beams_complex(self, p0, domain: Domain, wpt: MSWPT, dpdt=None):
+        """Complex-input beams (msgb_solver.py:635-689): cpos and cneg of
+        compute_coefficients('both') thresholded separately (capacity
+        total_coeffs each); returns (ForwardBeams [pos | neg], npos)."""
+        import torch
+        a = wpt.forward(p0, self.input_type)
+        b = wpt.forward(dpdt, self.input_type) if dpdt is not None else None
+        cpos = torch.empty_like(a)
+        cneg = torch.empty_like(a)
+        _lib.call("bh_mswpt_combine_velocity", C.byref(wpt.tiling),
+                  _lib.dbl_array(domain.grid_size), float(domain.c), _lib.ptr(a), _lib.ptr(b), 0,
+                  _lib.ptr(cpos), _lib.ptr(cneg), _lib.stream_handle())
+        k = self._effective_top_n(wpt)
+        ip, vp = topn_device(cpos, k)
+        ineg, vneg = topn_device(cneg, k)
+        kp, kn = ip.numel(), ineg.numel()
+        beams = ForwardBeams(kp + kn, domain.ndim, wpt.device)
+        gs = _lib.dbl_array(domain.grid_size)
+        for off, idx, val in ((0, ip, vp), (kp, ineg, vneg)):
+            _lib.call("bh_forward_params", C.byref(wpt.tiling), gs, _lib.ptr(idx), _lib.ptr(val),
+                      idx.numel(), beams.struct(off, idx.numel()), _lib.stream_handle())
+        beams.idx = (ip, ineg)
+        return beams, kp
 
     def solve_ivp(self, p0, dpdt, domain: Domain, wpt: MSWPT, sensors, ts):
         """Wave-equation IVP with initial velocity (msgb_solver.py:1113-1147)."""
@@ -365,15 +426,20 @@ class MSGBSolver:
     def solve_ivp_with_params(self, p0, dpdt, domain: Domain, wpt: MSWPT, sensors, ts):
         return self.forward_with_params(p0, domain, sensors, ts, wpt, dpdt=dpdt)
 
-    def _params_tuple(self, beams: ForwardBeams):
-        """Mirrored parameters in the reference's tuple order (batched for scan/vmap)."""
+    def _params_tuple(self, beams: ForwardBeams, npos=None):
+        """Parameters in the reference's tuple order (batched for scan/vmap):
+        mirrored (npos None) or [pos | neg] with modes +1 / -1."""
         import torch
-        d = beams.d
         M0 = torch.diag_embed(beams.alpha)
         ones = torch.ones(beams.count, dtype=torch.float64, device=beams.x0.device)
-        dup = lambda t: torch.cat([t, t], dim=0)
-        params = (dup(beams.p0), dup(M0), dup(beams.x0), dup(beams.omega), dup(beams.a0),
-                  torch.cat([ones, -ones]))
+        if npos is None:
+            dup = lambda t: torch.cat([t, t], dim=0)
+            params = (dup(beams.p0), dup(M0), dup(beams.x0), dup(beams.omega), dup(beams.a0),
+                      torch.cat([ones, -ones]))
+        else:
+            modes = ones.clone()
+            modes[int(npos):] = -1.0
+            params = (beams.p0, M0, beams.x0, beams.omega, beams.a0, modes)
         if self.aggregate_method in ("scan", "vmap"):
             params = batch_params(params, self.batch_size, zero_padded_args=(4,))
         return params

@@ -36,6 +36,8 @@
  *                       (forward_solver_utils.py:755-786)
  *   bh_fourier_crop_time  prepare_data_domain_for_msgb Fourier crop
  *                       (examples/reconstruction/2d_time_reversal_and_adjoint.py:111-165)
+ *   bh_forward_sum_modes  the complex-input forward (msgb_solver.py:635-689)
+ *                       and the complex beam sum (gb/core.py:105-168)
  *   bh_forward_sum_culled, bh_static_sum_culled: the same sums with a certified
  *                       envelope cut (no reference counterpart; same seams)
  *   bh_forward_transpose, bh_coeff_scatter, bh_coeff_gather, bh_mswpt_adjoint: the exact
@@ -139,12 +141,13 @@ size_t bh_mswpt_workspace_size(const bh_tiling* t);
 int bh_mswpt_forward(const bh_tiling* t, const double* sum_gsquare, const double* gpacked,
                      const double* input, int input_kind, double scale, int half_mask,
                      double* coeffs, void* ws, size_t ws_bytes, bh_stream s);
-/* cpos = 0.5 (a + i b / vg) for the coefficients a = MSWPT(p0),
- * b = MSWPT(dpdt); vg = c |2 pi centre / L| of each coefficient's box; with
- * half_mask the second half of every level is zeroed ('pos_only'). */
+/* cpos = 0.5 (a + i b / vg) and (cneg nullable) cneg = 0.5 (a - i b / vg)
+ * for the coefficients a = MSWPT(p0), b = MSWPT(dpdt) (b NULL: dpdt = 0);
+ * vg = c |2 pi centre / L| of each coefficient's box; with half_mask the
+ * second half of every level of cpos is zeroed ('pos_only'). */
 int bh_mswpt_combine_velocity(const bh_tiling* t, const double* grid_size, double c,
-                              const double* a, const double* b, int half_mask, double* out,
-                              bh_stream s);
+                              const double* a, const double* b, int half_mask, double* cpos,
+                              double* cneg, bh_stream s);
 /* out[prod N] complex = MSWPT^-1(coeffs) in the spatial (output_fourier = 0)
  * or centred Fourier (1) domain. */
 int bh_mswpt_inverse(const bh_tiling* t, const double* gpacked, const double* coeffs,
@@ -255,6 +258,20 @@ int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror, double c,
                           const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
                           int precision, void* out, unsigned long long* pairs, void* ws,
                           size_t ws_bytes, bh_stream s);
+/* bh_forward_sum_culled with per-beam modes and an optional complex sum:
+ * mirror = 0: beams [0, npos) propagate with mode +1 and [npos, count) with
+ * mode -1 (the complex-input path of msgb_solver.py:635-689, whose positive
+ * and negative coefficient sets are thresholded separately); mirror != 0:
+ * every beam twice, modes +1 and -1 (npos ignored), as bh_forward_sum_culled.  complex_out = 0
+ * writes the real sum 2 Re sum A G (compute_gaussian_beam_real,
+ * gb/core.py:194-329); complex_out = 1 writes sum A G (compute_gaussian_beam
+ * summed over beams, gb/core.py:105-168) as interleaved complex (nt, S), FP32
+ * pairs in FP32-split, FP64 pairs in FP64.  Same workspace as
+ * bh_forward_sum_culled. */
+int bh_forward_sum_modes(const bh_fwd_beams* b, int ndim, int64_t npos, int mirror, double c,
+                         const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
+                         int precision, int complex_out, void* out, unsigned long long* pairs,
+                         void* ws, size_t ws_bytes, bh_stream s);
 size_t bh_static_sum_culled_workspace_size(int64_t count, int ndim, int64_t P);
 int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
                          double scale, double cull_log2, int precision, void* out,

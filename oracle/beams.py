@@ -250,3 +250,35 @@ def forward_pipeline(p0, wpt: MSWPT, grid_size, N, c, ts, sensors, thr):
     xt, pt, Mt, At = solve_hom_diag(X, P, AL, A0, MO, np.asarray(ts), c)
     data = real_beam_sum(xt, pt, Mt, At, OM, np.asarray(sensors))
     return data, (idx, val, P, AL, X, OM, A0, MO)
+
+
+def complex_forward_pipeline(p0, dpdt, wpt: MSWPT, grid_size, N, c, ts, sensors, thr,
+                             use_real=True):
+    """MSGBSolver.forward for complex p0 or dpdt (msgb_solver.py:635-689):
+    cpos and cneg of compute_coefficients('both') each thresholded with
+    top_n (capacity total_coeffs), positive beams with mode +1 and negative
+    with mode -1, no mirroring; summed real (gb/core.py:194-329) or complex
+    (gb/core.py:105-168 summed over beams)."""
+    from .topn import top_n
+    if dpdt is None:
+        dpdt = np.zeros_like(np.asarray(p0, dtype=complex).real)
+    cpos, cneg = compute_coefficients(wpt, p0, dpdt, c, grid_size, mode="both")
+    k = min(int(thr), wpt.total_coeffs)
+    ip, vp = top_n(cpos, k)
+    ineg, vn = top_n(cneg, k)
+    parts = [forward_parameters(wpt, ip, grid_size, N), forward_parameters(wpt, ineg, grid_size, N)]
+    P = np.concatenate([parts[0][0], parts[1][0]])
+    AL = np.concatenate([parts[0][1], parts[1][1]])
+    X = np.concatenate([parts[0][2], parts[1][2]])
+    OM = np.concatenate([parts[0][3], parts[1][3]])
+    A0 = np.concatenate([parts[0][4] * vp, parts[1][4] * vn])
+    MO = np.concatenate([np.ones(len(ip)), -np.ones(len(ineg))])
+    xt, pt, Mt, At = solve_hom_diag(X, P, AL, A0, MO, np.asarray(ts), c)
+    if use_real:
+        data = real_beam_sum(xt, pt, Mt, At, OM, np.asarray(sensors))
+    else:
+        data = np.zeros((len(ts), np.asarray(sensors).shape[0]), dtype=complex)
+        for b in range(len(OM)):
+            data += complex_beam(xt[b:b + 1], pt[b:b + 1], Mt[b:b + 1], At[b:b + 1],
+                                 OM[b:b + 1], np.asarray(sensors))[0]
+    return data, (ip, vp, ineg, vn)

@@ -281,3 +281,74 @@ def test_solve_ivp_with_velocity_matches_oracle(gpu):
     # test_msgb_solver_misc.py:253-270: a zero velocity reproduces forward()
     z = _n(s.solve_ivp(p0, np.zeros(N), dom, wpt, sensors, ts))
     assert np.array_equal(z, _n(s.forward(p0, dom, sensors, ts, wpt)))
+
+
+@pytest.mark.parametrize("use_real", [True, False])
+def test_complex_input_forward_matches_oracle(gpu, use_real):
+    """msgb_solver.py:635-689: complex p0 (and complex dpdt) thresholded per
+    sign; real (scan_real) or complex (scan) sums."""
+    import torch
+    N = (64, 64)
+    rng = np.random.default_rng(81)
+    dom = bx.Domain(N=N, dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    wpt = bx.MSWPT(bx.DyadicDecomposition(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror")
+    ow = OMSWPT(Tiling(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror")
+    p0 = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    dpdt = (rng.standard_normal(N) + 1j * rng.standard_normal(N)) * 1e7
+    ts = dom.generate_time_domain()[:70]
+    mask = np.zeros(N)
+    mask[0, :] = 1
+    mask[:, 0] = 1
+    sensors = bx.Sensor(dom, binary_mask=mask)
+    sm = "scan_real" if use_real else "scan_complex"
+    for dv in (None, dpdt):
+        ref, (ip, _, ineg, _) = obeams.complex_forward_pipeline(
+            p0, dv, ow, dom.grid_size, N, 1500.0, ts, sensors.positions, 400, use_real)
+        for prec, E, tol in (("fp64", 0.0, 1e-12), ("fp64", 40.0, 1e-10), ("fp32_split", 40.0, 1e-5)):
+            s = bx.MSGBSolver(thr=400, thr_strat="top_n", batch_size=64, input_type="spatial",
+                              ode_solver=gb_solvers.solve_hom_diag, sum_method=sm, precision=prec,
+                              cull_log2=E)
+            got, params = s.forward_with_params(p0, dom, sensors, ts, wpt, dpdt=dv)
+            got = _n(got)
+            assert got.dtype == (np.float64 if use_real else np.complex128) or prec != "fp64"
+            assert rel(ref, got) < tol, (dv is None, prec, E)
+            modes = _n(params[5]).reshape(-1)[:800]   # scan batches pad with the last entry
+            assert np.sum(modes == 1) == 400 and np.sum(modes == -1) == 400
+
+
+def test_real_input_complex_sum_matches_oracle(gpu):
+    """sum_method "scan_complex" (gb/core.py:105-168) with real p0: the mirrored beams
+    summed as complex Gaussian beams."""
+    N = (64, 64)
+    rng = np.random.default_rng(82)
+    dom = bx.Domain(N=N, dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    wpt = bx.MSWPT(bx.DyadicDecomposition(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror")
+    ow = OMSWPT(Tiling(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror")
+    p0 = rng.standard_normal(N)
+    ts = dom.generate_time_domain()[:50]
+    mask = np.zeros(N)
+    mask[-1, :] = 1
+    sensors = bx.Sensor(dom, binary_mask=mask)
+    L = dom.grid_size
+    cpos = obeams.compute_coefficients_pos_only(ow, p0, 1500.0, L)
+    idx, val = otopn.top_n(cpos, 500)
+    p, al, x0, om, a0, mode = obeams.forward_parameters(ow, idx, L, N)
+    dup = lambda v: np.concatenate([v, v])
+    xt, pt, Mt, At = obeams.solve_hom_diag(dup(x0), dup(p), dup(al), dup(a0 * val),
+                                           np.concatenate([mode, -mode]), ts, 1500.0)
+    ref = np.zeros((ts.size, sensors.positions.shape[0]), dtype=complex)
+    for b in range(xt.shape[0]):
+        ref += obeams.complex_beam(xt[b:b + 1], pt[b:b + 1], Mt[b:b + 1], At[b:b + 1],
+                                   dup(om)[b:b + 1], sensors.positions)[0]
+    for prec, tol in (("fp64", 1e-12), ("fp32_split", 1e-5)):
+        s = bx.MSGBSolver(thr=500, thr_strat="top_n", batch_size=64, input_type="spatial",
+                          ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_complex", precision=prec,
+                          cull_log2=0.0 if prec == "fp64" else 40.0)
+        got = _n(s.forward(p0, dom, sensors, ts, wpt))
+        assert np.iscomplexobj(got)
+        assert rel(ref, got) < tol
+        # its real part is half the scan_real result (2 Re sum)
+        sr = bx.MSGBSolver(thr=500, thr_strat="top_n", batch_size=64, input_type="spatial",
+                           ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real",
+                           precision=prec, cull_log2=0.0 if prec == "fp64" else 40.0)
+        assert rel(_n(sr.forward(p0, dom, sensors, ts, wpt)), 2 * got.real) < tol
