@@ -69,6 +69,9 @@ SIGNATURES = {
                                    _P, _P, _SZ, _P]),
     "bh_mswpt_inverse": (C.c_int, [C.POINTER(Tiling), _P, _P, C.c_int, _P, _P, _SZ, _P]),
     "bh_topn_workspace_size": (_SZ, [_I64, _I64]),
+    "bh_threshold_workspace_size": (_SZ, [_I64]),
+    "bh_threshold": (C.c_int, [_P, _I64, C.c_int, C.c_double, C.POINTER(Tiling), _P, _P, _P, _P, _SZ,
+                               _P]),
     "bh_topn": (C.c_int, [_P, _I64, _I64, _P, _P, _P, _SZ, _P]),
     "bh_forward_params": (C.c_int, [C.POINTER(Tiling), C.POINTER(C.c_double), _P, _P, _I64,
                                     FwdBeams, _P]),

@@ -46,4 +46,11 @@ struct TilingDev {
 int check_tiling(const bh_tiling* t);
 TilingDev to_dev(const bh_tiling* t);
 
+// The k-th largest (1 <= k <= n) of n keys that are bit patterns of
+// non-negative doubles (so they order like the values), by the top_n radix
+// select; written to *out (device).  ws: kth_key_ws_bytes() bytes.
+size_t kth_key_ws_bytes();
+void kth_largest_key(const unsigned long long* keys, long long n, long long k,
+                     unsigned long long* out, void* ws, hipStream_t s);
+
 }  // namespace bh

@@ -299,6 +299,28 @@ TopnWs carve(void* ws, long long n, long long k) {
 
 }  // namespace
 
+namespace {
+__global__ void k_emit_key(const SelState* st, unsigned long long* out) { *out = st->prefix; }
+}  // namespace
+
+namespace bh {
+size_t kth_key_ws_bytes() { return 256 * 4 + 256; }
+
+void kth_largest_key(const unsigned long long* keys, long long n, long long k,
+                     unsigned long long* out, void* ws, hipStream_t s) {
+  unsigned int* hist = (unsigned int*)ws;
+  SelState* st = (SelState*)((char*)ws + 256 * 4);
+  const long long nblk = (n + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(k_init_state, dim3(1), dim3(256), 0, s, st, hist, k);
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_hist, dim3((unsigned)nblk), dim3(kSelBlock), 0, s, keys, n, st, shift,
+                       hist);
+    hipLaunchKernelGGL(k_pick, dim3(1), dim3(256), 0, s, st, hist, shift);
+  }
+  hipLaunchKernelGGL(k_emit_key, dim3(1), dim3(1), 0, s, st, out);
+}
+}  // namespace bh
+
 extern "C" size_t bh_topn_workspace_size(int64_t n, int64_t k) {
   if (n <= 0 || k <= 0) return 0;
   if (k > n) k = n;

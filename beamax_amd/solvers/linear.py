@@ -28,7 +28,7 @@ import numpy as np
 from .. import _lib
 from ..geometry import Domain, Sensor
 from ..transforms import MSWPT
-from .msgb_solver import ForwardBeams, MSGBSolver, _np, topn_device
+from .msgb_solver import ForwardBeams, MSGBSolver, _np
 
 
 class FrozenForward:
@@ -77,7 +77,7 @@ class FrozenForward:
     def from_field(cls, solver: MSGBSolver, p, domain: Domain, sensors, ts, wpt: MSWPT):
         """Freeze the index set the reference's forward would select for ``p``."""
         cpos = wpt.forward(p, solver.input_type, scale=0.5, half_mask=True)
-        idx, _ = topn_device(cpos, solver._effective_top_n(wpt, half_frame=True))
+        idx, _ = solver._select(cpos, wpt, half_frame=True)
         return cls(solver, domain, sensors, ts, wpt, idx)
 
     @property

@@ -145,6 +145,37 @@ def topn_device(coeffs, k: int):
     return idx, val
 
 
+_THR_CODE = {"hard": 0, "percentile": 1, "hard_reassign": 2, "bao_energy": 3, "perc_max_abs": 4}
+
+
+def threshold_device(coeffs, strategy: str, val, wpt: MSWPT = None):
+    """threshold_coefficients (forward_solver_utils.py:181-230) on the GPU:
+    (idx int64, values complex128).  top_n returns ascending (|c|, index);
+    the other strategies return ascending indices, only the selected entries
+    (the reference pads percentile/perc_max_abs to n with index -1 and value
+    0, i.e. zero-amplitude beams).  The count is read back to the host, as
+    the reference thresholds eagerly."""
+    import torch
+    if strategy == "top_n":
+        return topn_device(coeffs, val)
+    if strategy not in _THR_CODE:
+        raise ValueError(f"Invalid thresholding strategy: {strategy}")
+    if strategy == "bao_energy" and wpt is None:
+        raise ValueError("bao_energy thresholding requires wpt.")
+    n = coeffs.numel()
+    dev = coeffs.device
+    idx = torch.empty(n, dtype=torch.int64, device=dev)
+    vals = torch.empty(n, dtype=torch.complex128, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = _lib.workspace(_lib.load().bh_threshold_workspace_size(n), dev)
+    t = C.byref(wpt.tiling) if strategy == "bao_energy" else None
+    _lib.call("bh_threshold", _lib.ptr(coeffs), n, _THR_CODE[strategy], float(val), t,
+              _lib.ptr(idx), _lib.ptr(vals), _lib.ptr(cnt), _lib.ptr(ws), ws.numel(),
+              _lib.stream_handle())
+    k = int(cnt.item())
+    return idx[:k].contiguous(), vals[:k].contiguous()
+
+
 def detector_plane(positions, domain: Domain):
     """Normal axis, normal coordinate and inward sign of a planar detector
     (tr_solver_utils.py:245-250 find_constant_columns; 356-371 inward sign)."""
@@ -238,10 +269,12 @@ class MSGBSolver:
         cap = wpt.half_frame_capacity if half_frame else wpt.total_coeffs
         return min(int(self.thr), int(cap))
 
+    def _select(self, coeffs, wpt: MSWPT, *, half_frame: bool = False):
+        """threshold_coefficients with this solver's strategy and threshold."""
+        return threshold_device(coeffs, self.thr_strat,
+                                self._effective_top_n(wpt, half_frame=half_frame), wpt)
+
     def _check_supported(self, complex_sum_ok: bool = False):
-        if self.thr_strat != "top_n":
-            raise NotImplementedError(f"thr_strat={self.thr_strat!r} is not on the GPU path; "
-                                      "use 'top_n'.")
         if not self.use_real and not complex_sum_ok:
             raise NotImplementedError("complex beam summation is built for the forward sum only; "
                                       "use *_real here.")
@@ -274,8 +307,7 @@ class MSGBSolver:
             _lib.call("bh_mswpt_combine_velocity", C.byref(wpt.tiling),
                       _lib.dbl_array(domain.grid_size), float(domain.c), _lib.ptr(a), _lib.ptr(b),
                       1, _lib.ptr(cpos), None, _lib.stream_handle())
-        k = self._effective_top_n(wpt, half_frame=True)
-        idx, val = topn_device(cpos, k)
+        idx, val = self._select(cpos, wpt, half_frame=True)
         beams = ForwardBeams(idx.numel(), domain.ndim, wpt.device)
         gs = _lib.dbl_array(domain.grid_size)
         _lib.call("bh_forward_params", C.byref(wpt.tiling), gs, _lib.ptr(idx), _lib.ptr(val),
@@ -407,9 +439,8 @@ class MSGBSolver:
         _lib.call("bh_mswpt_combine_velocity", C.byref(wpt.tiling),
                   _lib.dbl_array(domain.grid_size), float(domain.c), _lib.ptr(a), _lib.ptr(b), 0,
                   _lib.ptr(cpos), _lib.ptr(cneg), _lib.stream_handle())
-        k = self._effective_top_n(wpt)
-        ip, vp = topn_device(cpos, k)
-        ineg, vneg = topn_device(cneg, k)
+        ip, vp = self._select(cpos, wpt)
+        ineg, vneg = self._select(cneg, wpt)
         kp, kn = ip.numel(), ineg.numel()
         beams = ForwardBeams(kp + kn, domain.ndim, wpt.device)
         gs = _lib.dbl_array(domain.grid_size)
@@ -503,7 +534,7 @@ class MSGBSolver:
         solve_hom_TR to t = 0 (msgb_solver.py:1168-1233, 1347-1366)."""
         src = self.adjoint_source(data, data_domain.dx[0], c, window)
         coeffs = data_wpt.forward(src, self.input_type)
-        idx, val = topn_device(coeffs, self._effective_top_n(data_wpt))
+        idx, val = self._select(coeffs, data_wpt)
         if plane is None:
             plane = detector_plane(sources.positions, sources.domain)
         normal_axis, normal_value, inward = plane
@@ -559,7 +590,7 @@ class MSGBSolver:
         top_n -> compute_TR_parameters -> solve_hom_TR to t = 0
         (msgb_solver.py:691-746)."""
         coeffs = data_wpt.forward(data, self.input_type, scale=0.5)
-        idx, val = topn_device(coeffs, self._effective_top_n(data_wpt))
+        idx, val = self._select(coeffs, data_wpt)
         if plane is None:
             plane = detector_plane(sources.positions, sources.domain)
         normal_axis, normal_value, inward = plane
@@ -590,7 +621,7 @@ class MSGBSolver:
         self._validate_boundary_data(data, data_domain, sources, ts=ts)
         if not sources.domain.is_homogeneous:
             raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
-        if self.thr_strat != "top_n" or self.sharding is not None:
+        if self.sharding is not None:
             self._check_supported()
         if not isinstance(data, torch.Tensor):
             data = torch.as_tensor(np.asarray(data, dtype=np.float64), device=data_wpt.device)
@@ -620,7 +651,7 @@ class MSGBSolver:
         self._validate_boundary_data(data, data_domain, sources, ts=ts)
         if not (domain.is_homogeneous and sources.domain.is_homogeneous):
             raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
-        if self.thr_strat != "top_n" or self.sharding is not None:
+        if self.sharding is not None:
             self._check_supported()
         if not isinstance(data, torch.Tensor):
             data = torch.as_tensor(np.asarray(data, dtype=np.float64), device=data_wpt.device)

@@ -21,6 +21,8 @@
  *   bh_fft              utils.unitary_fft/unitary_ifft building block
  *                       (beamax/utils/fft.py:5-38)
  *   bh_topn             _threshold_top_n (forward_solver_utils.py:65-89)
+ *   bh_threshold        _threshold_hard / _percentile / _hard_reassign /
+ *                       _bao_energy / _perc_max_abs (forward_solver_utils.py:14-178)
  *   bh_forward_params   compute_forward_parameters (forward_solver_utils.py:255-373)
  *   bh_forward_sum      compute_forward_result with solve_hom_diag and
  *                       compute_gaussian_beam_real (forward_solver_utils.py:677-713,
@@ -159,6 +161,18 @@ int bh_mswpt_inverse(const bh_tiling* t, const double* gpacked, const double* co
 size_t bh_topn_workspace_size(int64_t n, int64_t k);
 int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out, double* val_out,
             void* ws, size_t ws_bytes, bh_stream s);
+
+/* The other thresholding strategies (forward_solver_utils.py:14-178):
+ * strategy 0 hard (|c| > val), 1 percentile (|c| > the val-th percentile of
+ * |c|, linear interpolation), 2 hard_reassign (|c| >= val max|c|, values
+ * scaled by sqrt(total / retained energy)), 3 bao_energy (|c| |centre|^d >
+ * val; needs the tiling t), 4 perc_max_abs (|c| > val max|c|).  Writes the
+ * selected indices (ascending) and values to idx_out[0..count) and
+ * val_out[0..count) (capacity n each) and count to *count_out (device). */
+size_t bh_threshold_workspace_size(int64_t n);
+int bh_threshold(const double* c, int64_t n, int strategy, double val, const bh_tiling* t,
+                 int64_t* idx_out, double* val_out, int64_t* count_out, void* ws,
+                 size_t ws_bytes, bh_stream s);
 
 /* Forward beam parameters for flat coefficient indices idx[k] with values
  * vals[k] (complex); grid_size = N*dx (host, ndim). */

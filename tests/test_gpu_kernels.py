@@ -383,3 +383,35 @@ def test_culled_static_matches_dense(gpu, d):
                   _lib.PRECISION[prec], _lib.ptr(out), None, _lib.ptr(ws), ws.numel(),
                   _lib.stream_handle())
         assert rel(ref, _n(out).astype(np.float64)) < tol
+
+
+# ---------------------------------------------------------------------------
+# the other thresholding strategies
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["random", "ties"])
+@pytest.mark.parametrize("strategy,val", [("hard", 1.1), ("hard", 0.0), ("percentile", 0.0),
+                                          ("percentile", 37.5), ("percentile", 99.0),
+                                          ("percentile", 100.0), ("hard_reassign", 0.25),
+                                          ("bao_energy", 30.0), ("perc_max_abs", 0.5),
+                                          ("perc_max_abs", 1.0)])
+def test_threshold_strategies_match_oracle(gpu, strategy, val, kind):
+    from beamax_amd.solvers.msgb_solver import threshold_device
+    w, o = _pair((2, (64, 32), (4, 8), (2, 1)))
+    rng = np.random.default_rng(90)
+    n = w.total_coeffs
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    if kind == "ties":
+        c = _tie_heavy(rng, n)
+    c[rng.random(n) < 0.4] = 0.0
+    fn = {"hard": otopn.threshold_hard, "percentile": otopn.threshold_percentile,
+          "hard_reassign": otopn.threshold_hard_reassign,
+          "perc_max_abs": otopn.threshold_perc_max_abs}
+    if strategy == "bao_energy":
+        ridx, rval = otopn.threshold_bao_energy(c, val, o)
+    else:
+        ridx, rval = fn[strategy](c, val)
+    idx, vals = threshold_device(_t(c, gpu), strategy, val, w)
+    assert np.array_equal(_n(idx), ridx)
+    if ridx.size:
+        tol = 1e-13 if strategy == "hard_reassign" else 0.0
+        assert np.max(np.abs(_n(vals) - rval)) <= tol * np.max(np.abs(rval))

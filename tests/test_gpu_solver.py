@@ -352,3 +352,33 @@ def test_real_input_complex_sum_matches_oracle(gpu):
                            ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real",
                            precision=prec, cull_log2=0.0 if prec == "fp64" else 40.0)
         assert rel(_n(sr.forward(p0, dom, sensors, ts, wpt)), 2 * got.real) < tol
+
+
+@pytest.mark.parametrize("strategy,val", [("hard", 0.02), ("perc_max_abs", 0.05),
+                                          ("percentile", 97.0)])
+def test_forward_with_other_thresholds_matches_oracle(gpu, strategy, val):
+    N = (64, 64)
+    rng = np.random.default_rng(91)
+    dom = bx.Domain(N=N, dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    wpt = bx.MSWPT(bx.DyadicDecomposition(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror")
+    ow = OMSWPT(Tiling(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror")
+    p0 = rng.standard_normal(N)
+    ts = dom.generate_time_domain()[:60]
+    mask = np.zeros(N)
+    mask[0, :] = 1
+    sensors = bx.Sensor(dom, binary_mask=mask)
+    L = dom.grid_size
+    cpos = obeams.compute_coefficients_pos_only(ow, p0, 1500.0, L)
+    f = {"hard": otopn.threshold_hard, "perc_max_abs": otopn.threshold_perc_max_abs,
+         "percentile": otopn.threshold_percentile}[strategy]
+    idx, val_ = f(cpos, val)
+    assert idx.size > 0
+    p, al, x0, om, a0, mode = obeams.forward_parameters(ow, idx, L, N)
+    dup = lambda v: np.concatenate([v, v])
+    ref = ocpu.forward_sum(dup(x0), dup(p), dup(al), dup(a0 * val_), dup(om),
+                           np.concatenate([mode, -mode]), 1500.0, ts, sensors.positions)
+    s = bx.MSGBSolver(thr=val, thr_strat=strategy, batch_size=64, input_type="spatial",
+                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real", precision="fp64",
+                      cull_log2=0.0)
+    got = _n(s.forward(p0, dom, sensors, ts, wpt))
+    assert rel(ref, got) < 1e-12

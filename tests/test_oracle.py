@@ -445,3 +445,58 @@ def test_frozen_forward_transpose_dot_product():
     FTr = obeams.frozen_transpose(r, w, idx, L, N, 1500.0, ts, sens)
     lhs, rhs = np.sum(Fq * r), np.sum(q * FTr)
     assert abs(lhs - rhs) <= 1e-11 * np.sum(np.abs(Fq * r))
+
+
+# ---------------------------------------------------------------------------
+# the other thresholding strategies (forward_solver_utils.py:14-178)
+# ---------------------------------------------------------------------------
+def _thr_input(seed, n=5000):
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    c[rng.random(n) < 0.3] = 0.0           # masked zeros, as in the half frame
+    return c
+
+
+@pytest.mark.parametrize("q", [0.0, 10.0, 50.0, 73.3, 99.9, 100.0])
+def test_percentile_threshold_matches_numpy(q):
+    c = _thr_input(1)
+    a = otopn.magnitude(c)
+    # numpy's own interpolation pins the rule (they agree off exact ties)
+    ref = np.flatnonzero(a > np.percentile(a, q))
+    idx, vals = otopn.threshold_percentile(c, q)
+    assert np.array_equal(idx, ref)
+    assert np.array_equal(vals, c[ref])
+    if q == 100.0:
+        assert idx.size == 0
+
+
+def test_hard_and_max_fraction_thresholds():
+    c = _thr_input(2)
+    a = np.abs(c)
+    idx, _ = otopn.threshold_hard(c, 1.2)
+    assert np.array_equal(idx, [i for i in range(c.size) if a[i] > 1.2])
+    idx, _ = otopn.threshold_perc_max_abs(c, 0.4)
+    assert np.array_equal(idx, [i for i in range(c.size) if a[i] > 0.4 * a.max()])
+
+
+def test_hard_reassign_preserves_energy():
+    c = _thr_input(3)
+    idx, vals = otopn.threshold_hard_reassign(c, 0.3)
+    a = np.abs(c)
+    assert np.array_equal(idx, np.flatnonzero((a >= 0.3 * a.max()) & (a > 0)))
+    assert abs(np.sum(np.abs(vals) ** 2) - np.sum(a ** 2)) <= 1e-12 * np.sum(a ** 2)
+    assert np.allclose(vals / c[idx], np.sqrt(np.sum(a ** 2) / np.sum(a[idx] ** 2)))
+    idx0, vals0 = otopn.threshold_hard_reassign(np.zeros(10, complex), 0.5)
+    assert idx0.size == 0
+
+
+def test_bao_energy_weights_by_centre_norm():
+    t = Tiling(2, (32, 32), (4, 8), (1, 1))
+    w = MSWPT(t, 2, "rectangular_mirror")
+    rng = np.random.default_rng(4)
+    c = rng.standard_normal(w.total_coeffs) + 1j * rng.standard_normal(w.total_coeffs)
+    idx, vals = otopn.threshold_bao_energy(c, 40.0, w)
+    lvl, multi = obeams.decode_indices(w, np.arange(c.size))
+    cen = t.centres[multi[0] + t.boxes_cumsum[lvl]]
+    ref = [i for i in range(c.size) if abs(c[i]) * (cen[i, 0] ** 2 + cen[i, 1] ** 2) > 40.0]
+    assert np.array_equal(idx, ref) and np.array_equal(vals, c[ref])
