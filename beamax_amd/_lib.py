@@ -53,6 +53,11 @@ class Segs(C.Structure):
                 ("col", C.c_void_p)]
 
 
+class BatchSolve(C.Structure):
+    _fields_ = [("beams", FwdBeams), ("c", C.c_double), ("ts", C.c_void_p), ("nt", C.c_int64),
+                ("seg", Segs), ("out", C.c_void_p)]
+
+
 _P = C.c_void_p
 _I64 = C.c_int64
 _SZ = C.c_size_t
@@ -109,6 +114,10 @@ SIGNATURES = {
     "bh_coeff_scatter": (C.c_int, [_P, _P, _I64, C.c_double, _I64, _P, _P]),
     "bh_coeff_gather": (C.c_int, [_P, _I64, _P, _I64, _P, _P]),
     "bh_mswpt_adjoint": (C.c_int, [C.POINTER(Tiling), _P, _P, _P, C.c_int, _P, _P, _SZ, _P]),
+    "bh_forward_batch_plan_size": (_SZ, [C.c_int]),
+    "bh_forward_batch_plan": (C.c_int, [C.POINTER(BatchSolve), C.c_int, C.c_int, C.c_double, C.c_int,
+                                        _P, _SZ, C.POINTER(C.c_int64), _P]),
+    "bh_forward_batch_run": (C.c_int, [_P, C.c_int, _I64, C.c_int, C.c_int, C.c_int, _P, _P]),
     "bh_static_sum_workspace_size": (_SZ, [_I64, C.c_int]),
     "bh_static_sum": (C.c_int, [C.POINTER(StaticBeams), C.c_int, _P, _I64, C.c_double, C.c_int,
                                 _P, _P, _SZ, _P]),

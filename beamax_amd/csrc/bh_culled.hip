@@ -13,6 +13,7 @@
 // the result is deterministic.  Every evaluated term is computed exactly as
 // in the dense kernels.
 #include <type_traits>
+#include <vector>
 
 #include "bh_segcull.hpp"
 
@@ -50,8 +51,10 @@ __device__ inline void shift_phase(RecS<D>& R, double cyc) { R.ph0 += cyc; }
 template <int D>
 __device__ inline void shift_phase(RecD<D>& R, double cyc) { R.ang += bh::kTwoPi * cyc; }
 
+// One work unit: point-group block bx, time row by, beam range bz of A.split.
 template <int D, int SPT, bool F64, bool STATIC>
-__global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
+__device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, long long by,
+                                            long long bz) {
   using Rec = typename RecOf<D, F64>::type;
   constexpr int GPB = 4 * SPT;
   __shared__ Rec rec[kRecChunk];
@@ -60,8 +63,8 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
   __shared__ int gseg[GPB], gj0[GPB];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const long long gbase = (long long)blockIdx.x * GPB;
-  const long long t = STATIC ? 0 : (long long)blockIdx.y;
+  const long long gbase = bx * GPB;
+  const long long t = STATIC ? 0 : by;
   const double tv = STATIC ? 0.0 : A.ts[t];
   const int ng = (int)((A.G.ngroups - gbase) < GPB ? (A.G.ngroups - gbase) : GPB);
   // accumulators and group tables in LDS; each group's point is reloaded
@@ -76,8 +79,8 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
   unsigned long long npairs = 0;
   const long long total = STATIC ? A.count : A.total;
   const long long nchunk = (total + kRecChunk - 1) / kRecChunk;
-  const long long c_lo = nchunk * blockIdx.z / A.split;
-  const long long c_hi = nchunk * (blockIdx.z + 1) / A.split;
+  const long long c_lo = nchunk * bz / A.split;
+  const long long c_hi = nchunk * (bz + 1) / A.split;
   for (long long cb = c_lo * kRecChunk; cb < c_hi * kRecChunk && cb < total; cb += kRecChunk) {
     __syncthreads();   // previous chunk's records and masks fully consumed
     const long long bi = cb + threadIdx.x;
@@ -179,13 +182,41 @@ __global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
     if (col < 0) continue;
     const long long o = STATIC ? col : t * A.G.S + col;
     if (A.split > 1) {
-      A.part[(long long)blockIdx.z * A.rows * A.G.S + o] = accs[gl][lane];
+      A.part[bz * A.rows * A.G.S + o] = accs[gl][lane];
     } else {
       const long long oo = o * A.ostride + A.ooff;
       if constexpr (F64) ((double*)A.out)[oo] = A.scale * accs[gl][lane];
       else ((float*)A.out)[oo] = (float)(A.scale * accs[gl][lane]);
     }
   }
+}
+
+template <int D, int SPT, bool F64, bool STATIC>
+__global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
+  culled_body<D, SPT, F64, STATIC>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Many forward sums in one launch: solve s owns blocks [start[s], start[s+1])
+// of the 1-D grid, laid out as (time row, point-group block).
+struct BatchDesc {
+  CulledArgs a;
+  long long ngx;   // point-group blocks per time row
+};
+
+template <int D, int SPT, bool F64>
+__global__ __launch_bounds__(256) void k_culled_batch(const BatchDesc* __restrict__ d,
+                                                      const long long* __restrict__ start,
+                                                      int nsolve) {
+  const long long b = blockIdx.x;
+  int lo = 0, hi = nsolve - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const CulledArgs A = d[lo].a;
+  const long long local = b - start[lo];
+  culled_body<D, SPT, F64, false>(A, local % d[lo].ngx, local / d[lo].ngx, 0);
 }
 
 // ordered sum of the split partials (deterministic)
@@ -404,6 +435,111 @@ extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh
   else if (ndim == 2) { if (f64) BH_SC(2, true); else BH_SC(2, false); }
   else { if (f64) BH_SC(3, true); else BH_SC(3, false); }
 #undef BH_SC
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" size_t bh_forward_batch_plan_size(int nsolve) {
+  if (nsolve <= 0) return 0;
+  return ((size_t)nsolve * sizeof(BatchDesc) + 255) / 256 * 256 + (size_t)(nsolve + 1) * 8;
+}
+
+extern "C" int bh_forward_batch_plan(const bh_batch_solve* solves, int nsolve, int ndim,
+                                     double cull_log2, int gpb, void* plan, size_t plan_bytes,
+                                     int64_t* nblocks, bh_stream s) {
+  bh::clear_error();
+  if (!solves || nsolve <= 0 || !plan || !nblocks)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_batch_plan: null pointer or nsolve=%d", nsolve);
+  if (gpb != 4 && gpb != 8 && gpb != 16 && gpb != 32)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_batch_plan: gpb=%d (4, 8, 16 or 32)", gpb);
+  if (ndim < 1 || ndim > 3) BH_FAIL(BH_ERR_ARG, "bh_forward_batch_plan: ndim=%d", ndim);
+  const size_t need = bh_forward_batch_plan_size(nsolve);
+  if (plan_bytes < need)
+    BH_FAIL(BH_ERR_WORKSPACE, "bh_forward_batch_plan: plan %zu < %zu bytes", plan_bytes, need);
+  std::vector<BatchDesc> d((size_t)nsolve);
+  std::vector<long long> start((size_t)nsolve + 1);
+  long long nb = 0;
+  for (int i = 0; i < nsolve; ++i) {
+    const bh_batch_solve& q = solves[i];
+    if (int rc = check_segments(&q.seg, ndim)) return rc;
+    if (q.nt < 0 || q.nt > 65535 || !q.out || (q.nt > 0 && !q.ts))
+      BH_FAIL(BH_ERR_ARG, "bh_forward_batch_plan: solve %d: nt=%lld or null pointer", i,
+              (long long)q.nt);
+    if (q.beams.count > 0 && (!q.beams.x0 || !q.beams.p0 || !q.beams.omega ||
+                              !q.beams.alpha || !q.beams.a0))
+      BH_FAIL(BH_ERR_ARG, "bh_forward_batch_plan: solve %d: null beam array", i);
+    CulledArgs a{};
+    a.b = q.beams;
+    a.count = q.beams.count;
+    a.total = 2 * q.beams.count;
+    a.npos = q.beams.count;
+    a.mirror = 1;
+    a.phase = 0.0;
+    a.c = q.c;
+    a.ts = q.ts;
+    a.recs = nullptr;
+    a.G = to_segdev(&q.seg);
+    a.E = cull_log2;
+    a.scale = 1.0;
+    a.out = q.out;
+    a.pairs = nullptr;
+    a.part = nullptr;
+    a.split = 1;
+    a.rows = q.nt;
+    a.ostride = 1;
+    a.ooff = 0;
+    d[i].a = a;
+    d[i].ngx = (q.seg.ngroups + gpb - 1) / gpb;
+    start[i] = nb;
+    nb += d[i].ngx * q.nt;
+  }
+  start[nsolve] = nb;
+  if (nb > 0x7fffffffLL) BH_FAIL(BH_ERR_ARG, "bh_forward_batch_plan: %lld blocks", nb);
+  const size_t dbytes = ((size_t)nsolve * sizeof(BatchDesc) + 255) / 256 * 256;
+  hipStream_t st = (hipStream_t)s;
+  BH_HIP(hipMemcpyAsync(plan, d.data(), (size_t)nsolve * sizeof(BatchDesc),
+                        hipMemcpyHostToDevice, st));
+  BH_HIP(hipMemcpyAsync((char*)plan + dbytes, start.data(), (size_t)(nsolve + 1) * 8,
+                        hipMemcpyHostToDevice, st));
+  BH_HIP(hipStreamSynchronize(st));   // the host staging vectors end here
+  *nblocks = nb;
+  return BH_OK;
+}
+
+extern "C" int bh_forward_batch_run(const void* plan, int nsolve, int64_t nblocks, int ndim,
+                                    int precision, int gpb, unsigned long long* pairs,
+                                    bh_stream s) {
+  bh::clear_error();
+  if (!plan || nsolve <= 0) BH_FAIL(BH_ERR_ARG, "bh_forward_batch_run: null plan");
+  if (precision != BH_PRECISION_FP64 && precision != BH_PRECISION_FP32_SPLIT)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_batch_run: precision=%d", precision);
+  if (pairs) BH_FAIL(BH_ERR_UNSUPPORTED, "bh_forward_batch_run: pair counting is not wired");
+  if (nblocks <= 0) return BH_OK;
+  const size_t dbytes = ((size_t)nsolve * sizeof(BatchDesc) + 255) / 256 * 256;
+  const BatchDesc* d = (const BatchDesc*)plan;
+  const long long* start = (const long long*)((const char*)plan + dbytes);
+  hipStream_t st = (hipStream_t)s;
+  const bool f64 = precision == BH_PRECISION_FP64;
+  dim3 grid((unsigned)nblocks);
+#define BH_BATCH(DD, SS)                                                                       \
+  do {                                                                                        \
+    if (f64) hipLaunchKernelGGL((k_culled_batch<DD, SS, true>), grid, dim3(256), 0, st, d,     \
+                                start, nsolve);                                               \
+    else hipLaunchKernelGGL((k_culled_batch<DD, SS, false>), grid, dim3(256), 0, st, d, start, \
+                            nsolve);                                                          \
+  } while (0)
+#define BH_BATCH_D(DD)                                                                         \
+  do {                                                                                        \
+    if (gpb == 32) BH_BATCH(DD, 8);                                                           \
+    else if (gpb == 16) BH_BATCH(DD, 4);                                                      \
+    else if (gpb == 8) BH_BATCH(DD, 2);                                                       \
+    else BH_BATCH(DD, 1);                                                                     \
+  } while (0)
+  if (ndim == 1) BH_BATCH_D(1);
+  else if (ndim == 2) BH_BATCH_D(2);
+  else BH_BATCH_D(3);
+#undef BH_BATCH_D
+#undef BH_BATCH
   BH_LAUNCH_CHECK();
   return BH_OK;
 }

@@ -201,7 +201,7 @@ class MSWPT:
         return t.contiguous()
 
     def forward(self, data, input_type: str = "spatial", *, scale: float = 1.0,
-                half_mask: bool = False, out=None):
+                half_mask: bool = False, out=None, ws=None):
         """Flat complex128 coefficients (total_coeffs,) of `data` (transforms.py:696-733).
 
         ``scale`` and ``half_mask`` fold the ``compute_coefficients`` 'pos_only'
@@ -232,7 +232,8 @@ class MSWPT:
             kind = _lib.INPUT_REAL_SPATIAL
         if out is None:
             out = torch.empty(self.total_coeffs, dtype=torch.complex128, device=self.device)
-        ws = _lib.workspace(self._ws_bytes, self.device)
+        if ws is None:
+            ws = _lib.workspace(self._ws_bytes, self.device)
         _lib.call("bh_mswpt_forward", C.byref(self._tiling), _lib.ptr(self.sum_gsquare),
                   _lib.ptr(self.gfilts_packed), _lib.ptr(x), kind, float(scale), int(half_mask),
                   _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream_handle())

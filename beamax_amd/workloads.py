@@ -78,3 +78,52 @@ def packets_3d(N, dx, n_packets=32, seed=0, sigma_cells=(3.0, 10.0), k_frac=(0.1
         ph = 2 * np.pi * kmag[j] * (dirs[j, 0] * rx + dirs[j, 1] * ry + dirs[j, 2] * rz)
         p0 += amp[j] * env * np.cos(ph)
     return p0
+
+
+_BATCH_LEVELS = {128: (3, (4, 8, 16)), 192: (3, (4, 8, 16)), 256: (3, (4, 8, 16)),
+                 384: (3, (4, 8, 16)), 512: (4, (4, 8, 16, 32))}
+
+
+def batch_forward_problems(device, n_solves=64, seed=0, sizes=(128, 192, 256, 384, 512),
+                           nt_range=(400, 2000), beam_range=(1000, 65536)):
+    """The batching workload: per solve a square grid drawn from ``sizes``
+    (dx 1e-4, c 1500, cfl 0.3; 3 levels (4,8,16), 4 levels (4,8,16,32) at
+    512), rho 2 rectangular_mirror MSWPT, top_n drawn log-uniform in
+    ``beam_range`` (capped at the half frame), nt uniform in ``nt_range`` on
+    the CFL time step, a line sensor on one random edge, and p0 the real part
+    of the synthesis of two random unit coefficients (two frames, as the
+    128^2 README case), normalised to max |p0| = 1.  Draws from
+    ``numpy.random.default_rng(seed)`` in that order, per solve.  MSWPT plans
+    are shared between solves of the same grid."""
+    import torch
+    from .batch import ForwardProblem
+    from .decomposition import DyadicDecomposition
+    from .geometry import Domain
+    from .transforms import MSWPT
+    rng = np.random.default_rng(seed)
+    plans = {}
+    out = []
+    for _ in range(n_solves):
+        n = int(rng.choice(sizes))
+        thr = int(round(float(np.exp(rng.uniform(np.log(beam_range[0]), np.log(beam_range[1]))))))
+        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+        edge = int(rng.integers(0, 4))
+        if n not in plans:
+            lv, nb = _BATCH_LEVELS[n]
+            dom = Domain(N=(n, n), dx=(1e-4, 1e-4), c=1500.0, cfl=0.3, periodic=(False, False))
+            plans[n] = (dom, MSWPT(DyadicDecomposition(lv, (n, n), nb, (1, 1)), 2,
+                                   "rectangular_mirror", device=device))
+        dom, wpt = plans[n]
+        e = np.zeros(wpt.total_coeffs, dtype=complex)
+        e[rng.integers(wpt.coeffs_cumsum[1], wpt.total_coeffs, 2)] = 1.0
+        p0 = wpt.inverse(torch.as_tensor(e, device=device)).real.contiguous()
+        p0 = p0 / p0.abs().max()
+        dt = float(dom.generate_time_domain()[1])
+        ts = np.arange(nt) * dt
+        j = np.arange(n) * 1e-4
+        far = (n - 1) * 1e-4
+        line = {0: (np.zeros(n), j), 1: (np.full(n, far), j),
+                2: (j, np.zeros(n)), 3: (j, np.full(n, far))}[edge]
+        pts = np.stack(line, axis=1)
+        out.append(ForwardProblem(p0, dom, wpt, pts, ts, thr))
+    return out

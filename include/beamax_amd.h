@@ -38,6 +38,8 @@
  *                       (forward_solver_utils.py:755-786)
  *   bh_fourier_crop_time  prepare_data_domain_for_msgb Fourier crop
  *                       (examples/reconstruction/2d_time_reversal_and_adjoint.py:111-165)
+ *   bh_forward_batch_plan, bh_forward_batch_run: a batch of forward sums in
+ *                       one launch (no reference counterpart)
  *   bh_forward_sum_modes  the complex-input forward (msgb_solver.py:635-689)
  *                       and the complex beam sum (gb/core.py:105-168)
  *   bh_forward_sum_culled, bh_static_sum_culled: the same sums with a certified
@@ -286,6 +288,30 @@ int bh_forward_sum_modes(const bh_fwd_beams* b, int ndim, int64_t npos, int mirr
                          const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
                          int precision, int complex_out, void* out, unsigned long long* pairs,
                          void* ws, size_t ws_bytes, bh_stream s);
+/* Many independent forward sums in one launch (a batch of solves of
+ * different grids, beam counts, time grids and sensors).  Each solve: a
+ * mirrored beam table, its speed c, device time grid ts[nt], segment tables
+ * and output (nt, seg.S) (float in FP32-split, double in FP64).
+ * bh_forward_batch_plan writes the device plan (bh_forward_batch_plan_size
+ * bytes; it copies from the host and synchronises the stream, so call it
+ * outside graph capture); bh_forward_batch_run is one kernel launch over all
+ * solves (capturable), same terms and summation order per point as
+ * bh_forward_sum_culled without a beam split.  gpb = point groups per block
+ * (4, 8, 16, 32; the tile shape an autotuner picks).  No reference
+ * counterpart: the reference runs solves one by one. */
+typedef struct bh_batch_solve {
+  bh_fwd_beams beams;
+  double c;
+  const double* ts;
+  int64_t nt;
+  bh_segments seg;
+  void* out;
+} bh_batch_solve;
+size_t bh_forward_batch_plan_size(int nsolve);
+int bh_forward_batch_plan(const bh_batch_solve* solves, int nsolve, int ndim, double cull_log2,
+                          int gpb, void* plan, size_t plan_bytes, int64_t* nblocks, bh_stream s);
+int bh_forward_batch_run(const void* plan, int nsolve, int64_t nblocks, int ndim, int precision,
+                         int gpb, unsigned long long* pairs, bh_stream s);
 size_t bh_static_sum_culled_workspace_size(int64_t count, int ndim, int64_t P);
 int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
                          double scale, double cull_log2, int precision, void* out,
