@@ -53,6 +53,12 @@ class Segs(C.Structure):
                 ("col", C.c_void_p)]
 
 
+class Speed(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("method", C.c_int32), ("boundary", C.c_int32),
+                ("N", C.c_int64 * 3), ("spacing", C.c_double * 3), ("origin", C.c_double * 3),
+                ("values", C.c_void_p), ("lam", C.c_double)]
+
+
 class BatchSolve(C.Structure):
     _fields_ = [("beams", FwdBeams), ("c", C.c_double), ("ts", C.c_void_p), ("nt", C.c_int64),
                 ("seg", Segs), ("out", C.c_void_p)]
@@ -114,6 +120,11 @@ SIGNATURES = {
     "bh_coeff_scatter": (C.c_int, [_P, _P, _I64, C.c_double, _I64, _P, _P]),
     "bh_coeff_gather": (C.c_int, [_P, _I64, _P, _I64, _P, _P]),
     "bh_mswpt_adjoint": (C.c_int, [C.POINTER(Tiling), _P, _P, _P, C.c_int, _P, _P, _SZ, _P]),
+    "bh_speed_eval": (C.c_int, [C.POINTER(Speed), _P, _I64, _P, _P, _P, _P]),
+    "bh_ode_forward": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, C.POINTER(Speed), _P, _I64,
+                                 C.c_double, C.c_double, _I64, _P, _P, _P]),
+    "bh_forward_sum_traj": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, _P, _P, _I64,
+                                      C.POINTER(Segs), C.c_double, C.c_int, _P, _P, _P, _SZ, _P]),
     "bh_forward_batch_plan_size": (_SZ, [C.c_int]),
     "bh_forward_batch_plan": (C.c_int, [C.POINTER(BatchSolve), C.c_int, C.c_int, C.c_double, C.c_int,
                                         _P, _SZ, C.POINTER(C.c_int64), _P]),

@@ -29,6 +29,7 @@ struct CulledArgs {
   long long count, total, npos;
   int mirror;
   double phase;   // added to every beam's arg(A) (cycles)
+  const double* traj;   // non-null: beam states per (time row, beam) from bh_ode_forward
   double c;
   const double* ts;
   // static source
@@ -100,8 +101,20 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
           al[a] = mkc(A.b.alpha[2 * (j * D + a)], A.b.alpha[2 * (j * D + a) + 1]);
         }
         State<D> St;
-        hom_diag_state_fast<D>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]),
-                               neg ? -A.c : A.c, tv, St);
+        if (A.traj) {
+          // x, p, M (complex), A (complex) of beam bi at this time row
+          const double* y = A.traj + (t * A.total + bi) * (2 * D + 2 * D * D + 2);
+          for (int a = 0; a < D; ++a) {
+            St.x[a] = y[a];
+            St.p[a] = y[D + a];
+            for (int e = 0; e < D; ++e)
+              St.M[a][e] = mkc(y[2 * D + 2 * (a * D + e)], y[2 * D + 2 * (a * D + e) + 1]);
+          }
+          St.A = mkc(y[2 * D + 2 * D * D], y[2 * D + 2 * D * D + 1]);
+        } else {
+          hom_diag_state_fast<D>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]),
+                                 neg ? -A.c : A.c, tv, St);
+        }
         make_rec<D>(St, A.b.omega[j], R);
         if (A.phase != 0.0) shift_phase<D>(R, A.phase);
       }
@@ -317,7 +330,8 @@ namespace {
 int forward_culled(const char* name, const bh_fwd_beams* b, int ndim, long long npos, int mirror,
                    double c, const double* ts, int64_t nt, const bh_segments* seg,
                    double cull_log2, int precision, int complex_out, void* out,
-                   unsigned long long* pairs, void* ws, size_t ws_bytes, bh_stream s) {
+                   unsigned long long* pairs, void* ws, size_t ws_bytes, bh_stream s,
+                   const double* traj = nullptr) {
   bh::clear_error();
   if (int rc = check_segments(seg, ndim)) return rc;
   if (!b || !ts || !out) BH_FAIL(BH_ERR_ARG, "%s: null pointer", name);
@@ -345,6 +359,7 @@ int forward_culled(const char* name, const bh_fwd_beams* b, int ndim, long long 
   a.mirror = mirror ? 1 : 0;
   a.c = c;
   a.ts = ts;
+  a.traj = traj;
   a.recs = nullptr;
   a.G = to_segdev(seg);
   a.E = cull_log2;
@@ -374,6 +389,19 @@ extern "C" int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror
                                      bh_stream s) {
   return forward_culled("bh_forward_sum_culled", b, ndim, b ? (long long)b->count : 0, mirror, c,
                         ts, nt, seg, cull_log2, precision, 0, out, pairs, ws, ws_bytes, s);
+}
+
+extern "C" int bh_forward_sum_traj(const bh_fwd_beams* b, int ndim, int mirror,
+                                   const double* traj, const double* ts, int64_t nt,
+                                   const bh_segments* seg, double cull_log2, int precision,
+                                   void* out, unsigned long long* pairs, void* ws,
+                                   size_t ws_bytes, bh_stream s) {
+  if (!traj) {
+    bh::clear_error();
+    BH_FAIL(BH_ERR_ARG, "bh_forward_sum_traj: null trajectory table");
+  }
+  return forward_culled("bh_forward_sum_traj", b, ndim, b ? (long long)b->count : 0, mirror, 0.0,
+                        ts, nt, seg, cull_log2, precision, 0, out, pairs, ws, ws_bytes, s, traj);
 }
 
 extern "C" int bh_forward_sum_modes(const bh_fwd_beams* b, int ndim, int64_t npos, int mirror,

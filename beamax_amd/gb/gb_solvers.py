@@ -7,6 +7,9 @@ with shapes (b, Nt, d), (b, Nt, d), (b, Nt, d, d), (b, Nt, 1)
 Inside ``MSGBSolver`` the same closed form is fused into the summation
 kernel; these functions also serve as markers selecting that path.
 ``solve_hom_TR`` (gb_solvers.py:484-594) is fused into ``bh_adjoint_params``.
+``solve_ODE_base`` (gb_solvers.py:941-1016) integrates the beam ODEs in a
+heterogeneous (grid) medium on the GPU (``bh_ode_forward``); inside
+``MSGBSolver`` it selects the trajectory-fed summation.
 """
 
 from __future__ import annotations
@@ -59,12 +62,85 @@ def solve_hom_TR(xT, pT, mT, aT, mode, ts, c, lam=None, config=None):
         "solve_hom_TR runs fused inside MSGBSolver.adjoint (bh_adjoint_params)")
 
 
+def speed_of(c):
+    """The device speed model of a SolverFn ``c`` argument: a float, a
+    ``GridSpeed``, or a ``Domain.c_fn`` / ``Domain``."""
+    from ..geometry import Domain
+    from ..utils.interp import GridSpeed
+    if isinstance(c, GridSpeed):
+        return c
+    owner = getattr(c, "__self__", None)
+    if isinstance(c, Domain) or isinstance(owner, Domain):
+        dom = c if isinstance(c, Domain) else owner
+        if dom.is_homogeneous:
+            return GridSpeed(np.full((1,) * dom.ndim, float(dom.c)))
+        return dom.speed_model
+    if isinstance(c, (int, float, np.floating)):
+        return None
+    raise NotImplementedError("the sound speed must be a number, a grid (GridSpeed) or a "
+                              "Domain's c_fn to run on the GPU")
+
+
+def solve_ODE_base(x0, p0, M0, a0, mode, ts, c, lam=0.0, solver_config=None):
+    """Beam ODEs in a (heterogeneous) medium, adaptive Dormand-Prince 5(4)
+    (gb_solvers.py:870-1016), diagonal M0.  Returns (xt, pt, Mt, At) with
+    shapes (b, Nt, d), (b, Nt, d), (b, Nt, d, d), (b, Nt, 1)."""
+    import ctypes as C
+    import torch
+    from ..solvers.msgb_solver import ForwardBeams
+    dev = x0.device if isinstance(x0, torch.Tensor) else torch.device("cuda")
+    f64 = lambda v: torch.as_tensor(v, device=dev).to(torch.float64).contiguous()
+    c128 = lambda v: torch.as_tensor(v, device=dev).to(torch.complex128).contiguous()
+    x0, p0, ts = f64(x0), f64(p0), f64(ts).reshape(-1)
+    mode = f64(mode).reshape(-1)
+    M0 = c128(M0)
+    a0 = c128(a0).reshape(-1)
+    b, d = x0.shape
+    alpha = torch.diagonal(M0, dim1=1, dim2=2)
+    if bool((M0 - torch.diag_embed(alpha)).abs().max() > 0) if b else False:
+        raise NotImplementedError("solve_ODE_base on the GPU takes a diagonal M0")
+    sp = speed_of(c)
+    if sp is None:
+        from ..utils.interp import GridSpeed
+        sp = GridSpeed(np.full((1,) * d, float(c)))
+    cfg = solver_config if solver_config is not None else SolverConfig()
+    nt = ts.numel()
+    ns = 2 * d + 2 * d * d + 2
+    xt = torch.empty((b, nt, d), dtype=torch.float64, device=dev)
+    pt = torch.empty_like(xt)
+    Mt = torch.empty((b, nt, d, d), dtype=torch.complex128, device=dev)
+    At = torch.empty((b, nt, 1), dtype=torch.complex128, device=dev)
+    for sign in (1.0, -1.0):
+        sel = torch.nonzero(mode == sign).reshape(-1)
+        k = sel.numel()
+        if k == 0:
+            continue
+        beams = ForwardBeams(k, d, dev)
+        beams.x0.copy_(x0[sel]); beams.p0.copy_(p0[sel]); beams.alpha.copy_(alpha[sel])
+        beams.a0.copy_(a0[sel]); beams.omega.zero_()
+        traj = torch.empty((nt, k, ns), dtype=torch.float64, device=dev)
+        status = torch.empty(k, dtype=torch.int32, device=dev)
+        sps = sp.struct(dev, lam=float(lam or 0.0))
+        _lib.call("bh_ode_forward", C.byref(beams.struct()), d, 0 if sign > 0 else -1,
+                  C.byref(sps), _lib.ptr(ts), nt, float(cfg.rtol), float(cfg.atol),
+                  int(cfg.max_steps), _lib.ptr(traj), _lib.ptr(status), _lib.stream_handle())
+        if bool((status != 0).any()):
+            raise RuntimeError("beam ODE integration failed: more than max_steps steps or a "
+                               "non-finite state")
+        tr = traj.permute(1, 0, 2)                     # (k, nt, ns)
+        xt[sel] = tr[..., :d]
+        pt[sel] = tr[..., d:2 * d]
+        Mt[sel] = torch.view_as_complex(tr[..., 2 * d:2 * d + 2 * d * d].contiguous()
+                                        .reshape(k, nt, d * d, 2)).reshape(k, nt, d, d)
+        At[sel] = torch.view_as_complex(tr[..., 2 * d + 2 * d * d:].contiguous()
+                                        .reshape(k, nt, 1, 2))
+    return xt, pt, Mt, At
+
+
 @dataclass(frozen=True)
 class SolverConfig:
-    """Numerical ODE settings (gb_solvers.py:597-703); kept for API parity.
-
-    The closed-form propagators used on the GPU take no tolerances.
-    """
+    """Numerical ODE settings (gb_solvers.py:597-703): rtol, atol and
+    max_steps drive ``solve_ODE_base``; the closed forms take none."""
 
     rtol: float = 1e-7
     atol: float = 1e-9

@@ -1,9 +1,11 @@
 """Domain and Sensor (host geometry).
 
-API of ``beamax.geometry`` (reference ``beamax/geometry.py:15-616``).  Only the
-homogeneous medium (scalar ``c``) drives the HIP operator; a callable or
-grid-valued ``c`` is accepted by ``Domain`` for API compatibility and rejected
-by the solver.
+API of ``beamax.geometry`` (reference ``beamax/geometry.py:15-616``).  A
+scalar ``c`` is the homogeneous medium (closed-form propagators); a grid of
+shape N becomes the reference's default interpolant (n-linear, clamp, or wrap
+when every axis is periodic: geometry.py:176-221), as does a ``GridSpeed``
+passed directly; other callables are accepted for API compatibility and
+rejected by the solver.
 """
 
 from __future__ import annotations
@@ -11,6 +13,16 @@ from __future__ import annotations
 from typing import Optional, Tuple
 
 import numpy as np
+
+
+def _np_of(x):
+    try:
+        import torch
+        if isinstance(x, torch.Tensor):
+            return x.detach().cpu().numpy()
+    except ImportError:
+        pass
+    return x
 
 
 class Domain:
@@ -100,18 +112,40 @@ class Domain:
         return isinstance(self.c, float)
 
     @property
-    def sound_speed_array(self):
+    def speed_model(self):
+        """The ``GridSpeed`` the HIP kernels evaluate (None for a scalar c)."""
+        from .utils.interp import GridSpeed
+        if isinstance(self.c, float):
+            return None
+        if isinstance(self.c, GridSpeed):
+            return self.c
         if callable(self.c):
-            return np.asarray(self.c(self.grid), dtype=float)
+            raise NotImplementedError("a Python callable sound speed cannot run on the GPU; "
+                                      "pass a grid (array of shape N) or a GridSpeed")
+        if not hasattr(self, "_speed"):
+            self._speed = GridSpeed(np.asarray(self.c, dtype=float), spacing=self.dx,
+                                    origin=(0.0,) * self.ndim,
+                                    boundary="wrap" if all(self.periodic) else "clamp")
+        return self._speed
+
+    @property
+    def sound_speed_array(self):
+        from .utils.interp import GridSpeed
+        if isinstance(self.c, GridSpeed) and self.c.grid_values.shape == self.N \
+                and self.c.spacing == self.dx and not any(self.c.origin):
+            # the interpolant takes the (smoothed) grid values at the nodes
+            return np.asarray(self.c.grid_values, dtype=float)
+        if callable(self.c):
+            return np.asarray(_np_of(self.c(self.grid)), dtype=float)
         return np.broadcast_to(np.asarray(self.c, dtype=float), self.N)
 
     def c_fn(self, x):
         """Sound speed at positions x (..., d)."""
-        if callable(self.c):
-            return self.c(x)
         if isinstance(self.c, float):
             return np.full(np.asarray(x).shape[:-1], self.c)
-        raise NotImplementedError("grid-valued sound speed needs interpolation (heterogeneous media)")
+        if callable(self.c):
+            return self.c(x)
+        return self.speed_model(x)
 
     def compute_max_speed(self) -> float:
         return float(np.max(self.sound_speed_array))

@@ -280,9 +280,20 @@ class MSGBSolver:
                                       "use *_real here.")
         if self.sharding is not None:
             raise NotImplementedError("multi-device sharding is not part of this single-GPU build")
-        if self.ode_solver is not gb_solvers.solve_hom_diag:
-            raise NotImplementedError("only the closed-form solve_hom_diag propagator is fused "
-                                      "into the GPU summation (homogeneous media)")
+        if self.ode_solver not in (gb_solvers.solve_hom_diag, gb_solvers.solve_ODE_base):
+            raise NotImplementedError("the GPU forward runs solve_hom_diag (closed form, fused "
+                                      "into the summation) or solve_ODE_base (adaptive RK in a "
+                                      "grid medium)")
+
+    @staticmethod
+    def _c0(domain: Domain) -> float:
+        """The speed the closed forms use: c at the origin
+        (gb_solvers.py:356-422, c(zeros(d)))."""
+        if domain.is_homogeneous:
+            return float(domain.c)
+        import torch
+        sp = domain.speed_model
+        return float(sp.evaluate(np.zeros((1, domain.ndim)))[0].reshape(-1)[0].item())
 
     @staticmethod
     def _sensor_positions(sensors):
@@ -351,7 +362,7 @@ class MSGBSolver:
             ws = _lib.workspace(
                 _lib.load().bh_forward_sum_culled_workspace_size(tsd.numel(), seg.S), dev)
             _lib.call("bh_forward_sum_modes", C.byref(b), d, 0 if npos is None else int(npos),
-                      int(npos is None), float(domain.c), _lib.ptr(tsd), tsd.numel(), C.byref(ss),
+                      int(npos is None), self._c0(domain), _lib.ptr(tsd), tsd.numel(), C.byref(ss),
                       self.cull_log2, _lib.PRECISION[self.precision], int(not self.use_real),
                       _lib.ptr(out), _lib.ptr(self.pair_counter), _lib.ptr(ws), ws.numel(),
                       _lib.stream_handle())
@@ -368,7 +379,7 @@ class MSGBSolver:
                 ss = seg.struct()
                 ws = _lib.workspace(
                     _lib.load().bh_forward_sum_culled_workspace_size(tsd.numel(), seg.S), dev)
-                _lib.call("bh_forward_sum_culled", C.byref(b), d, 1, float(domain.c), _lib.ptr(tsd),
+                _lib.call("bh_forward_sum_culled", C.byref(b), d, 1, self._c0(domain), _lib.ptr(tsd),
                           tsd.numel(), C.byref(ss), self.cull_log2, _lib.PRECISION[self.precision],
                           _lib.ptr(out), _lib.ptr(self.pair_counter), _lib.ptr(ws), ws.numel(),
                           _lib.stream_handle())
@@ -382,7 +393,7 @@ class MSGBSolver:
             out = torch.empty((nt, S), dtype=dt, device=dev)
         per = _lib.int32_array([int(p) for p in domain.periodic])
         b = beams.struct()
-        _lib.call("bh_forward_sum", C.byref(b), d, 1, float(domain.c), _lib.ptr(tsd), nt,
+        _lib.call("bh_forward_sum", C.byref(b), d, 1, self._c0(domain), _lib.ptr(tsd), nt,
                   _lib.ptr(pos), S, _lib.dbl_array(domain.grid_size), per,
                   _lib.PRECISION[self.precision], _lib.ptr(out), _lib.stream_handle())
         return out
@@ -412,8 +423,10 @@ class MSGBSolver:
                   else int(np.count_nonzero(np.asarray(dpdt))))
             if nz == 0:
                 dpdt = None   # 0.5 * (a +- 1j * 0 / vg) == 0.5 * a exactly
-        if not domain.is_homogeneous:
-            raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
+        numerical = self.ode_solver is gb_solvers.solve_ODE_base
+        if (not domain.is_homogeneous or numerical) and dpdt is not None:
+            raise NotImplementedError("dpdt != 0 with the numerical ODE path (group velocity at "
+                                      "the packet positions) is not on the GPU path")
         self._check_supported(complex_sum_ok=True)
         out_shape = tuple(np.asarray(_np(sensors.positions) if isinstance(sensors, Sensor)
                                      else pos).shape[:-1])
@@ -423,9 +436,55 @@ class MSGBSolver:
             return (data.reshape((data.shape[0],) + out_shape),
                     self._params_tuple(beams, npos=npos))
         beams, _, _ = self.prepare_forward_beams(p0, domain, wpt, dpdt=dpdt)
-        data = self.sum_forward(beams, domain, pos, ts)
+        if numerical:
+            if is_complex or not self.use_real:
+                raise NotImplementedError("complex inputs or sums with solve_ODE_base are not on "
+                                          "the GPU path")
+            data = self.sum_forward_ode(beams, domain, pos, ts)
+        else:
+            data = self.sum_forward(beams, domain, pos, ts)
         data = data.reshape((data.shape[0],) + out_shape)
         return data, self._params_tuple(beams)
+
+    def sum_forward_ode(self, beams: ForwardBeams, domain: Domain, sensor_positions, ts):
+        """Mirrored real beam sum with the beam states integrated in the
+        medium (solve_ODE_base, gb_solvers.py:941-1016): bh_ode_forward
+        writes every beam's state at every time, bh_forward_sum_traj sums them
+        with the same envelope cut as the closed-form path."""
+        import torch
+        dev = beams.x0.device
+        d = domain.ndim
+        sp = gb_solvers.speed_of(domain.c_fn) if not domain.is_homogeneous else None
+        if sp is None:
+            from ..utils.interp import GridSpeed
+            sp = GridSpeed(np.full((1,) * d, float(domain.c)))
+        tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
+        nt = tsd.numel()
+        total = 2 * beams.count
+        ns = 2 * d + 2 * d * d + 2
+        if nt * total * ns * 8 > 96 * 2 ** 30:
+            raise MemoryError(f"the trajectory table ({nt} x {total} beams) exceeds 96 GiB")
+        traj = torch.empty((nt, total, ns), dtype=torch.float64, device=dev)
+        status = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        cfg = self.ode_config
+        sps = sp.struct(dev, lam=domain.lam)
+        b = beams.struct()
+        _lib.call("bh_ode_forward", C.byref(b), d, 1, C.byref(sps), _lib.ptr(tsd), nt,
+                  float(cfg.rtol), float(cfg.atol), int(cfg.max_steps), _lib.ptr(traj),
+                  _lib.ptr(status), _lib.stream_handle())
+        if total and bool((status[:total] != 0).any()):
+            raise RuntimeError("beam ODE integration failed: more than max_steps steps or a "
+                               "non-finite state")
+        pts = np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d)
+        seg = self._segments(pts, domain.dx, dev)
+        dt = torch.float64 if self.precision == "fp64" else torch.float32
+        out = torch.empty((nt, seg.S), dtype=dt, device=dev)
+        ws = _lib.workspace(_lib.load().bh_forward_sum_culled_workspace_size(nt, seg.S), dev)
+        ss = seg.struct()
+        _lib.call("bh_forward_sum_traj", C.byref(b), d, 1, _lib.ptr(traj), _lib.ptr(tsd), nt,
+                  C.byref(ss), self.cull_log2, _lib.PRECISION[self.precision], _lib.ptr(out),
+                  _lib.ptr(self.pair_counter), _lib.ptr(ws), ws.numel(), _lib.stream_handle())
+        return out
 
     def prepare_forward_beams_complex(self, p0, domain: Domain, wpt: MSWPT, dpdt=None):
         """Complex-input beams (msgb_solver.py:635-689): cpos and cneg of

@@ -38,6 +38,9 @@
  *                       (forward_solver_utils.py:755-786)
  *   bh_fourier_crop_time  prepare_data_domain_for_msgb Fourier crop
  *                       (examples/reconstruction/2d_time_reversal_and_adjoint.py:111-165)
+ *   bh_speed_eval, bh_ode_forward, bh_forward_sum_traj: heterogeneous media
+ *                       (utils/interp.py:158-270, gb_solvers.py:787-1016 with
+ *                       compute_gaussian_beam_real on the saved states)
  *   bh_forward_batch_plan, bh_forward_batch_run: a batch of forward sums in
  *                       one launch (no reference counterpart)
  *   bh_forward_sum_modes  the complex-input forward (msgb_solver.py:635-689)
@@ -316,6 +319,40 @@ size_t bh_static_sum_culled_workspace_size(int64_t count, int ndim, int64_t P);
 int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
                          double scale, double cull_log2, int precision, void* out,
                          unsigned long long* pairs, void* ws, size_t ws_bytes, bh_stream s);
+
+/* Grid-valued sound speed (utils/interp.py:158-270 make_c_function_from_grid):
+ * method 0 n-linear on grid values, 1 cubic B-spline on prefiltered
+ * coefficients; boundary 0 nearest (clamp), 1 wrap, 2 reflect, 3 mirror;
+ * values: device array of shape N (C order); lam: absorption coefficient of
+ * the amplitude equation (Domain.lam). */
+typedef struct {
+  int32_t ndim, method, boundary;
+  int64_t N[3];
+  double spacing[3], origin[3];
+  const double* values;
+  double lam;
+} bh_speed;
+/* c, grad c (n x ndim) and Hessian (n x ndim x ndim) at points x (n x ndim). */
+int bh_speed_eval(const bh_speed* sp, const double* x, int64_t n, double* c, double* grad,
+                  double* hess, bh_stream s);
+/* Beam ODEs in the medium (coupled_rhs_absorption + solve_ODE_base,
+ * gb_solvers.py:787-1016): each beam of b (and its mode -1 copy when mirror
+ * != 0, appended after the count originals) from M0 = diag(alpha), A0 = a0 at
+ * ts[0] (mirror < 0: every beam with mode -1 instead), written at every ts[k]
+ * to traj[(k * total + beam) * NS], NS = 2d +
+ * 2d^2 + 2 doubles: x, p, M (complex, row-major), A (complex).  Adaptive
+ * Dormand-Prince 5(4) with rtol/atol; status[beam] = 0, 1 (more than
+ * max_steps steps), 2 (non-finite state). */
+int bh_ode_forward(const bh_fwd_beams* b, int ndim, int mirror, const bh_speed* sp,
+                   const double* ts, int64_t nt, double rtol, double atol, int64_t max_steps,
+                   double* traj, int* status, bh_stream s);
+/* bh_forward_sum_culled with the beam states read from a bh_ode_forward
+ * trajectory table (beams [0, total) with the table's modes) instead of the
+ * homogeneous closed form; omega from b (beam j and its mirror share it). */
+int bh_forward_sum_traj(const bh_fwd_beams* b, int ndim, int mirror, const double* traj,
+                        const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
+                        int precision, void* out, unsigned long long* pairs, void* ws,
+                        size_t ws_bytes, bh_stream s);
 
 /* Exact transpose of the frozen-index forward beam sum.  For the beam table b
  * with unit coefficients (a0 without the MSWPT coefficient) and a residual

@@ -500,3 +500,109 @@ def test_bao_energy_weights_by_centre_norm():
     cen = t.centres[multi[0] + t.boxes_cumsum[lvl]]
     ref = [i for i in range(c.size) if abs(c[i]) * (cen[i, 0] ** 2 + cen[i, 1] ** 2) > 40.0]
     assert np.array_equal(idx, ref) and np.array_equal(vals, c[ref])
+
+
+# ---------------------------------------------------------------------------
+# heterogeneous media: grid interpolant and beam ODEs (utils/interp.py,
+# gb_solvers.py:787-1016; tests/test_gb.py:669-839)
+# ---------------------------------------------------------------------------
+from oracle import medium as omed  # noqa: E402
+
+
+@pytest.mark.parametrize("method", ["linear", "bspline3"])
+def test_interp_reproduces_nodes_and_polynomials(method):
+    rng = np.random.default_rng(41)
+    n = (24, 20)
+    vals = 1500 + 50 * rng.standard_normal(n)
+    sp = omed.Speed(vals, (1e-3, 2e-3), method=method, boundary="nearest")
+    gi, gj = np.meshgrid(np.arange(4, 18), np.arange(4, 15), indexing="ij")
+    nodes = np.stack([gi.ravel() * 1e-3, gj.ravel() * 2e-3], 1)
+    c, _, _ = sp(nodes)
+    assert np.max(np.abs(c - vals[gi.ravel(), gj.ravel()])) < 1e-9 * 1500
+    # affine (linear) / quadratic (bspline3) fields are reproduced away from the edges
+    X, Y = np.meshgrid(np.arange(n[0]) * 1e-3, np.arange(n[1]) * 2e-3, indexing="ij")
+    if method == "linear":
+        field = 1400 + 3e4 * X - 2e4 * Y
+    else:
+        field = 1400 + 3e4 * X - 2e4 * Y + 1e6 * X * Y + 5e6 * X ** 2
+    sp = omed.Speed(field, (1e-3, 2e-3), method=method, boundary="nearest")
+    q = np.stack([rng.uniform(8e-3, 15e-3, 50), rng.uniform(16e-3, 24e-3, 50)], 1)
+    c, g, H = sp(q)
+    xq, yq = q[:, 0], q[:, 1]
+    if method == "linear":
+        assert np.allclose(c, 1400 + 3e4 * xq - 2e4 * yq, rtol=1e-12)
+        assert np.allclose(g, [3e4, -2e4], rtol=1e-9)
+    else:
+        assert np.allclose(c, 1400 + 3e4 * xq - 2e4 * yq + 1e6 * xq * yq + 5e6 * xq ** 2, rtol=1e-6)
+        assert np.allclose(g[:, 0], 3e4 + 1e6 * yq + 1e7 * xq, rtol=1e-5)
+        # the prefilter's edge handling leaks ~1e-4 into second derivatives here
+        assert np.allclose(H[:, 0, 1], 1e6, rtol=1e-3) and np.allclose(H[:, 0, 0], 1e7, rtol=1e-3)
+
+
+@pytest.mark.parametrize("method", ["linear", "bspline3"])
+@pytest.mark.parametrize("boundary", ["nearest", "wrap", "reflect", "mirror"])
+def test_interp_derivatives_match_finite_differences(method, boundary):
+    rng = np.random.default_rng(42)
+    vals = 1500 + 40 * rng.standard_normal((9, 7, 6))
+    sp = omed.Speed(vals, (1.0, 1.5, 2.0), method=method, boundary=boundary)
+    x = rng.uniform(-2.0, 12.0, (40, 3))
+    # keep away from cell faces where the linear derivatives jump
+    x = np.floor(x / np.array([1.0, 1.5, 2.0])) * np.array([1.0, 1.5, 2.0]) + \
+        np.array([1.0, 1.5, 2.0]) * rng.uniform(0.2, 0.8, (40, 3))
+    c, g, H = sp(x)
+    h = 1e-5
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = h
+        cp, gp_, _ = sp(x + e)
+        cm, gm, _ = sp(x - e)
+        assert np.allclose(g[:, a], (cp - cm) / (2 * h), rtol=1e-6, atol=1e-6)
+        assert np.allclose(H[:, :, a], (gp_ - gm) / (2 * h), rtol=1e-5, atol=1e-3)
+
+
+def test_ode_matches_1d_linear_medium_closed_form():
+    """tests/test_gb.py:669-756: c = a + b x gives p = p0 e^{-bt},
+    x = (x0 + a/b) e^{bt} - a/b, M = M0 e^{-2bt}, A = A0 e^{bt/2}."""
+    a_lin, b_lin = 1.0, 1.0
+    grid = np.arange(0, 301) * 0.02            # linear interpolation of an affine map: exact
+    sp = omed.Speed(a_lin + b_lin * grid, (0.02,), method="linear")
+    ts = np.linspace(0.0, 1.0, 50)
+    xt, pt, Mt, At = omed.integrate(np.array([0.1]), np.array([1.2]), np.array([[0.2j]]),
+                                    1.0 + 0j, 1.0, ts, sp)
+    e = np.exp(b_lin * ts)
+    assert np.allclose(pt[:, 0], 1.2 / e, rtol=1e-9)
+    assert np.allclose(xt[:, 0], (0.1 + a_lin / b_lin) * e - a_lin / b_lin, rtol=1e-9)
+    assert np.allclose(Mt[:, 0, 0], 0.2j / e ** 2, rtol=1e-9)
+    assert np.allclose(At, np.exp(b_lin * ts / 2), rtol=1e-9)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_ode_constant_speed_matches_closed_form(d):
+    rng = np.random.default_rng(43 + d)
+    sp = omed.Speed(np.full((3,) * d, 1500.0), (1.0,) * d)
+    xi = rng.standard_normal(d) * 2000
+    om = np.linalg.norm(xi)
+    p = 2 * np.pi * xi / om
+    alpha = 1j * (rng.random(d) + 0.3) * 2 * (np.pi * 500) ** 2 / om
+    ts = np.linspace(0, 3e-6, 9)
+    for mode in (1.0, -1.0):
+        xt, pt, Mt, At = omed.integrate(np.zeros(d), p, np.diag(alpha), 0.7 - 0.2j, mode, ts, sp)
+        rx, rp, rM, rA = obeams.solve_hom_diag(np.zeros((1, d)), p[None], alpha[None],
+                                               np.array([0.7 - 0.2j]), np.array([mode]), ts, 1500.0)
+        assert np.allclose(xt, rx[0], rtol=1e-9, atol=1e-15)
+        assert np.allclose(Mt, rM[0], rtol=1e-8)
+        assert np.allclose(At, rA[0], rtol=1e-8)
+
+
+def test_ode_conserves_hamiltonian_in_a_heterogeneous_medium():
+    rng = np.random.default_rng(44)
+    X, Y = np.meshgrid(np.arange(40), np.arange(40), indexing="ij")
+    field = 1500 + 100 * np.sin(X / 7.0) * np.cos(Y / 9.0)
+    sp = omed.Speed(field, (1e-4, 1e-4), method="bspline3")
+    p = 2 * np.pi * np.array([0.6, 0.8])
+    ts = np.linspace(0, 1.5e-6, 30)
+    xt, pt, Mt, At = omed.integrate(np.array([1.5e-3, 2e-3]), p, np.diag([2e3j, 3e3j]), 1.0 + 0j,
+                                    1.0, ts, sp)
+    H = sp(xt)[0] * np.linalg.norm(pt, axis=1)
+    assert np.max(np.abs(H - H[0])) <= 1e-8 * abs(H[0])
+    assert np.ptp(xt[:, 0]) > 0 and np.all(np.isfinite(Mt))
