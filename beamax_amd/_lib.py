@@ -123,6 +123,11 @@ SIGNATURES = {
     "bh_speed_eval": (C.c_int, [C.POINTER(Speed), _P, _I64, _P, _P, _P, _P]),
     "bh_ode_forward": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, C.POINTER(Speed), _P, _I64,
                                  C.c_double, C.c_double, _I64, _P, _P, _P]),
+    "bh_boundary_terminal": (C.c_int, [C.POINTER(Tiling), C.POINTER(C.c_double), _P, _P, _I64,
+                                       C.c_int, C.c_double, C.c_double, C.POINTER(Speed),
+                                       C.c_double, C.c_int, C.c_int, StaticBeams, _P, _P, _P]),
+    "bh_ode_static": (C.c_int, [C.POINTER(StaticBeams), C.c_int, C.POINTER(Speed), _P, _P, _P,
+                                C.c_double, C.c_double, C.c_double, C.c_double, _I64, _P, _P]),
     "bh_forward_sum_traj": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, _P, _P, _I64,
                                       C.POINTER(Segs), C.c_double, C.c_int, _P, _P, _P, _SZ, _P]),
     "bh_forward_batch_plan_size": (_SZ, [C.c_int]),

@@ -15,6 +15,7 @@
 // and everything after are FP32; partial sums over 64 beams are FP32 and are
 // accumulated across chunks in FP64.
 #include "bh_beamcore.hpp"
+#include "bh_medium.hpp"
 
 namespace {
 
@@ -290,6 +291,12 @@ struct AdjArgs {
   int swap;
   int time_reversal;   // 1: compute_TR_parameters only (no B^-1, no exclusion)
   bh_static_beams out;
+  // heterogeneous medium: c = c(x_T) from the speed model, and the beams are
+  // left at their terminal time (t_start) for the ODE integrator
+  int have_speed;
+  SpeedDev S;
+  double* t_start;   // per beam (terminal output only)
+  double* mode;      // per beam signum (terminal output only)
 };
 
 template <int D>
@@ -311,6 +318,10 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
     A.out.A[2 * i] = 0.0;
     A.out.A[2 * i + 1] = 0.0;
     A.out.omega[i] = 1.0;
+    if (A.t_start) {
+      A.t_start[i] = 0.0;
+      A.mode[i] = 1.0;
+    }
     return;
   }
   const Decoded q = decode_flat(T, flat);
@@ -327,13 +338,37 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
   const double signum = -sgn_tau;
   const double om = fmax(fabs(tau), 1e-6);
   const double t_start = (double)q.k[0] / Ll[0];
-  const double c = A.c;
-  // positions and unit momentum on the detector plane
-  double xT[D], pu[D];
+  int perm[D];
+  for (int a = 0; a < D; ++a) perm[a] = a;
+  if (A.swap && D >= 2) { perm[0] = 1; perm[1] = 0; }
+  // detector position (tangential coordinates from the packet index)
+  double xT[D];
+  {
+    int tj = 0;
+    for (int a = 0; a < D; ++a) {
+      if (D == 1 || a == A.normal_axis) {
+        xT[a] = A.normal_value;
+      } else {
+        xT[a] = (double)q.k[tj + 1] / Ll[tj + 1];
+        ++tj;
+      }
+    }
+  }
+  // local speed at x_T (tr_solver_utils.py:336-343, adjoint_solver_utils.py:185)
+  double c = A.c;
+  double gk[D];   // grad c at x_T in the kernel frame (zero in a homogeneous medium)
+  for (int a = 0; a < D; ++a) gk[a] = 0.0;
+  if (A.have_speed) {
+    double xp[D], g[D], H[D * D];
+    for (int a = 0; a < D; ++a) xp[a] = xT[perm[a]];   // physical frame
+    speed_eval<D>(A.S, xp, c, g, H);
+    for (int a = 0; a < D; ++a) gk[a] = g[perm[a]];    // perm is an involution
+  }
+  // unit momentum on the detector plane
+  double pu[D];
   double ptan[D > 1 ? D - 1 : 1];
   double pn;
   if (D == 1) {
-    xT[0] = A.normal_value;
     pn = 1.0 / c;   // |xi_tau| / c with xi_tau = +-1 (tr_solver_utils.py:335-341)
     pu[0] = A.inward * pn;
   } else {
@@ -347,10 +382,8 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
     int tj = 0;
     for (int a = 0; a < D; ++a) {
       if (a == A.normal_axis) {
-        xT[a] = A.normal_value;
         pu[a] = A.inward * pn;
       } else {
-        xT[a] = (double)q.k[tj + 1] / Ll[tj + 1];
         pu[a] = ptan[tj];
         ++tj;
       }
@@ -363,21 +396,27 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
     const double ps = bh::kPi * sig[a];
     alpha[a] = mkc(0.0, 2.0 * ps * ps / om);
   }
-  // mT_inverse (tr_solver_utils.py:116-170), pdot = 0, axis 0 as the normal
+  // mT_inverse (tr_solver_utils.py:116-170) with xdot = H_p, pdot = -H_x at
+  // x_T, axis 0 as the normal; pdot = 0 in a homogeneous medium
   double np2 = 0.0;
   for (int a = 0; a < D; ++a) np2 += pT[a] * pT[a];
   double np = sqrt(np2);
-  double xd[D];
-  for (int a = 0; a < D; ++a) xd[a] = signum * c * pT[a] / np;
+  double xd[D], pd[D];
+  for (int a = 0; a < D; ++a) {
+    xd[a] = signum * c * pT[a] / np;
+    pd[a] = -(signum * gk[a] * np);
+  }
   cd MT[D][D];
   for (int a = 0; a < D; ++a)
     for (int b = 0; b < D; ++b) MT[a][b] = mkc(0.0);
   {
-    cd num = alpha[0];
+    double pdx = pd[0] * xd[0];
+    for (int j = 1; j < D; ++j) pdx -= pd[j] * xd[j];
+    cd num = alpha[0] + mkc(pdx);
     for (int j = 1; j < D; ++j) num = num + (alpha[j] * xd[j]) * xd[j];
     MT[0][0] = num / (xd[0] * xd[0]);
     for (int j = 1; j < D; ++j) {
-      MT[0][j] = (-(alpha[j] * xd[j])) / xd[0];
+      MT[0][j] = (mkc(pd[j]) - alpha[j] * xd[j]) / xd[0];
       MT[j][0] = MT[0][j];
       MT[j][j] = alpha[j];
     }
@@ -409,6 +448,23 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
   aT = aT * binv;
   const double2 v = A.vals[i];
   aT = aT * mkc(v.x, v.y);
+  if (A.t_start) {
+    // terminal state in the physical frame; the medium ODE takes it to t = 0
+    for (int a = 0; a < D; ++a) {
+      A.out.x[i * D + a] = xT[perm[a]];
+      A.out.p[i * D + a] = pT[perm[a]];
+      for (int b = 0; b < D; ++b) {
+        A.out.M[2 * ((i * D + a) * D + b)] = MT[perm[a]][perm[b]].re;
+        A.out.M[2 * ((i * D + a) * D + b) + 1] = MT[perm[a]][perm[b]].im;
+      }
+    }
+    A.out.A[2 * i] = aT.re;
+    A.out.A[2 * i + 1] = aT.im;
+    A.out.omega[i] = om;
+    A.t_start[i] = t_start;
+    A.mode[i] = signum;
+    return;
+  }
   // propagate t_start -> 0
   np2 = 0.0;
   for (int a = 0; a < D; ++a) np2 += pT[a] * pT[a];
@@ -437,9 +493,6 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
       M0[a][b] = s;
     }
   const cd A0 = aT / csqrt_(det);
-  int perm[D];
-  for (int a = 0; a < D; ++a) perm[a] = a;
-  if (A.swap && D >= 2) { perm[0] = 1; perm[1] = 0; }
   for (int a = 0; a < D; ++a) {
     A.out.x[i * D + a] = x[perm[a]];
     A.out.p[i * D + a] = pT[perm[a]];
@@ -639,7 +692,8 @@ extern "C" int bh_solve_hom_diag(int64_t b, int ndim, const double* x0, const do
 static int boundary_params(const bh_tiling* t, const double* data_grid_size, const int64_t* idx,
                            const double* vals, int64_t k, int normal_axis, double normal_value,
                            double inward_sign, double c, double guard, int swap_axes, int tr,
-                           bh_static_beams out, bh_stream s);
+                           bh_static_beams out, bh_stream s, const bh_speed* sp = nullptr,
+                           double* t_start = nullptr, double* mode = nullptr);
 
 extern "C" int bh_adjoint_params(const bh_tiling* t, const double* data_grid_size,
                                  const int64_t* idx, const double* vals, int64_t k,
@@ -658,10 +712,25 @@ extern "C" int bh_tr_params(const bh_tiling* t, const double* data_grid_size, co
                          c, 0.0, swap_axes, 1, out, s);
 }
 
+extern "C" int bh_boundary_terminal(const bh_tiling* t, const double* data_grid_size,
+                                    const int64_t* idx, const double* vals, int64_t k,
+                                    int normal_axis, double normal_value, double inward_sign,
+                                    const bh_speed* sp, double guard, int time_reversal,
+                                    int swap_axes, bh_static_beams out, double* t_start,
+                                    double* mode, bh_stream s) {
+  if (!sp || !t_start || !mode) {
+    bh::clear_error();
+    BH_FAIL(BH_ERR_ARG, "bh_boundary_terminal: null speed model or output");
+  }
+  return boundary_params(t, data_grid_size, idx, vals, k, normal_axis, normal_value, inward_sign,
+                         1.0, guard, swap_axes, time_reversal ? 1 : 0, out, s, sp, t_start, mode);
+}
+
 static int boundary_params(const bh_tiling* t, const double* data_grid_size, const int64_t* idx,
                            const double* vals, int64_t k, int normal_axis, double normal_value,
                            double inward_sign, double c, double guard, int swap_axes, int tr,
-                           bh_static_beams out, bh_stream s) {
+                           bh_static_beams out, bh_stream s, const bh_speed* sp, double* t_start,
+                           double* mode) {
   bh::clear_error();
   if (int rc = bh::check_tiling(t)) return rc;
   if (k < 0) BH_FAIL(BH_ERR_ARG, "bh_adjoint_params: k=%lld", (long long)k);
@@ -686,6 +755,25 @@ static int boundary_params(const bh_tiling* t, const double* data_grid_size, con
   a.swap = swap_axes;
   a.time_reversal = tr;
   a.out = out;
+  a.have_speed = sp ? 1 : 0;
+  a.S = SpeedDev{};
+  if (sp) {
+    if (sp->ndim != t->ndim || !sp->values)
+      BH_FAIL(BH_ERR_ARG, "bh_boundary_terminal: speed model ndim=%d for %d-D data", sp->ndim,
+              t->ndim);
+    a.S.ndim = sp->ndim;
+    a.S.method = sp->method;
+    a.S.boundary = sp->boundary;
+    for (int i = 0; i < 3; ++i) {
+      a.S.N[i] = i < sp->ndim ? sp->N[i] : 1;
+      a.S.spacing[i] = i < sp->ndim ? sp->spacing[i] : 1.0;
+      a.S.origin[i] = i < sp->ndim ? sp->origin[i] : 0.0;
+    }
+    a.S.values = sp->values;
+    a.S.lam = sp->lam;
+  }
+  a.t_start = t_start;
+  a.mode = mode;
   const unsigned g = bh::grid1d(k, 128);
   hipStream_t st = (hipStream_t)s;
   if (t->ndim == 1) hipLaunchKernelGGL(k_adjoint_params<1>, dim3(g), dim3(128), 0, st, a);

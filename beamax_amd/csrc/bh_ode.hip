@@ -122,6 +122,73 @@ __device__ inline double err_norm(const double* y0, const double* y1, const doub
   return sqrt(s / n);
 }
 
+// Advance y (with k1 = f(y) on entry and exit) over s in [s, sout], s < sout,
+// for dy/ds = dir * f(y): dir = +1 integrates forward in time, -1 backward
+// (s = t_start - t).  h is the proposed step, carried between calls.
+// Returns 0, 1 (step limit) or 2 (non-finite error estimate).
+template <int D>
+__device__ inline int dopri_advance(const SpeedDev& S, double mode, double dir, double* y,
+                                    double* k1, double& s, double sout, double& h,
+                                    long long& steps, long long max_steps, double rtol,
+                                    double atol) {
+  constexpr int NS = Ode<D>::NS;
+  double k2[NS], k3[NS], k4[NS], k5[NS], k6[NS], k7[NS], yt[NS], y5[NS], er[NS];
+  auto rhs = [&](const double* yin, double* f) {
+    ode_rhs<D>(S, mode, yin, f);
+    for (int q = 0; q < NS; ++q) f[q] *= dir;
+  };
+  while (s < sout) {
+    if (++steps > max_steps) return 1;
+    const bool last = s + h >= sout;
+    const double hs = last ? sout - s : h;
+    // Dormand-Prince 5(4) stages
+    for (int q = 0; q < NS; ++q) yt[q] = y[q] + hs * (1.0 / 5.0) * k1[q];
+    rhs(yt, k2);
+    for (int q = 0; q < NS; ++q) yt[q] = y[q] + hs * ((3.0 / 40.0) * k1[q] + (9.0 / 40.0) * k2[q]);
+    rhs(yt, k3);
+    for (int q = 0; q < NS; ++q)
+      yt[q] = y[q] + hs * ((44.0 / 45.0) * k1[q] - (56.0 / 15.0) * k2[q] + (32.0 / 9.0) * k3[q]);
+    rhs(yt, k4);
+    for (int q = 0; q < NS; ++q)
+      yt[q] = y[q] + hs * ((19372.0 / 6561.0) * k1[q] - (25360.0 / 2187.0) * k2[q] +
+                           (64448.0 / 6561.0) * k3[q] - (212.0 / 729.0) * k4[q]);
+    rhs(yt, k5);
+    for (int q = 0; q < NS; ++q)
+      yt[q] = y[q] + hs * ((9017.0 / 3168.0) * k1[q] - (355.0 / 33.0) * k2[q] +
+                           (46732.0 / 5247.0) * k3[q] + (49.0 / 176.0) * k4[q] -
+                           (5103.0 / 18656.0) * k5[q]);
+    rhs(yt, k6);
+    for (int q = 0; q < NS; ++q)
+      y5[q] = y[q] + hs * ((35.0 / 384.0) * k1[q] + (500.0 / 1113.0) * k3[q] +
+                           (125.0 / 192.0) * k4[q] - (2187.0 / 6784.0) * k5[q] +
+                           (11.0 / 84.0) * k6[q]);
+    rhs(y5, k7);
+    // y5 - y4 with the embedded 4th-order weights
+    for (int q = 0; q < NS; ++q)
+      er[q] = hs * ((35.0 / 384.0 - 5179.0 / 57600.0) * k1[q] +
+                    (500.0 / 1113.0 - 7571.0 / 16695.0) * k3[q] +
+                    (125.0 / 192.0 - 393.0 / 640.0) * k4[q] +
+                    (-2187.0 / 6784.0 + 92097.0 / 339200.0) * k5[q] +
+                    (11.0 / 84.0 - 187.0 / 2100.0) * k6[q] - (1.0 / 40.0) * k7[q]);
+    const double en = err_norm<D>(y, y5, er, rtol, atol);
+    if (!(en == en)) return 2;
+    double fac = en > 0.0 ? 0.9 * pow(en, -0.2) : 10.0;
+    fac = fmin(10.0, fmax(0.2, fac));
+    if (en <= 1.0) {
+      s = last ? sout : s + hs;
+      for (int q = 0; q < NS; ++q) {
+        y[q] = y5[q];
+        k1[q] = k7[q];   // first same as last
+      }
+      // a step shortened to land on an output keeps the earlier proposal
+      h = (last && hs < h) ? fmax(hs * fac, h) : hs * fac;
+    } else {
+      h = hs * fac;
+    }
+  }
+  return 0;
+}
+
 struct OdeArgs {
   bh_fwd_beams b;
   long long count, total;   // total = 2 count when mirrored
@@ -166,71 +233,93 @@ __global__ __launch_bounds__(64) void k_ode_forward(OdeArgs A) {
   double h = A.nt > 1 ? A.ts[1] - A.ts[0] : 0.0;
   long long steps = 0;
   int st = 0;
-  double k1[NS], k2[NS], k3[NS], k4[NS], k5[NS], k6[NS], k7[NS], yt[NS], y5[NS], er[NS];
+  double k1[NS];
   ode_rhs<D>(A.S, mode, y, k1);
-  for (long long out = 1; out < A.nt && st == 0; ++out) {
-    const double tout = A.ts[out];
-    while (t < tout) {
-      if (++steps > A.max_steps) {
-        st = 1;
-        break;
-      }
-      const bool last = t + h >= tout;
-      const double hs = last ? tout - t : h;
-      // Dormand-Prince 5(4) stages
-      for (int q = 0; q < NS; ++q) yt[q] = y[q] + hs * (1.0 / 5.0) * k1[q];
-      ode_rhs<D>(A.S, mode, yt, k2);
-      for (int q = 0; q < NS; ++q) yt[q] = y[q] + hs * ((3.0 / 40.0) * k1[q] + (9.0 / 40.0) * k2[q]);
-      ode_rhs<D>(A.S, mode, yt, k3);
-      for (int q = 0; q < NS; ++q)
-        yt[q] = y[q] + hs * ((44.0 / 45.0) * k1[q] - (56.0 / 15.0) * k2[q] + (32.0 / 9.0) * k3[q]);
-      ode_rhs<D>(A.S, mode, yt, k4);
-      for (int q = 0; q < NS; ++q)
-        yt[q] = y[q] + hs * ((19372.0 / 6561.0) * k1[q] - (25360.0 / 2187.0) * k2[q] +
-                             (64448.0 / 6561.0) * k3[q] - (212.0 / 729.0) * k4[q]);
-      ode_rhs<D>(A.S, mode, yt, k5);
-      for (int q = 0; q < NS; ++q)
-        yt[q] = y[q] + hs * ((9017.0 / 3168.0) * k1[q] - (355.0 / 33.0) * k2[q] +
-                             (46732.0 / 5247.0) * k3[q] + (49.0 / 176.0) * k4[q] -
-                             (5103.0 / 18656.0) * k5[q]);
-      ode_rhs<D>(A.S, mode, yt, k6);
-      for (int q = 0; q < NS; ++q)
-        y5[q] = y[q] + hs * ((35.0 / 384.0) * k1[q] + (500.0 / 1113.0) * k3[q] +
-                             (125.0 / 192.0) * k4[q] - (2187.0 / 6784.0) * k5[q] +
-                             (11.0 / 84.0) * k6[q]);
-      ode_rhs<D>(A.S, mode, y5, k7);
-      // y5 - y4 with the embedded 4th-order weights
-      for (int q = 0; q < NS; ++q)
-        er[q] = hs * ((35.0 / 384.0 - 5179.0 / 57600.0) * k1[q] +
-                      (500.0 / 1113.0 - 7571.0 / 16695.0) * k3[q] +
-                      (125.0 / 192.0 - 393.0 / 640.0) * k4[q] +
-                      (-2187.0 / 6784.0 + 92097.0 / 339200.0) * k5[q] +
-                      (11.0 / 84.0 - 187.0 / 2100.0) * k6[q] - (1.0 / 40.0) * k7[q]);
-      const double en = err_norm<D>(y, y5, er, A.rtol, A.atol);
-      if (!(en == en)) {
-        st = 2;
-        break;
-      }
-      double fac = en > 0.0 ? 0.9 * pow(en, -0.2) : 10.0;
-      fac = fmin(10.0, fmax(0.2, fac));
-      if (en <= 1.0) {
-        t = last ? tout : t + hs;
-        for (int q = 0; q < NS; ++q) {
-          y[q] = y5[q];
-          k1[q] = k7[q];   // first same as last
-        }
-        // a step shortened to land on an output keeps the earlier proposal
-        h = (last && hs < h) ? fmax(hs * fac, h) : hs * fac;
-      } else {
-        h = hs * fac;
-      }
-    }
+  for (long long out = 1; out < A.nt; ++out) {
+    st = dopri_advance<D>(A.S, mode, 1.0, y, k1, t, A.ts[out], h, steps, A.max_steps, A.rtol,
+                          A.atol);
     if (st) break;
     emit(out);
   }
   // rows not reached (step limit / non-finite) are left unwritten: the
   // caller raises on a non-zero status
   A.status[bi] = st;
+}
+
+struct OdeStaticArgs {
+  bh_static_beams b;
+  long long count;
+  SpeedDev S;
+  const double* t_start;
+  const double* t_end;   // null: every beam ends at t = 0
+  const double* mode;
+  double rtol, atol, dt0;
+  long long max_steps;
+  int* status;
+};
+
+// Beams (x, p, full complex M, A at t_start) taken to t_end (0 for the
+// terminal-value beams of the adjoint) in place: solve_ODE_batch_t with
+// ts_i = (t_start, t_end) (gb_solvers.py:1017-1113), whose right-hand side has
+// no absorption term (lam = 0).  Beams with zero amplitude, t_end = t_start
+// or mode 0 (no group velocity; the closed form leaves them in place) are not
+// integrated.
+template <int D>
+__global__ __launch_bounds__(64) void k_ode_static(OdeStaticArgs A) {
+  constexpr int NS = Ode<D>::NS;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= A.count) return;
+  const double t0 = A.t_start[i];
+  const double t1 = A.t_end ? A.t_end[i] : 0.0;
+  const double mode = A.mode[i];
+  const double ar = A.b.A[2 * i], ai = A.b.A[2 * i + 1];
+  if (t1 == t0 || mode == 0.0 || (ar == 0.0 && ai == 0.0)) {
+    A.status[i] = 0;
+    return;
+  }
+  double y[NS];
+  for (int a = 0; a < D; ++a) {
+    y[a] = A.b.x[i * D + a];
+    y[D + a] = A.b.p[i * D + a];
+  }
+  for (int q = 0; q < 2 * D * D; ++q) y[2 * D + q] = A.b.M[2 * i * D * D + q];
+  y[2 * D + 2 * D * D] = ar;
+  y[2 * D + 2 * D * D + 1] = ai;
+  const double dir = t1 < t0 ? -1.0 : 1.0;
+  const double span = fabs(t1 - t0);
+  double h = A.dt0 > 0.0 ? A.dt0 : span;
+  double sv = 0.0;
+  long long steps = 0;
+  double k1[NS];
+  ode_rhs<D>(A.S, mode, y, k1);
+  for (int q = 0; q < NS; ++q) k1[q] *= dir;
+  const int st = dopri_advance<D>(A.S, mode, dir, y, k1, sv, span, h, steps, A.max_steps, A.rtol,
+                                  A.atol);
+  if (st == 0) {
+    for (int a = 0; a < D; ++a) {
+      A.b.x[i * D + a] = y[a];
+      A.b.p[i * D + a] = y[D + a];
+    }
+    for (int q = 0; q < 2 * D * D; ++q) A.b.M[2 * i * D * D + q] = y[2 * D + q];
+    A.b.A[2 * i] = y[2 * D + 2 * D * D];
+    A.b.A[2 * i + 1] = y[2 * D + 2 * D * D + 1];
+  }
+  A.status[i] = st;
+}
+
+SpeedDev speed_dev(const bh_speed* sp, int ndim) {
+  SpeedDev S{};
+  S.ndim = sp->ndim;
+  S.method = sp->method;
+  S.boundary = sp->boundary;
+  for (int i = 0; i < 3; ++i) {
+    S.N[i] = i < ndim ? sp->N[i] : 1;
+    S.spacing[i] = i < ndim ? sp->spacing[i] : 1.0;
+    S.origin[i] = i < ndim ? sp->origin[i] : 0.0;
+  }
+  S.values = sp->values;
+  S.lam = sp->lam;
+  return S;
 }
 
 }  // namespace
@@ -252,16 +341,7 @@ extern "C" int bh_ode_forward(const bh_fwd_beams* b, int ndim, int mirror, const
   a.count = b->count;
   a.mirror = mirror > 0 ? 1 : (mirror < 0 ? -1 : 0);
   a.total = b->count * (mirror > 0 ? 2 : 1);
-  a.S.ndim = sp->ndim;
-  a.S.method = sp->method;
-  a.S.boundary = sp->boundary;
-  for (int i = 0; i < 3; ++i) {
-    a.S.N[i] = i < ndim ? sp->N[i] : 1;
-    a.S.spacing[i] = i < ndim ? sp->spacing[i] : 1.0;
-    a.S.origin[i] = i < ndim ? sp->origin[i] : 0.0;
-  }
-  a.S.values = sp->values;
-  a.S.lam = sp->lam;
+  a.S = speed_dev(sp, ndim);
   a.ts = ts;
   a.nt = nt;
   a.rtol = rtol;
@@ -274,6 +354,43 @@ extern "C" int bh_ode_forward(const bh_fwd_beams* b, int ndim, int mirror, const
   if (ndim == 1) hipLaunchKernelGGL(k_ode_forward<1>, dim3(grid), dim3(64), 0, st, a);
   else if (ndim == 2) hipLaunchKernelGGL(k_ode_forward<2>, dim3(grid), dim3(64), 0, st, a);
   else hipLaunchKernelGGL(k_ode_forward<3>, dim3(grid), dim3(64), 0, st, a);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_ode_static(const bh_static_beams* b, int ndim, const bh_speed* sp,
+                             const double* t_start, const double* t_end, const double* mode,
+                             double lam, double rtol, double atol, double dt0,
+                             int64_t max_steps, int* status, bh_stream s) {
+  bh::clear_error();
+  if (!b || !sp || !t_start || !mode || !status)
+    BH_FAIL(BH_ERR_ARG, "bh_ode_static: null pointer");
+  if (ndim < 1 || ndim > 3 || sp->ndim != ndim)
+    BH_FAIL(BH_ERR_ARG, "bh_ode_static: ndim=%d, speed ndim=%d", ndim, sp->ndim);
+  if (!(rtol > 0.0) || !(atol > 0.0) || max_steps <= 0)
+    BH_FAIL(BH_ERR_ARG, "bh_ode_static: rtol, atol and max_steps must be positive");
+  if (!sp->values) BH_FAIL(BH_ERR_ARG, "bh_ode_static: null speed table");
+  if (b->count < 0) BH_FAIL(BH_ERR_ARG, "bh_ode_static: count=%lld", (long long)b->count);
+  if (b->count == 0) return BH_OK;
+  if (!b->x || !b->p || !b->M || !b->A) BH_FAIL(BH_ERR_ARG, "bh_ode_static: null beam table");
+  OdeStaticArgs a{};
+  a.b = *b;
+  a.count = b->count;
+  a.S = speed_dev(sp, ndim);
+  a.S.lam = lam;
+  a.t_start = t_start;
+  a.t_end = t_end;
+  a.mode = mode;
+  a.rtol = rtol;
+  a.atol = atol;
+  a.dt0 = dt0;
+  a.max_steps = max_steps;
+  a.status = status;
+  hipStream_t st = (hipStream_t)s;
+  const unsigned grid = bh::grid1d(a.count, 64);
+  if (ndim == 1) hipLaunchKernelGGL(k_ode_static<1>, dim3(grid), dim3(64), 0, st, a);
+  else if (ndim == 2) hipLaunchKernelGGL(k_ode_static<2>, dim3(grid), dim3(64), 0, st, a);
+  else hipLaunchKernelGGL(k_ode_static<3>, dim3(grid), dim3(64), 0, st, a);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }

@@ -9,7 +9,10 @@ kernel; these functions also serve as markers selecting that path.
 ``solve_hom_TR`` (gb_solvers.py:484-594) is fused into ``bh_adjoint_params``.
 ``solve_ODE_base`` (gb_solvers.py:941-1016) integrates the beam ODEs in a
 heterogeneous (grid) medium on the GPU (``bh_ode_forward``); inside
-``MSGBSolver`` it selects the trajectory-fed summation.
+``MSGBSolver`` it selects the trajectory-fed summation.  ``solve_ODE_batch_t``
+(gb_solvers.py:1017-1113) integrates per-beam intervals (``bh_ode_static``);
+``MSGBSolver`` runs it for the adjoint and time reversal in a heterogeneous
+medium.
 """
 
 from __future__ import annotations
@@ -134,6 +137,55 @@ def solve_ODE_base(x0, p0, M0, a0, mode, ts, c, lam=0.0, solver_config=None):
                                         .reshape(k, nt, d * d, 2)).reshape(k, nt, d, d)
         At[sel] = torch.view_as_complex(tr[..., 2 * d + 2 * d * d:].contiguous()
                                         .reshape(k, nt, 1, 2))
+    return xt, pt, Mt, At
+
+
+def solve_ODE_batch_t(x0, p0, M0, a0, mode, ts, c, lam=None, solver_config=None):
+    """Beam ODEs over per-beam time grids (gb_solvers.py:1017-1113) without
+    absorption (coupled_rhs, 870-908), full complex M0, on the GPU
+    (``bh_ode_static``).  ts: (b, 2) intervals (t0_i, t1_i), forward or
+    backward in time.  Returns (xt, pt, Mt, At) with shapes (b, 2, d),
+    (b, 2, d), (b, 2, d, d), (b, 2, 1).  Inside ``MSGBSolver`` it is the
+    propagation of the adjoint and time-reversal beams in a heterogeneous
+    medium."""
+    import ctypes as C
+    import torch
+    from ..solvers.msgb_solver import StaticBeams
+    dev = x0.device if isinstance(x0, torch.Tensor) else torch.device("cuda")
+    f64 = lambda v: torch.as_tensor(v, device=dev).to(torch.float64).contiguous()
+    c128 = lambda v: torch.as_tensor(v, device=dev).to(torch.complex128).contiguous()
+    x0, p0, mode, ts = f64(x0), f64(p0), f64(mode).reshape(-1), f64(ts)
+    b, d = x0.shape
+    if ts.ndim != 2 or ts.shape[0] != b or ts.shape[1] != 2:
+        raise NotImplementedError("solve_ODE_batch_t on the GPU takes ts of shape (b, 2): one "
+                                  "(t0, t1) interval per beam")
+    sp = speed_of(c)
+    if sp is None:
+        from ..utils.interp import GridSpeed
+        sp = GridSpeed(np.full((1,) * d, float(c)))
+    cfg = solver_config if solver_config is not None else SolverConfig()
+    beams = StaticBeams(b, d, dev)
+    beams.x.copy_(x0)
+    beams.p.copy_(p0)
+    beams.M.copy_(c128(M0).reshape(b, d, d))
+    beams.A.copy_(c128(a0).reshape(-1))
+    beams.omega.fill_(1.0)
+    t0 = ts[:, 0].contiguous()
+    t1 = ts[:, 1].contiguous()
+    status = torch.empty(max(b, 1), dtype=torch.int32, device=dev)
+    sps = sp.struct(dev)
+    st = beams.struct()
+    _lib.call("bh_ode_static", C.byref(st), d, C.byref(sps), _lib.ptr(t0), _lib.ptr(t1),
+              _lib.ptr(mode), 0.0, float(cfg.rtol), float(cfg.atol), float(cfg.dt0 or 0.0),
+              int(cfg.max_steps), _lib.ptr(status), _lib.stream_handle())
+    if b and bool((status[:b] != 0).any()):
+        raise RuntimeError("beam ODE integration failed: more than max_steps steps or a "
+                           "non-finite state")
+    x_0, p_0 = f64(x0), f64(p0)
+    xt = torch.stack([x_0, beams.x], dim=1)
+    pt = torch.stack([p_0, beams.p], dim=1)
+    Mt = torch.stack([c128(M0).reshape(b, d, d), beams.M], dim=1)
+    At = torch.stack([c128(a0).reshape(-1), beams.A], dim=1).reshape(b, 2, 1)
     return xt, pt, Mt, At
 
 

@@ -11,8 +11,11 @@ four-edge detector is therefore split into its four faces:
   ``prepare_data_domain_for_msgb`` does
   (examples/reconstruction/2d_time_reversal_and_adjoint.py:111-165);
 * faces whose normal is axis 1 are solved in the frame with axes 0 and 1
-  swapped (exact for a homogeneous medium) and their beams are swapped back;
-* the beams of all four faces are summed over the image in one launch.
+  swapped and their beams are swapped back; in a heterogeneous medium the
+  speed is read at the physical position of each detector point, and the
+  beams are integrated to t = 0 in the physical frame;
+* the beams of all four faces are summed over the image in one launch
+  (then weighted by c(x)^-2).
 """
 
 from __future__ import annotations
@@ -74,7 +77,10 @@ class FourEdgeAdjoint:
         self.nt_data = int(nt_data)
         span = float(ts[-1] - ts[0])
         self.dt_data = span / (self.nt_data - 1)
-        self.c = float(domain.c)
+        hom = domain.is_homogeneous
+        self.c = float(domain.c) if hom else None
+        self.medium = None if hom else (domain.speed_model, domain.speed_model)
+        c_face = self.c if hom else 1.0   # the face domains' speed is unused when heterogeneous
         n0, n1 = domain.N
         self.faces = four_edge_faces(domain.N)
         self.S = 2 * n1 + 2 * (n0 - 2)
@@ -84,7 +90,7 @@ class FourEdgeAdjoint:
             nf = len(f["cols"])
             N_rot = (n1, n0) if rot else (n0, n1)
             dx_rot = (domain.dx[1], domain.dx[0]) if rot else domain.dx
-            dom_rot = Domain(N=N_rot, dx=dx_rot, periodic=(False, False), c=self.c, cfl=domain.cfl)
+            dom_rot = Domain(N=N_rot, dx=dx_rot, periodic=(False, False), c=c_face, cfl=domain.cfl)
             tan_dx = dx_rot[1]
             normal_val = 0.0 if f["normal"] == 0.0 else (N_rot[0] - 1) * dx_rot[0]
             pos = np.stack([np.full(nf, normal_val), np.arange(nf) * tan_dx], axis=1)
@@ -93,7 +99,7 @@ class FourEdgeAdjoint:
             nmin = min(self.nt_data, nf)
             aspect = (self.nt_data // nmin, nf // nmin)
             f["data_domain"] = Domain(N=(self.nt_data, nf), dx=(self.dt_data, tan_dx),
-                                      periodic=(False, False), c=self.c, cfl=domain.cfl)
+                                      periodic=(False, False), c=c_face, cfl=domain.cfl)
             key = (self.nt_data, nf, aspect)
             if key not in wpts:
                 dd = DyadicDecomposition(num_levels=num_levels, N=(self.nt_data, nf),
@@ -106,6 +112,11 @@ class FourEdgeAdjoint:
         self.device = self.faces[0]["wpt"].device
         self.points = torch.as_tensor(np.ascontiguousarray(domain.grid.reshape(-1, 2)),
                                       device=self.device)
+        if hom:
+            self.weight = None
+        else:
+            cimg = self.medium[1].evaluate(self.points, device=self.device)[0]
+            self.weight = 1.0 / (cimg * cimg)
 
     def crop(self, data, face):
         import torch
@@ -125,10 +136,15 @@ class FourEdgeAdjoint:
             face_data = self.crop(data, f)
             tabs.append(self.solver.prepare_adjoint_beams(
                 face_data, f["sources"], f["data_domain"], f["wpt"], self.c,
-                swap_axes=f["swap"], plane=f["plane"]))
+                swap_axes=f["swap"], plane=f["plane"], medium=self.medium))
         return StaticBeams.concat(tabs)
 
     def __call__(self, data):
+        import torch
         b = self.beams(data)
-        q = self.solver.sum_static(b, self.points, 1.0 / self.c ** 2, dx=self.domain.dx)
+        if self.weight is None:
+            q = self.solver.sum_static(b, self.points, 1.0 / self.c ** 2, dx=self.domain.dx)
+        else:
+            q = self.solver.sum_static(b, self.points, 1.0, dx=self.domain.dx)
+            q = (q.to(torch.float64) * self.weight).to(q.dtype)
         return q.reshape(self.domain.N)

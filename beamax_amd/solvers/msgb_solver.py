@@ -567,15 +567,24 @@ class MSGBSolver:
         if math.prod(expected) != pos.shape[0]:
             raise ValueError("sources must form a complete Cartesian detector grid.")
 
-    def adjoint_source(self, data, dt: float, c: float, window=None):
-        """_form_adjoint_source (msgb_solver.py:68-120) on the GPU; returns (Nt, Ns...)."""
+    def adjoint_source(self, data, dt: float, c, window=None):
+        """_form_adjoint_source (msgb_solver.py:68-120) on the GPU; returns (Nt, Ns...).
+
+        ``c``: the sound speed at the detectors, a number or one value per
+        detector column (device tensor)."""
         import torch
         dev = data.device
         x = data.to(torch.float64).contiguous()
         nt = x.shape[0]
         ns = x.numel() // nt
         out = torch.empty_like(x)
-        csrc = torch.full((ns,), float(c), dtype=torch.float64, device=dev)
+        if isinstance(c, torch.Tensor):
+            csrc = c.to(device=dev, dtype=torch.float64).reshape(-1).contiguous()
+            if csrc.numel() != ns:
+                raise ValueError(f"c_at_sources has {csrc.numel()} detector samples, which does "
+                                 f"not match {ns} detector columns")
+        else:
+            csrc = torch.full((ns,), float(c), dtype=torch.float64, device=dev)
         w = None
         w2d = 0
         if window is not None:
@@ -588,22 +597,86 @@ class MSGBSolver:
 
     def prepare_adjoint_beams(self, data, sources: Sensor, data_domain: Domain, data_wpt: MSWPT,
                               c: float, window=None, swap_axes: bool = False,
-                              plane=None) -> StaticBeams:
+                              plane=None, medium=None) -> StaticBeams:
         """adjoint source -> data MSWPT -> top_n -> compute_adj_parameters ->
-        solve_hom_TR to t = 0 (msgb_solver.py:1168-1233, 1347-1366)."""
+        solve_hom_TR to t = 0 (msgb_solver.py:1168-1233, 1347-1366).
+
+        ``medium = (detector_speed, image_speed)`` (``GridSpeed`` models in
+        the physical frame) selects the heterogeneous path: the source weight
+        c(sources)^2, the parameters at c(x_T) and the beam ODEs from t_start
+        to 0 in the image medium (msgb_solver.py:1352-1384); ``c`` is then
+        unused."""
+        if plane is None:
+            plane = detector_plane(sources.positions, sources.domain)
+        if medium is not None:
+            c = medium[0].evaluate(self._physical(sources.positions, swap_axes),
+                                   device=data_wpt.device)[0]
         src = self.adjoint_source(data, data_domain.dx[0], c, window)
         coeffs = data_wpt.forward(src, self.input_type)
         idx, val = self._select(coeffs, data_wpt)
-        if plane is None:
-            plane = detector_plane(sources.positions, sources.domain)
-        normal_axis, normal_value, inward = plane
-        beams = StaticBeams(idx.numel(), sources.domain.ndim, data_wpt.device)
-        _lib.call("bh_adjoint_params", C.byref(data_wpt.tiling),
-                  _lib.dbl_array(data_domain.grid_size), _lib.ptr(idx), _lib.ptr(val), idx.numel(),
-                  normal_axis, normal_value, inward, float(c), self.adjoint_relative_guard,
-                  int(swap_axes), beams.struct(), _lib.stream_handle())
+        if medium is not None:
+            beams = self._medium_beams(idx, val, data_wpt, data_domain, sources.domain.ndim, plane,
+                                       swap_axes, False, medium)
+        else:
+            normal_axis, normal_value, inward = plane
+            beams = StaticBeams(idx.numel(), sources.domain.ndim, data_wpt.device)
+            _lib.call("bh_adjoint_params", C.byref(data_wpt.tiling),
+                      _lib.dbl_array(data_domain.grid_size), _lib.ptr(idx), _lib.ptr(val),
+                      idx.numel(), normal_axis, normal_value, inward, float(c),
+                      self.adjoint_relative_guard, int(swap_axes), beams.struct(),
+                      _lib.stream_handle())
         beams.idx, beams.val, beams.coeffs = idx, val, coeffs   # diagnostics
         return beams
+
+    @staticmethod
+    def _physical(positions, swap_axes: bool):
+        """Detector positions in the physical frame (axes 0 and 1 swapped back)."""
+        pos = np.asarray(_np(positions) if not isinstance(positions, np.ndarray) else positions,
+                         dtype=np.float64)
+        if swap_axes:
+            pos = pos.copy()
+            pos[..., [0, 1]] = pos[..., [1, 0]]
+        return np.ascontiguousarray(pos)
+
+    def _medium_beams(self, idx, val, data_wpt: MSWPT, data_domain: Domain, d: int, plane,
+                      swap_axes: bool, time_reversal: bool, medium) -> StaticBeams:
+        """Boundary beams in a heterogeneous medium: compute_TR_parameters /
+        compute_adj_parameters at the local speed c(x_T) of the detector
+        medium (tr_solver_utils.py:336-435, adjoint_solver_utils.py:185-220),
+        then solve_ODE_batch_t from t_start to 0 through the image medium
+        (gb_solvers.py:1017-1113)."""
+        import torch
+        dev = data_wpt.device
+        normal_axis, normal_value, inward = plane
+        k = idx.numel()
+        beams = StaticBeams(k, d, dev)
+        t_start = torch.empty(k, dtype=torch.float64, device=dev)
+        mode = torch.empty(k, dtype=torch.float64, device=dev)
+        det_speed, img_speed = medium
+        sp_det = det_speed.struct(dev)
+        _lib.call("bh_boundary_terminal", C.byref(data_wpt.tiling),
+                  _lib.dbl_array(data_domain.grid_size), _lib.ptr(idx), _lib.ptr(val), k,
+                  normal_axis, normal_value, inward, C.byref(sp_det),
+                  self.adjoint_relative_guard, int(time_reversal), int(swap_axes),
+                  beams.struct(), _lib.ptr(t_start), _lib.ptr(mode), _lib.stream_handle())
+        cfg = self.ode_config
+        status = torch.empty(max(k, 1), dtype=torch.int32, device=dev)
+        sp_img = img_speed.struct(dev)
+        b = beams.struct()
+        _lib.call("bh_ode_static", C.byref(b), d, C.byref(sp_img), _lib.ptr(t_start), None,
+                  _lib.ptr(mode), 0.0, float(cfg.rtol), float(cfg.atol), float(cfg.dt0 or 0.0),
+                  int(cfg.max_steps), _lib.ptr(status), _lib.stream_handle())
+        if k and bool((status[:k] != 0).any()):
+            raise RuntimeError("time-reversal beam ODE integration failed: more than max_steps "
+                               "steps or a non-finite state")
+        beams.t_start, beams.mode = t_start, mode   # diagnostics
+        return beams
+
+    def _medium(self, sources: Sensor, domain: Domain):
+        """(detector speed, image speed) for a heterogeneous problem, else None."""
+        if domain.is_homogeneous and sources.domain.is_homogeneous:
+            return None
+        return gb_solvers.speed_of(sources.domain), gb_solvers.speed_of(domain)
 
     def sum_static(self, beams: StaticBeams, points, scale: float, out=None, dx=None):
         """out[P] = scale * sum_b field_b(points) (compute_TR_result at t = 0).
@@ -644,20 +717,25 @@ class MSGBSolver:
         return out
 
     def prepare_tr_beams(self, data, sources: Sensor, data_domain: Domain, data_wpt: MSWPT,
-                         c: float, swap_axes: bool = False, plane=None) -> StaticBeams:
+                         c: float, swap_axes: bool = False, plane=None,
+                         medium=None) -> StaticBeams:
         """compute_coefficients('both') (the cpos half, = 0.5 * MSWPT(data)) ->
         top_n -> compute_TR_parameters -> solve_hom_TR to t = 0
-        (msgb_solver.py:691-746)."""
+        (msgb_solver.py:691-746); ``medium`` as in ``prepare_adjoint_beams``."""
         coeffs = data_wpt.forward(data, self.input_type, scale=0.5)
         idx, val = self._select(coeffs, data_wpt)
         if plane is None:
             plane = detector_plane(sources.positions, sources.domain)
-        normal_axis, normal_value, inward = plane
-        beams = StaticBeams(idx.numel(), sources.domain.ndim, data_wpt.device)
-        _lib.call("bh_tr_params", C.byref(data_wpt.tiling),
-                  _lib.dbl_array(data_domain.grid_size), _lib.ptr(idx), _lib.ptr(val), idx.numel(),
-                  normal_axis, normal_value, inward, float(c), int(swap_axes), beams.struct(),
-                  _lib.stream_handle())
+        if medium is not None:
+            beams = self._medium_beams(idx, val, data_wpt, data_domain, sources.domain.ndim, plane,
+                                       swap_axes, True, medium)
+        else:
+            normal_axis, normal_value, inward = plane
+            beams = StaticBeams(idx.numel(), sources.domain.ndim, data_wpt.device)
+            _lib.call("bh_tr_params", C.byref(data_wpt.tiling),
+                      _lib.dbl_array(data_domain.grid_size), _lib.ptr(idx), _lib.ptr(val),
+                      idx.numel(), normal_axis, normal_value, inward, float(c), int(swap_axes),
+                      beams.struct(), _lib.stream_handle())
         beams.idx, beams.val, beams.coeffs = idx, val, coeffs
         return beams
 
@@ -678,14 +756,15 @@ class MSGBSolver:
         if pos is None:
             raise ValueError("Unsupported sensor type")
         self._validate_boundary_data(data, data_domain, sources, ts=ts)
-        if not sources.domain.is_homogeneous:
-            raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
         if self.sharding is not None:
             self._check_supported()
+        medium = self._medium(sources, domain)
         if not isinstance(data, torch.Tensor):
             data = torch.as_tensor(np.asarray(data, dtype=np.float64), device=data_wpt.device)
         beams = self.prepare_tr_beams(data.to(data_wpt.device).to(torch.float64), sources,
-                                      data_domain, data_wpt, float(sources.domain.c))
+                                      data_domain, data_wpt,
+                                      None if medium else float(sources.domain.c),
+                                      medium=medium)
         q = self.sum_static(beams, pos, 2.0, dx=domain.dx)
         shape = np.asarray(sensors.positions if isinstance(sensors, Sensor) else pos).shape[:-1]
         return q.reshape(tuple(shape)), beams
@@ -708,15 +787,21 @@ class MSGBSolver:
         if pos is None:
             raise ValueError("Unsupported sensor type for `sensors` in adjoint().")
         self._validate_boundary_data(data, data_domain, sources, ts=ts)
-        if not (domain.is_homogeneous and sources.domain.is_homogeneous):
-            raise NotImplementedError("heterogeneous sound speed is not on the GPU path")
         if self.sharding is not None:
             self._check_supported()
+        medium = self._medium(sources, domain)
         if not isinstance(data, torch.Tensor):
             data = torch.as_tensor(np.asarray(data, dtype=np.float64), device=data_wpt.device)
-        c_src = float(sources.domain.c)
+        c_src = None if medium else float(sources.domain.c)
         beams = self.prepare_adjoint_beams(data.to(data_wpt.device), sources, data_domain,
-                                           data_wpt, c_src, window)
-        q = self.sum_static(beams, pos, 1.0 / float(domain.c) ** 2, dx=domain.dx)
+                                           data_wpt, c_src, window, medium=medium)
+        if medium is None:
+            q = self.sum_static(beams, pos, 1.0 / float(domain.c) ** 2, dx=domain.dx)
+        else:
+            # the c(x)^-2 image weight (msgb_solver.py:123-127, 1384)
+            q = self.sum_static(beams, pos, 1.0, dx=domain.dx)
+            pts = _np(pos) if isinstance(pos, torch.Tensor) else np.asarray(pos, dtype=np.float64)
+            cimg = medium[1].evaluate(pts.reshape(-1, domain.ndim), device=q.device)[0]
+            q = (q.to(torch.float64) / (cimg * cimg)).to(q.dtype)
         shape = np.asarray(sensors.positions if isinstance(sensors, Sensor) else pos).shape[:-1]
         return q.reshape(tuple(shape)), beams

@@ -41,6 +41,10 @@
  *   bh_speed_eval, bh_ode_forward, bh_forward_sum_traj: heterogeneous media
  *                       (utils/interp.py:158-270, gb_solvers.py:787-1016 with
  *                       compute_gaussian_beam_real on the saved states)
+ *   bh_boundary_terminal, bh_ode_static: the adjoint / time reversal in a
+ *                       heterogeneous medium (compute_TR_parameters with c(x_T),
+ *                       tr_solver_utils.py:205-435, adjoint_solver_utils.py:75-220;
+ *                       solve_ODE_batch_t from t_start to 0, gb_solvers.py:1017-1113)
  *   bh_forward_batch_plan, bh_forward_batch_run: a batch of forward sums in
  *                       one launch (no reference counterpart)
  *   bh_forward_sum_modes  the complex-input forward (msgb_solver.py:635-689)
@@ -346,6 +350,28 @@ int bh_speed_eval(const bh_speed* sp, const double* x, int64_t n, double* c, dou
 int bh_ode_forward(const bh_fwd_beams* b, int ndim, int mirror, const bh_speed* sp,
                    const double* ts, int64_t nt, double rtol, double atol, int64_t max_steps,
                    double* traj, int* status, bh_stream s);
+/* bh_adjoint_params (time_reversal = 0) or bh_tr_params (1) with the sound
+ * speed read from a speed model at each beam's detector point x_T (its value
+ * sets the momentum, B^-1 and xdot; its gradient the pdot terms of
+ * mT_inverse), leaving each beam at its terminal state instead of
+ * propagating it: out holds (x_T, p_T, M_T, a_T, omega), t_start[i] the
+ * terminal time and mode[i] the branch sign (device arrays of k doubles). */
+int bh_boundary_terminal(const bh_tiling* t, const double* data_grid_size, const int64_t* idx,
+                         const double* vals, int64_t k, int normal_axis, double normal_value,
+                         double inward_sign, const bh_speed* sp, double guard, int time_reversal,
+                         int swap_axes, bh_static_beams out, double* t_start, double* mode,
+                         bh_stream s);
+/* Beams taken from t_start[i] to t_end[i] (t_end NULL: t = 0, the adjoint's
+ * terminal-value beams) in place through the beam ODEs, full complex M:
+ * solve_ODE_batch_t with ts_i = (t_start_i, t_end_i) (gb_solvers.py:1017-1113,
+ * coupled_rhs 870-908) when lam = 0, the coupled_rhs_absorption amplitude
+ * term (820-867) otherwise.  Adaptive Dormand-Prince 5(4); first step dt0
+ * (<= 0: the whole interval).  Beams with zero amplitude, t_end = t_start or
+ * mode 0 are left as they are.  status[i] = 0, 1 (more than max_steps
+ * steps) or 2 (non-finite state; the beam is left at its start state). */
+int bh_ode_static(const bh_static_beams* b, int ndim, const bh_speed* sp, const double* t_start,
+                  const double* t_end, const double* mode, double lam, double rtol, double atol,
+                  double dt0, int64_t max_steps, int* status, bh_stream s);
 /* bh_forward_sum_culled with the beam states read from a bh_ode_forward
  * trajectory table (beams [0, total) with the table's modes) instead of the
  * homogeneous closed form; omega from b (beam j and its mirror share it). */

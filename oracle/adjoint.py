@@ -2,7 +2,7 @@
 
 FP64 NumPy restatement of ``msgb_solver.py:68-127, 1168-1386``,
 ``tr_solver_utils.py:116-435, 524-750`` and ``adjoint_solver_utils.py``
-(reference), homogeneous medium.
+(reference); heterogeneous media through ``oracle.medium``.
 """
 
 from __future__ import annotations
@@ -30,8 +30,8 @@ def adjoint_source(data, dt, c_at_sources, window=None):
     return -(c ** 2) * der
 
 
-def mT_inverse(xdot, mT_st):
-    """tr_solver_utils.py:116-170 with pdot = 0 (homogeneous medium).
+def mT_inverse(xdot, mT_st, pdot=None):
+    """tr_solver_utils.py:116-170; pdot = 0 (homogeneous medium) by default.
 
     Axis 0 is treated as the detector normal, as in the reference."""
     b, d = xdot.shape
@@ -40,8 +40,9 @@ def mT_inverse(xdot, mT_st):
     C = mT_st[:, 1:, 1:]
     x1 = xdot[:, 0:1]
     xs = xdot[:, 1:]
-    pd1 = np.zeros_like(x1)
-    pds = np.zeros_like(xs)
+    pdot = np.zeros_like(xdot) if pdot is None else np.asarray(pdot, float)
+    pd1 = pdot[:, 0:1]
+    pds = pdot[:, 1:]
     ur = ((pds - np.einsum("bij,bj->bi", C, xs) - B) / x1).reshape(b, 1, d - 1)
     num = (A + np.einsum("bi,bi->b", pd1, x1) - np.einsum("bi,bi->b", pds, xs)
            + np.einsum("bi,bij,bj->b", xs, C, xs) + 2 * np.einsum("bi,bi->b", B, xs))
@@ -55,10 +56,15 @@ def _isclose(a, b):
     return abs(a - b) <= 1e-8 + 1e-5 * abs(b)
 
 
-def tr_parameters(wpt: MSWPT, idx, data_N, data_dx, src_positions, src_N, src_dx, c):
+def tr_parameters(wpt: MSWPT, idx, data_N, data_dx, src_positions, src_N, src_dx, c,
+                  speed=None):
     """tr_solver_utils.py:205-435 (d_spatial > 1 branch and 1-D branch).
 
-    Returns pts, Mts, xts, omega, ats (B,), signum (B,), t_start (B,).
+    With ``speed`` (an ``oracle.medium.Speed``) the sound speed and its
+    gradient are read at each beam's x_T (tr_solver_utils.py:336-343): c
+    sets p_T and xdot, grad c the pdot terms of mT_inverse.
+    Returns pts, Mts, xts, omega, ats (B,), signum (B,), t_start (B,), tau,
+    ktan, c_x (B,).
     """
     t = wpt.tiling
     pos = np.asarray(src_positions, dtype=float)
@@ -90,7 +96,12 @@ def tr_parameters(wpt: MSWPT, idx, data_N, data_dx, src_positions, src_N, src_dx
         xts = np.full((B, 1), normal_value)
     else:
         xts = np.where(mask_tan, xstar[:, tan_index], normal_value)
-    cx = np.full((B, 1), float(c))
+    if speed is None:
+        cx = np.full((B, 1), float(c))
+        gx = np.zeros((B, d_sp))
+    else:
+        cv, gx, _ = speed(xts)
+        cx = cv.reshape(B, 1)
     if d_sp == 1:
         ptan = cn[:, 1:]
         rad = np.maximum((cn[:, :1] / cx) ** 2 - np.sum(cn[:, 1:] ** 2, axis=1, keepdims=True), 0.0)
@@ -120,14 +131,15 @@ def tr_parameters(wpt: MSWPT, idx, data_N, data_dx, src_positions, src_N, src_dx
         pts = pts / cx
     alpha = 2j * (np.pi * sig) ** 2 / om[:, None]
     Minit = np.einsum("bi,ij->bij", alpha, np.eye(d_sp))
+    npt = np.linalg.norm(pts, axis=-1, keepdims=True)
+    xdot = signum[:, None] * cx * pts / npt
+    pdot = -signum[:, None] * gx * npt
     if d_sp == 1:
-        # mT_inverse with d = 1: M = (A) / xdot^2
-        xdot = signum[:, None] * c * pts / np.linalg.norm(pts, axis=-1, keepdims=True)
-        Mts = (Minit[:, 0, 0] / xdot[:, 0] ** 2).reshape(B, 1, 1)
+        # mT_inverse with d = 1: M = (A + pdot xdot) / xdot^2
+        Mts = ((Minit[:, 0, 0] + pdot[:, 0] * xdot[:, 0]) / xdot[:, 0] ** 2).reshape(B, 1, 1)
     else:
-        xdot = signum[:, None] * c * pts / np.linalg.norm(pts, axis=-1, keepdims=True)
         with np.errstate(divide="ignore", invalid="ignore"):
-            Mts = mT_inverse(xdot, Minit)
+            Mts = mT_inverse(xdot, Minit, pdot)
     ats = np.prod(np.sqrt((np.pi * L[None, :]) / (Ll * np.asarray(data_N, float)[None, :])) * sig,
                   axis=1)
     ats = ats * frame_phase(t, box, multi[1:, :].T, wpt.redundancy)
@@ -135,7 +147,7 @@ def tr_parameters(wpt: MSWPT, idx, data_N, data_dx, src_positions, src_N, src_dx
     pts = np.where(graz[:, None], 1.0, pts)
     ats = np.where(graz, 0.0, ats)
     Mts = np.where(graz[:, None, None], 1j * np.eye(d_sp)[None], Mts)
-    return pts, Mts, xts, om, ats, signum, t_start, tau[:, 0], ktan
+    return pts, Mts, xts, om, ats, signum, t_start, tau[:, 0], ktan, cx[:, 0]
 
 
 def principal_b_inverse(tau, ktan, c, guard):
@@ -150,15 +162,36 @@ def principal_b_inverse(tau, ktan, c, guard):
     return np.where(valid, -1j * np.sign(om) / (2.0 * c * sg), 0.0)
 
 
-def adj_parameters(wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, guard=5e-2):
-    """adjoint_solver_utils.py:75-220: TR geometry with a <- a * (-B^-1)."""
-    pts, Mts, xts, om, ats, signum, t_start, tau, ktan = tr_parameters(
-        wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c)
-    binv = -principal_b_inverse(tau, ktan, c, guard)
+def adj_parameters(wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, guard=5e-2,
+                   speed=None):
+    """adjoint_solver_utils.py:75-220: TR geometry with a <- a * (-B^-1),
+    B^-1 with the local speed c(x_T) (adjoint_solver_utils.py:185-197)."""
+    pts, Mts, xts, om, ats, signum, t_start, tau, ktan, cx = tr_parameters(
+        wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, speed)
+    binv = -principal_b_inverse(tau, ktan, cx, guard)
     ex = binv == 0.0
     pts = np.where(ex[:, None], 1.0, pts)
     Mts = np.where(ex[:, None, None], 1j * np.eye(Mts.shape[-1])[None], Mts)
     return pts, Mts, xts, om, ats * binv, signum, t_start
+
+
+def propagate_tr(x, p, M, A, mode, t_start, speed, rtol=1e-11, atol=1e-13):
+    """solve_ODE_batch_t (gb_solvers.py:1017-1113) from each beam's t_start to
+    0 through coupled_rhs (no absorption, 870-908), integrated by
+    ``oracle.medium.integrate``; beams with zero amplitude, t_start = 0 or
+    mode 0 are left in place."""
+    from .medium import integrate
+    x = np.array(x, dtype=float)
+    p = np.array(p, dtype=float)
+    M = np.array(M, dtype=complex)
+    A = np.array(A, dtype=complex)
+    for i in range(x.shape[0]):
+        if t_start[i] == 0.0 or mode[i] == 0.0 or A[i] == 0:
+            continue
+        xt, pt, Mt, At = integrate(x[i], p[i], M[i], A[i], mode[i], [t_start[i], 0.0], speed,
+                                   lam=0.0, rtol=rtol, atol=atol)
+        x[i], p[i], M[i], A[i] = xt[-1], pt[-1], Mt[-1], At[-1]
+    return x, p, M, A
 
 
 def tr_field_sum(x, p, M, A, omega, points):
@@ -175,31 +208,52 @@ def tr_field_sum(x, p, M, A, omega, points):
 
 
 def adjoint_pipeline(data, dt, data_wpt: MSWPT, data_N, data_dx, src_positions, src_N,
-                     src_dx, c, points, thr, guard=5e-2, window=None):
-    """MSGBSolver.adjoint (msgb_solver.py:1299-1386) with solve_hom_TR."""
+                     src_dx, c, points, thr, guard=5e-2, window=None, src_speed=None,
+                     speed=None):
+    """MSGBSolver.adjoint (msgb_solver.py:1299-1386): solve_hom_TR in a
+    homogeneous medium; with ``src_speed`` / ``speed`` (the detector's and
+    the image's ``oracle.medium.Speed``) the source weight c(sources)^2,
+    the parameters at c(x_T), the ODE propagation and the c(x)^-2 image
+    weight of the heterogeneous path."""
     from .topn import top_n
-    src = adjoint_source(data, dt, np.full(np.asarray(src_positions).shape[0], float(c)), window)
+    points = np.asarray(points, float)
+    src_positions = np.asarray(src_positions, float)
+    if src_speed is None:
+        c_src = np.full(src_positions.shape[0], float(c))
+    else:
+        c_src = src_speed(src_positions)[0]
+    src = adjoint_source(data, dt, c_src, window)
     coeffs = data_wpt.forward(src, "spatial")
     n_eff = min(int(thr), data_wpt.total_coeffs)
     idx, val = top_n(coeffs, n_eff)
     pts, Mts, xts, om, ats, signum, t_start = adj_parameters(
-        data_wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, guard)
+        data_wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, guard, src_speed)
     ats = ats * val
-    x, p, M, A = solve_hom_TR(xts, pts, Mts, ats, signum, t_start, c)
-    q = tr_field_sum(x, p, M, A, om, np.asarray(points, float))
-    return q / (c ** 2), (idx, val, x, p, M, A, om)
+    if speed is None and src_speed is None:
+        x, p, M, A = solve_hom_TR(xts, pts, Mts, ats, signum, t_start, c)
+        w = 1.0 / c ** 2
+    else:
+        x, p, M, A = propagate_tr(xts, pts, Mts, ats, signum, t_start, speed)
+        w = 1.0 / speed(points)[0] ** 2
+    q = tr_field_sum(x, p, M, A, om, points)
+    return q * w, (idx, val, x, p, M, A, om)
 
 
 def time_reversal_pipeline(data, data_wpt: MSWPT, data_N, data_dx, src_positions, src_N, src_dx,
-                           c, points, thr):
-    """MSGBSolver.time_reversal (msgb_solver.py:691-746, 984-1111) with solve_hom_TR:
+                           c, points, thr, src_speed=None, speed=None):
+    """MSGBSolver.time_reversal (msgb_solver.py:691-746, 984-1111):
     cpos = 0.5 MSWPT(data) (dpdt = 0), top_n over all coefficients,
-    compute_TR_parameters, TR field at t = 0, times 2."""
+    compute_TR_parameters, TR field at t = 0, times 2; solve_hom_TR in a
+    homogeneous medium, parameters at c(x_T) and the ODE propagation with
+    ``src_speed`` / ``speed``."""
     from .topn import top_n
     cpos = 0.5 * (data_wpt.forward(np.asarray(data, float)) + 1j * 0.0)
     idx, val = top_n(cpos, min(int(thr), data_wpt.total_coeffs))
-    pts, Mts, xts, om, ats, signum, t_start, _, _ = tr_parameters(
-        data_wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c)
-    x, p, M, A = solve_hom_TR(xts, pts, Mts, ats * val, signum, t_start, c)
+    pts, Mts, xts, om, ats, signum, t_start, _, _, _ = tr_parameters(
+        data_wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, src_speed)
+    if speed is None and src_speed is None:
+        x, p, M, A = solve_hom_TR(xts, pts, Mts, ats * val, signum, t_start, c)
+    else:
+        x, p, M, A = propagate_tr(xts, pts, Mts, ats * val, signum, t_start, speed)
     q = tr_field_sum(x, p, M, A, om, np.asarray(points, float))
     return 2.0 * q, (idx, val, x, p, M, A, om)

@@ -241,7 +241,7 @@ def test_time_reversal_matches_oracle(gpu):
     ridx, _ = otopn.top_n(cref, 2500)
     assert len(set(ridx.tolist()) ^ set(_n(beams.idx).tolist())) <= 2
     gidx, gval = _n(beams.idx), _n(beams.val)
-    pts, Mts, xts, om, ats, signum, t_start, _, _ = oadj.tr_parameters(
+    pts, Mts, xts, om, ats, signum, t_start, _, _, _ = oadj.tr_parameters(
         ow, gidx, (nt, n), (dt, 1e-4), sources.positions, (n, n), (1e-4, 1e-4), 1500.0)
     x, p, M, A = obeams.solve_hom_TR(xts, pts, Mts, ats * gval, signum, t_start, 1500.0)
     assert rel(M, _n(beams.M)) < 1e-12 and rel(A, _n(beams.A)) < 1e-12

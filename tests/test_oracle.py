@@ -606,3 +606,108 @@ def test_ode_conserves_hamiltonian_in_a_heterogeneous_medium():
     H = sp(xt)[0] * np.linalg.norm(pt, axis=1)
     assert np.max(np.abs(H - H[0])) <= 1e-8 * abs(H[0])
     assert np.ptp(xt[:, 0]) > 0 and np.all(np.isfinite(Mt))
+
+
+# ---------------------------------------------------------------------------
+# adjoint / time reversal in a heterogeneous medium
+# (tr_solver_utils.py:21-170, 336-435; adjoint_solver_utils.py:185-220;
+#  gb_solvers.py:1017-1113)
+
+def _mT_forward(xdot, pdot, M):
+    """tr_solver_utils.py:67-113: spatial Hessian -> spacetime Hessian (the
+    relation mT_inverse inverts)."""
+    ul = np.einsum("bi,bij,bj->b", xdot, M, xdot) - np.einsum("bi,bi->b", pdot, xdot)
+    ur = (pdot - np.einsum("bij,bj->bi", M, xdot))[:, 1:]
+    out = M.copy()
+    out[:, 0, 0] = ul
+    out[:, 0, 1:] = ur
+    out[:, 1:, 0] = ur
+    return out
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_mT_inverse_with_pdot_inverts_mT_forward(d):
+    """The spacetime Hessian of the terminal beams is recovered from the
+    spatial one by the forward relation, with a non-zero pdot (grad c)."""
+    rng = np.random.default_rng(130 + d)
+    b = 7
+    xdot = rng.standard_normal((b, d))
+    xdot[:, 0] += 2.0 * np.sign(xdot[:, 0])
+    pdot = rng.standard_normal((b, d))
+    Mst = np.zeros((b, d, d), complex)
+    for i in range(b):
+        Mst[i] = np.diag(1j * rng.uniform(0.5, 2.0, d) + rng.standard_normal(d))
+    M = oadj.mT_inverse(xdot, Mst, pdot)
+    assert np.allclose(M, np.transpose(M, (0, 2, 1)))
+    assert np.allclose(_mT_forward(xdot, pdot, M), Mst, rtol=1e-12, atol=1e-12)
+    # and without pdot the two relations differ (the terms are not inert)
+    assert not np.allclose(_mT_forward(xdot, 0 * pdot, M), Mst)
+
+
+def test_tr_parameters_constant_speed_model_matches_scalar():
+    w, pos = _small_tr_case(15)
+    idx = np.arange(96)
+    sp = omed.Speed(np.full((16, 16), 0.3), (1.0, 1.0))
+    a = oadj.tr_parameters(w, idx, (16, 16), (1.0, 1.0), pos, (16, 16), (1.0, 1.0), 0.3)
+    b = oadj.tr_parameters(w, idx, (16, 16), (1.0, 1.0), pos, (16, 16), (1.0, 1.0), 0.3, speed=sp)
+    for u, v in zip(a, b):
+        assert np.allclose(u, v, rtol=1e-14, atol=0, equal_nan=True)
+    a = oadj.adj_parameters(w, idx, (16, 16), (1.0, 1.0), pos, (16, 16), (1.0, 1.0), 0.3)
+    b = oadj.adj_parameters(w, idx, (16, 16), (1.0, 1.0), pos, (16, 16), (1.0, 1.0), 0.3,
+                            speed=sp)
+    for u, v in zip(a, b):
+        assert np.allclose(u, v, rtol=1e-14, atol=0, equal_nan=True)
+
+
+def test_tr_parameters_use_the_local_speed():
+    """|p_T| = 2 pi / c(x_T) in 2-D (momentum on the slowness surface of the
+    local speed), B^-1 with the local c."""
+    w, pos = _small_tr_case(0)
+    idx = np.arange(96)
+    X, Y = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    field = 0.3 + 0.01 * Y + 0.005 * X
+    sp = omed.Speed(field, (1.0, 1.0))
+    pts, Mts, xts, om, ats, signum, t_start, tau, ktan, cx = oadj.tr_parameters(
+        w, idx, (16, 16), (1.0, 1.0), pos, (16, 16), (1.0, 1.0), 0.3, speed=sp)
+    assert np.allclose(cx, sp(xts)[0])
+    live = np.abs(pts[:, 0]) != 1.0
+    live &= np.abs(pts[:, 0]) > 0
+    assert live.sum() > 10
+    assert np.allclose(np.linalg.norm(pts[live], axis=1), 2 * np.pi / cx[live], rtol=1e-12)
+    _, _, _, _, aa, *_ = oadj.adj_parameters(w, idx, (16, 16), (1.0, 1.0), pos, (16, 16),
+                                             (1.0, 1.0), 0.3, speed=sp)
+    binv = -oadj.principal_b_inverse(tau, ktan, cx, 5e-2)
+    nz = binv != 0
+    assert nz.sum() > 5
+    assert np.allclose(aa[nz], (ats * binv)[nz])
+
+
+def test_propagate_tr_reverses_forward_and_matches_linear_medium():
+    """Backward integration from t_start to 0: (1) in c = 1 + x (1-D) it
+    matches the closed form x(t) = (x0 + 1) e^t - 1, p = p0 e^-t,
+    M = M0 e^-2t, A = A0 e^(t/2) run backwards; (2) in a 2-D medium a forward
+    run undone by propagate_tr returns the start state."""
+    grid = np.arange(0, 301) * 0.02
+    sp1 = omed.Speed(1.0 + grid, (0.02,), method="linear")
+    T = 0.8
+    e = np.exp(T)
+    x, p, M, A = oadj.propagate_tr(np.array([[1.1 * e - 1.0]]), np.array([[1.2 / e]]),
+                                   np.array([[[0.2j / e ** 2]]]), np.array([np.exp(T / 2)]),
+                                   np.array([1.0]), np.array([T]), sp1)
+    assert np.allclose(x[0, 0], 0.1, rtol=1e-9)
+    assert np.allclose(p[0, 0], 1.2, rtol=1e-9)
+    assert np.allclose(M[0, 0, 0], 0.2j, rtol=1e-9)
+    assert np.allclose(A[0], 1.0, rtol=1e-9)
+    X, Y = np.meshgrid(np.arange(40), np.arange(40), indexing="ij")
+    sp2 = omed.Speed(1500 + 100 * np.sin(X / 7.0) * np.cos(Y / 9.0), (1e-4, 1e-4))
+    x0 = np.array([1.5e-3, 2e-3])
+    p0 = 2 * np.pi * np.array([0.6, 0.8]) / 1500
+    M0 = np.array([[2e3j + 5, 1.0], [1.0, 3e3j]])
+    xt, pt, Mt, At = omed.integrate(x0, p0, M0, 0.3 + 0.4j, -1.0, [0.0, 1e-6], sp2)
+    x, p, M, A = oadj.propagate_tr(xt[-1:], pt[-1:], Mt[-1:], At[-1:], np.array([-1.0]),
+                                   np.array([1e-6]), sp2)
+    r = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert r(x[0], x0) < 1e-9 and r(p[0], p0) < 1e-9
+    assert r(M[0], M0) < 1e-7
+    assert abs(A[0] - (0.3 + 0.4j)) < 1e-7 * 0.5
+    assert r(Mt[-1], M0) > 1e-3          # the run moved the state
