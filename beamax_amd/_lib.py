@@ -134,6 +134,22 @@ SIGNATURES = {
     "bh_forward_batch_plan": (C.c_int, [C.POINTER(BatchSolve), C.c_int, C.c_int, C.c_double, C.c_int,
                                         _P, _SZ, C.POINTER(C.c_int64), _P]),
     "bh_forward_batch_run": (C.c_int, [_P, C.c_int, _I64, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "bh_nn_gemm_workspace_size": (_SZ, [_I64, _I64, _I64, C.c_int]),
+    "bh_nn_gemm": (C.c_int, [_P, C.c_int, _I64, _P, C.c_int, _I64, _I64, _I64, _I64, C.c_float,
+                             _P, C.c_int, _P, C.c_int, _I64, C.c_int, _P, _SZ, _P]),
+    "bh_nn_im2col3": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "bh_nn_col2im3": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P]),
+    "bh_nn_pool2": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P]),
+    "bh_nn_up2": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P]),
+    "bh_nn_add": (C.c_int, [_P, _P, _I64, _P, _P]),
+    "bh_nn_relu_bwd": (C.c_int, [_P, _P, _I64, _P, _P]),
+    "bh_nn_colsum_workspace_size": (_SZ, [_I64, C.c_int]),
+    "bh_nn_colsum": (C.c_int, [_P, _I64, C.c_int, _P, _P, _SZ, _P]),
+    "bh_nn_mse_workspace_size": (_SZ, [_I64]),
+    "bh_nn_mse": (C.c_int, [_P, C.c_int, _P, _I64, _P, C.c_int, _P, _SZ, _P]),
+    "bh_nn_adam": (C.c_int, [_P, _P, _P, _P, _I64, C.c_float, C.c_float, C.c_float, C.c_float,
+                             _I64, _P, _P]),
+    "bh_nn_cast": (C.c_int, [_P, C.c_int, _I64, C.c_float, _P, _P]),
     "bh_static_sum_workspace_size": (_SZ, [_I64, C.c_int]),
     "bh_static_sum": (C.c_int, [C.POINTER(StaticBeams), C.c_int, _P, _I64, C.c_double, C.c_int,
                                 _P, _P, _SZ, _P]),

@@ -156,12 +156,15 @@ class BatchedForward:
         self.graph.replay()
         return self.outputs
 
-    def tune(self, tiles=(4, 8, 16, 32), reps: int = 2):
+    def tune(self, tiles=None, reps: int = 2):
         """Time the batched launch per tile shape; keep and cache the fastest."""
         import torch
         self.prepare()
         best = None
         times = {}
+        if tiles is None:
+            # the FP32-split kernels take 8, 16 or 32 groups per block
+            tiles = (4, 8, 16, 32) if self.solver.precision == "fp64" else (8, 16, 32)
         for gpb in tiles:
             self.set_tile(gpb)
             self.sum()

@@ -58,24 +58,48 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
                                             long long bz) {
   using Rec = typename RecOf<D, F64>::type;
   constexpr int GPB = 4 * SPT;
+  // culling and work unit: U adjacent groups of one run (runs hold an even
+  // number of groups, so a unit never straddles two); the FP32-split form
+  // evaluates a unit's two groups from one set of pair coefficients
+  constexpr int U = F64 ? 1 : 2;
+  static_assert(F64 || SPT >= 2, "the FP32-split form needs at least 8 groups per block");
+  constexpr int NU = GPB / U;    // units per block
+  constexpr int UPW = SPT / U;   // units per wave (wave wv owns units wv + 4 i)
   __shared__ Rec rec[kRecChunk];
-  __shared__ unsigned long long mask[GPB][4];
-  __shared__ double accs[GPB][64];
+  __shared__ unsigned long long mask[NU][4];
   __shared__ int gseg[GPB], gj0[GPB];
+  // FP32-split: per unit its first point and step; per wave the compacted
+  // list of active (record, unit) pairs of the chunk and one pass of 64
+  // pair coefficients
+  __shared__ double gpt[F64 ? 1 : NU][D], gstep[F64 ? 1 : NU][D];
+  __shared__ unsigned char plist[F64 ? 1 : 4][F64 ? 1 : 256 * UPW];
+  __shared__ PairQ coef[F64 ? 1 : 4][F64 ? 1 : 64];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const long long gbase = bx * GPB;
   const long long t = STATIC ? 0 : by;
   const double tv = STATIC ? 0.0 : A.ts[t];
   const int ng = (int)((A.G.ngroups - gbase) < GPB ? (A.G.ngroups - gbase) : GPB);
-  // accumulators and group tables in LDS; each group's point is reloaded
-  // (L1/L2 resident) when the wave evaluates that group
-  for (int i = 0; i < SPT; ++i) accs[wv + 4 * i][lane] = 0.0;
+  const int nu = (ng + U - 1) / U;
+  // group tables in LDS; a wave's accumulators (its SPT groups: unit i,
+  // group h at i * U + h) in registers
+  double acc[SPT];
+  for (int i = 0; i < SPT; ++i) acc[i] = 0.0;
+  const double ld = (double)lane;
+  const float lf = (float)lane;
   if (threadIdx.x < GPB && (int)threadIdx.x < ng) {
     const long long g = gbase + threadIdx.x;
     const int k = A.G.grp_seg[g];
     gseg[threadIdx.x] = k;
     gj0[threadIdx.x] = (int)((g - A.G.g0[k]) * 64);
+    if constexpr (!F64) {
+      if (threadIdx.x % U == 0) {
+        for (int a = 0; a < D; ++a) {
+          gpt[threadIdx.x / U][a] = A.G.pts[g * 64 * D + a];
+          gstep[threadIdx.x / U][a] = A.G.v[(long long)k * D + a];
+        }
+      }
+    }
   }
   unsigned long long npairs = 0;
   const long long total = STATIC ? A.count : A.total;
@@ -127,7 +151,8 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
     if (live && A.E > 0.0) env_matrix<D>(R, Q);
     long long cur_seg = -1;
     int lo = 1, hi = 0;
-    for (int gl = 0; gl < ng; ++gl) {
+    for (int ul = 0; ul < nu; ++ul) {
+      const int gl = ul * U;
       bool act = live;
       if (live && A.E > 0.0) {
         const int k = gseg[gl];
@@ -136,13 +161,76 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
           cur_seg = k;
         }
         const int j0 = gj0[gl];
-        act = (hi >= lo) && (hi >= j0) && (lo <= j0 + 63);
+        act = (hi >= lo) && (hi >= j0) && (lo <= j0 + 64 * U - 1);
       }
       const unsigned long long m = __ballot(act);
-      if (lane == 0) mask[gl][wv] = m;
+      if (lane == 0) mask[ul][wv] = m;
       npairs += __popcll(m);
     }
     __syncthreads();
+    if constexpr (!F64) {
+      // 1. compact the wave's active pairs: unit by unit (its units
+      //    wv + 4 i), record order within a unit
+      int gs[UPW + 1];
+      int base = 0;
+      for (int i = 0; i < UPW; ++i) {
+        gs[i] = base;
+        const int ul = wv + 4 * i;
+        if (ul >= nu) continue;
+        for (int q = 0; q < 4; ++q) {
+          unsigned long long m = mask[ul][q];
+          const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)m);
+          const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
+          m = ((unsigned long long)hi32 << 32) | lo32;
+          if (!m) continue;
+          if ((m >> lane) & 1ull) {
+            const int pos = __builtin_amdgcn_mbcnt_hi(hi32, __builtin_amdgcn_mbcnt_lo(lo32, 0u));
+            plist[wv][base + pos] = (unsigned char)(q * 64 + lane);
+          }
+          base += __popcll(m);
+        }
+      }
+      gs[UPW] = base;
+      wave_lds_fence();
+      // 2. passes of up to 64 pairs: coefficients lane-parallel, then each
+      //    unit's slots in order; lane l of a unit evaluates its points
+      //    gpt + l gstep (pa) and gpt + (l + 64) gstep (pb).  A unit's terms
+      //    of the chunk form one FP32 chain in record order whatever the pass
+      //    boundaries, so every tile shape gives the same sums.
+      float pa[UPW], pb[UPW];
+      for (int i = 0; i < UPW; ++i) {
+        pa[i] = 0.f;
+        pb[i] = 0.f;
+      }
+      for (int b = 0; b < base; b += 64) {
+        const int e = b + lane;
+        if (e < base) {
+          int gi = 0;
+          for (int i = 1; i < UPW; ++i) gi = (e >= gs[i]) ? i : gi;
+          const int ul = wv + 4 * gi;
+          coef[wv][lane] = pair_quad<D>(rec[plist[wv][e]], gpt[ul], gstep[ul]);
+        }
+        wave_lds_fence();
+        for (int i = 0; i < UPW; ++i) {
+          const int lo = (gs[i] > b ? gs[i] : b) - b;
+          const int hi = (gs[i + 1] < b + 64 ? gs[i + 1] : b + 64) - b;
+          if (lo >= hi) continue;
+          // two pairs per step, each at lanes l and l + 64
+          int sl = lo;
+          for (; sl + 1 < hi; sl += 2) {
+            const PairQ c1 = coef[wv][sl], c2 = coef[wv][sl + 1];
+            eval_quad2(c1, ld, lf, pa[i], pb[i]);
+            eval_quad2(c2, ld, lf, pa[i], pb[i]);
+          }
+          if (sl < hi) eval_quad2(coef[wv][sl], ld, lf, pa[i], pb[i]);
+        }
+        wave_lds_fence();   // the slots are consumed before the next pass
+      }
+      for (int i = 0; i < UPW; ++i) {
+        acc[i * 2] += (double)pa[i];
+        acc[i * 2 + 1] += (double)pb[i];
+      }
+    } else
     for (int i = 0; i < SPT; ++i) {
       const int gl = wv + 4 * i;
       if (gl >= ng) continue;
@@ -184,22 +272,22 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
           }
         }
       }
-      accs[gl][lane] += (double)part;
+      acc[i] += (double)part;
     }
   }
-  if (A.pairs && lane == 0) atomicAdd(A.pairs, npairs);
+  if (A.pairs && lane == 0) atomicAdd(A.pairs, npairs * U);
   for (int i = 0; i < SPT; ++i) {
-    const int gl = wv + 4 * i;
+    const int gl = (wv + 4 * (i / U)) * U + i % U;   // unit wv + 4 (i / U), group i % U
     if (gl >= ng) continue;
     const int col = A.G.col[(gbase + gl) * 64 + lane];
     if (col < 0) continue;
     const long long o = STATIC ? col : t * A.G.S + col;
     if (A.split > 1) {
-      A.part[bz * A.rows * A.G.S + o] = accs[gl][lane];
+      A.part[bz * A.rows * A.G.S + o] = acc[i];
     } else {
       const long long oo = o * A.ostride + A.ooff;
-      if constexpr (F64) ((double*)A.out)[oo] = A.scale * accs[gl][lane];
-      else ((float*)A.out)[oo] = (float)(A.scale * accs[gl][lane]);
+      if constexpr (F64) ((double*)A.out)[oo] = A.scale * acc[i];
+      else ((float*)A.out)[oo] = (float)(A.scale * acc[i]);
     }
   }
 }
@@ -275,12 +363,13 @@ struct Shape {
   int split;
 };
 
-inline Shape culled_shape(long long G, long long ny, long long nchunk, bool have_part) {
+inline Shape culled_shape(long long G, long long ny, long long nchunk, bool have_part,
+                          int min_gpb) {
   const long long target = 4096;   // ~ 16 block rounds of 256 CUs
-  Shape sh{4, 1};
+  Shape sh{min_gpb, 1};
   const int opts[4] = {32, 16, 8, 4};
   for (int o : opts) {
-    if ((G + o - 1) / o * ny >= 1024 || o == 4) {
+    if ((G + o - 1) / o * ny >= 1024 || o <= min_gpb) {
       sh.gpb = o;
       break;
     }
@@ -300,7 +389,8 @@ int launch_culled(CulledArgs a, long long ny, long long nchunk, void* part_ws, s
                   hipStream_t s) {
   const long long G = a.G.ngroups;
   const size_t per_split = (size_t)ny * a.G.S * sizeof(double);
-  const Shape sh = culled_shape(G, ny, nchunk, part_ws != nullptr && part_bytes >= 2 * per_split);
+  const Shape sh = culled_shape(G, ny, nchunk, part_ws != nullptr && part_bytes >= 2 * per_split,
+                                F64 ? 4 : 8);
   a.split = sh.split;
   if (sh.split > 1 && (size_t)sh.split * per_split > part_bytes)
     a.split = (int)(part_bytes / per_split);
@@ -310,7 +400,7 @@ int launch_culled(CulledArgs a, long long ny, long long nchunk, void* part_ws, s
   if (sh.gpb == 32) hipLaunchKernelGGL((k_culled_sum<D, 8, F64, STATIC>), grid, dim3(256), 0, s, a);
   else if (sh.gpb == 16) hipLaunchKernelGGL((k_culled_sum<D, 4, F64, STATIC>), grid, dim3(256), 0, s, a);
   else if (sh.gpb == 8) hipLaunchKernelGGL((k_culled_sum<D, 2, F64, STATIC>), grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_culled_sum<D, 1, F64, STATIC>), grid, dim3(256), 0, s, a);
+  else if constexpr (F64) hipLaunchKernelGGL((k_culled_sum<D, 1, F64, STATIC>), grid, dim3(256), 0, s, a);
   if (a.split > 1) {
     const long long n = ny * a.G.S;
     hipLaunchKernelGGL(k_combine, dim3(bh::grid1d(n, 256)), dim3(256), 0, s, (const double*)part_ws,
@@ -543,11 +633,14 @@ extern "C" int bh_forward_batch_run(const void* plan, int nsolve, int64_t nblock
     BH_FAIL(BH_ERR_ARG, "bh_forward_batch_run: precision=%d", precision);
   if (pairs) BH_FAIL(BH_ERR_UNSUPPORTED, "bh_forward_batch_run: pair counting is not wired");
   if (nblocks <= 0) return BH_OK;
+  const bool f64 = precision == BH_PRECISION_FP64;
+  if (!f64 && gpb == 4)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_batch_run: the FP32-split form takes 8, 16 or 32 groups "
+                        "per block (gpb=4 is FP64 only)");
   const size_t dbytes = ((size_t)nsolve * sizeof(BatchDesc) + 255) / 256 * 256;
   const BatchDesc* d = (const BatchDesc*)plan;
   const long long* start = (const long long*)((const char*)plan + dbytes);
   hipStream_t st = (hipStream_t)s;
-  const bool f64 = precision == BH_PRECISION_FP64;
   dim3 grid((unsigned)nblocks);
 #define BH_BATCH(DD, SS)                                                                       \
   do {                                                                                        \
@@ -561,7 +654,8 @@ extern "C" int bh_forward_batch_run(const void* plan, int nsolve, int64_t nblock
     if (gpb == 32) BH_BATCH(DD, 8);                                                           \
     else if (gpb == 16) BH_BATCH(DD, 4);                                                      \
     else if (gpb == 8) BH_BATCH(DD, 2);                                                       \
-    else BH_BATCH(DD, 1);                                                                     \
+    else hipLaunchKernelGGL((k_culled_batch<DD, 1, true>), grid, dim3(256), 0, st, d, start,   \
+                            nsolve);                                                          \
   } while (0)
   if (ndim == 1) BH_BATCH_D(1);
   else if (ndim == 2) BH_BATCH_D(2);

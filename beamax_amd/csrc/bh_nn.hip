@@ -1,0 +1,574 @@
+// Kernels of the learned reconstruction network (a U-Net on 2-D images,
+// NHWC bf16 activations): a bf16 MFMA GEMM with FP32 accumulation (convolutions
+// run as im2col + GEMM, their gradients as GEMMs + col2im), and the
+// elementwise pieces of forward and backward passes and of the Adam update.
+// No reference counterpart: the reference has no reconstruction network.
+//
+// GEMM: C[M][N] = alpha * sum_k A(m, k) B(k, n) (+ bias[n], ReLU), with
+//   A stored [M][lda] (ta = 0, k contiguous) or [K][lda] (ta = 1),
+//   B stored [K][ldb] (tb = 0, n contiguous) or [N][ldb] (tb = 1),
+// the contiguous dimension of each operand a multiple of 8 (16-byte vectors).
+// Workgroup tile 64 x 64, k step 32, four waves of 32 x 32 each made of 2 x 2
+// v_mfma_f32_16x16x32_bf16 blocks; both operands are staged in LDS with k
+// contiguous so every fragment is one 16-byte LDS read.  Split-K (grid z)
+// writes FP32 partial tiles that bh_nn_sum_partials adds in order.
+#include "bh_internal.hpp"
+
+namespace {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float floatx4;
+typedef unsigned short bfbits;   // bf16 storage
+
+__device__ inline float bf2f(bfbits b) { return __uint_as_float((unsigned)b << 16); }
+__device__ inline bfbits f2bf(float f) {
+  const __bf16 h = (__bf16)f;   // round to nearest even
+  return __builtin_bit_cast(bfbits, h);
+}
+
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;
+
+struct GemmArgs {
+  const bfbits* A;
+  const bfbits* B;
+  void* C;
+  long long M, N, K, lda, ldb, ldc;
+  int ta, tb;
+  int out_bf16;
+  const float* bias;
+  int relu;
+  float alpha;
+  long long kchunk;         // k range per split (multiple of BK)
+  long long split_stride;   // elements between split partials (FP32 output)
+};
+
+__device__ inline void load_k_contig(const bfbits* src, long long rows, long long K, long long ld,
+                                     long long r0, long long k0, bfbits (*dst)[LDK]) {
+  // 64 rows x 32 k, thread t: row t / 4, k segment (t % 4) * 8
+  const int t = threadIdx.x;
+  const int row = t >> 2, ks = (t & 3) * 8;
+  const long long r = r0 + row, k = k0 + ks;
+  uint4 v = make_uint4(0, 0, 0, 0);
+  if (r < rows && k < K) v = *reinterpret_cast<const uint4*>(src + r * ld + k);
+  *reinterpret_cast<uint4*>(&dst[row][ks]) = v;
+}
+
+__device__ inline void load_k_strided(const bfbits* src, long long rows, long long K, long long ld,
+                                      long long r0, long long k0, bfbits (*dst)[LDK]) {
+  // storage [K][ld] with the 64 rows contiguous: thread t reads 8 rows at
+  // k = t / 8 and writes them transposed
+  const int t = threadIdx.x;
+  const int kk = t >> 3, rs = (t & 7) * 8;
+  const long long k = k0 + kk, r = r0 + rs;
+  uint4 v = make_uint4(0, 0, 0, 0);
+  if (k < K && r < rows) v = *reinterpret_cast<const uint4*>(src + k * ld + r);
+  const bfbits* e = reinterpret_cast<const bfbits*>(&v);
+  for (int i = 0; i < 8; ++i) dst[rs + i][kk] = e[i];
+}
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bfbits As[BM][LDK];
+  __shared__ __attribute__((aligned(16))) bfbits Bs[BN][LDK];
+  const long long m0 = blockIdx.y * (long long)BM, n0 = blockIdx.x * (long long)BN;
+  const long long kb = blockIdx.z * g.kchunk;
+  const long long ke = (kb + g.kchunk < g.K) ? kb + g.kchunk : g.K;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  floatx4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (long long k0 = kb; k0 < ke; k0 += BK) {
+    if (g.ta) load_k_strided(g.A, g.M, ke, g.lda, m0, k0, As);
+    else load_k_contig(g.A, g.M, ke, g.lda, m0, k0, As);
+    if (g.tb) load_k_contig(g.B, g.N, ke, g.ldb, n0, k0, Bs);
+    else load_k_strided(g.B, g.N, ke, g.ldb, n0, k0, Bs);
+    __syncthreads();
+    bf16x8 a[2], b[2];
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int i = 0; i < 2; ++i)
+      a[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&As[wm + i * 16 + fr][fk]));
+    for (int j = 0; j < 2; ++j)
+      b[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Bs[wn + j * 16 + fr][fk]));
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // C fragment: column lane % 16, rows (lane / 16) * 4 + r
+  const int cc = lane & 15, cr = (lane >> 4) * 4;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      const long long n = n0 + wn + j * 16 + cc;
+      if (n >= g.N) continue;
+      const float bv = g.bias ? g.bias[n] : 0.f;
+      for (int r = 0; r < 4; ++r) {
+        const long long m = m0 + wm + i * 16 + cr + r;
+        if (m >= g.M) continue;
+        float v = g.alpha * acc[i][j][r] + bv;
+        if (g.relu) v = fmaxf(v, 0.f);
+        if (g.out_bf16) ((bfbits*)g.C)[m * g.ldc + n] = f2bf(v);
+        else ((float*)g.C)[blockIdx.z * g.split_stride + m * g.ldc + n] = v;
+      }
+    }
+}
+
+__global__ void k_sum_partials(const float* __restrict__ part, int nsplit, long long n,
+                               float* __restrict__ out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < nsplit; ++z) s += part[z * n + i];
+  out[i] = s;
+}
+
+// col[p][tap * C + c] = X[n][y + dy][x + dx][c] (zero outside), tap = 3 (dy + 1) + dx + 1
+__global__ void k_im2col3(const bfbits* __restrict__ X, long long B, int H, int W, int C,
+                          bfbits* __restrict__ col) {
+  const int cv = C / 8;
+  const long long total = B * H * W * 9LL * cv;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c8 = (int)(i % cv);
+  const int tap = (int)((i / cv) % 9);
+  const long long p = i / (9LL * cv);
+  const int x = (int)(p % W), y = (int)((p / W) % H);
+  const long long n = p / ((long long)W * H);
+  const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+  uint4 v = make_uint4(0, 0, 0, 0);
+  if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+    v = *reinterpret_cast<const uint4*>(X + ((n * H + yy) * W + xx) * C + c8 * 8);
+  *reinterpret_cast<uint4*>(col + p * 9LL * C + tap * (long long)C + c8 * 8) = v;
+}
+
+// dX[n][y][x][c] = sum_tap dcol[(n, y - dy, x - dx)][tap * C + c] (the adjoint
+// of k_im2col3), times (mask > 0) when mask is given; FP32 accumulation in tap
+// order, bf16 or FP32 output
+__global__ void k_col2im3(const float* __restrict__ dcol, long long B, int H, int W, int C,
+                          const bfbits* __restrict__ mask, int out_bf16, void* __restrict__ dX) {
+  const long long total = B * H * W * (long long)C;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % C);
+  const long long p = i / C;
+  const int x = (int)(p % W), y = (int)((p / W) % H);
+  const long long n = p / ((long long)W * H);
+  float s = 0.f;
+  for (int tap = 0; tap < 9; ++tap) {
+    const int ys = y - (tap / 3 - 1), xs = x - (tap % 3 - 1);
+    if (ys < 0 || ys >= H || xs < 0 || xs >= W) continue;
+    s += dcol[((n * H + ys) * W + xs) * 9LL * C + tap * (long long)C + c];
+  }
+  if (mask && !(bf2f(mask[i]) > 0.f)) s = 0.f;
+  if (out_bf16) ((bfbits*)dX)[i] = f2bf(s);
+  else ((float*)dX)[i] = s;
+}
+
+// 2x2 average pooling and its adjoint (NHWC)
+__global__ void k_pool2(const bfbits* __restrict__ X, long long B, int H, int W, int C,
+                        bfbits* __restrict__ Y) {
+  const int Ho = H / 2, Wo = W / 2;
+  const long long total = B * Ho * Wo * (long long)C;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % C);
+  const long long p = i / C;
+  const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho);
+  const long long n = p / ((long long)Wo * Ho);
+  float s = 0.f;
+  for (int dy = 0; dy < 2; ++dy)
+    for (int dx = 0; dx < 2; ++dx)
+      s += bf2f(X[((n * H + 2 * y + dy) * W + 2 * x + dx) * C + c]);
+  Y[i] = f2bf(0.25f * s);
+}
+
+__global__ void k_pool2_bwd(const bfbits* __restrict__ dY, long long B, int H, int W, int C,
+                            bfbits* __restrict__ dX) {
+  const long long total = B * H * W * (long long)C;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % C);
+  const long long p = i / C;
+  const int x = (int)(p % W), y = (int)((p / W) % H);
+  const long long n = p / ((long long)W * H);
+  dX[i] = f2bf(0.25f * bf2f(dY[((n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]));
+}
+
+// nearest 2x upsampling and its adjoint
+__global__ void k_up2(const bfbits* __restrict__ X, long long B, int H, int W, int C,
+                      bfbits* __restrict__ Y) {
+  const long long total = B * 4LL * H * W * C;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % C);
+  const long long p = i / C;
+  const int x = (int)(p % (2 * W)), y = (int)((p / (2 * W)) % (2 * H));
+  const long long n = p / (4LL * W * H);
+  Y[i] = X[((n * H + y / 2) * W + x / 2) * C + c];
+}
+
+__global__ void k_up2_bwd(const bfbits* __restrict__ dY, long long B, int H, int W, int C,
+                          bfbits* __restrict__ dX) {
+  const long long total = B * H * W * (long long)C;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % C);
+  const long long p = i / C;
+  const int x = (int)(p % W), y = (int)((p / W) % H);
+  const long long n = p / ((long long)W * H);
+  float s = 0.f;
+  for (int dy = 0; dy < 2; ++dy)
+    for (int dx = 0; dx < 2; ++dx)
+      s += bf2f(dY[((n * 2 * H + 2 * y + dy) * 2 * W + 2 * x + dx) * C + c]);
+  dX[i] = f2bf(s);
+}
+
+// Z = X + Y (bf16, FP32 add)
+__global__ void k_add_bf16(const bfbits* __restrict__ X, const bfbits* __restrict__ Y, long long n,
+                           bfbits* __restrict__ Z) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) Z[i] = f2bf(bf2f(X[i]) + bf2f(Y[i]));
+}
+
+// dZ = dY where Yact > 0, else 0
+__global__ void k_relu_bwd(const bfbits* __restrict__ dY, const bfbits* __restrict__ Yact,
+                           long long n, bfbits* __restrict__ dZ) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) dZ[i] = bf2f(Yact[i]) > 0.f ? dY[i] : (bfbits)0;
+}
+
+// column sums of a [P][C] bf16 matrix in two fixed-order passes.  Pass 1:
+// block b sums rows [b * chunk, (b + 1) * chunk); its threads form G row
+// groups of C columns (coalesced along C), group sums added in group order.
+// Pass 2: one block per column adds the block partials by a fixed tree.
+__global__ __launch_bounds__(256) void k_colsum_part(const bfbits* __restrict__ X, long long P,
+                                                    int C, long long chunk,
+                                                    float* __restrict__ part) {
+  __shared__ float red[256];
+  const long long p0 = blockIdx.x * chunk;
+  const long long p1 = (p0 + chunk < P) ? p0 + chunk : P;
+  const int t = threadIdx.x;
+  for (int cb = 0; cb < C; cb += 256) {
+    const int cw = (C - cb < 256) ? C - cb : 256;   // columns in this pass
+    const int G = 256 / cw;
+    const int c = t % cw, r = t / cw;
+    float s = 0.f;
+    if (r < G)
+      for (long long p = p0 + r; p < p1; p += G) s += bf2f(X[p * C + cb + c]);
+    red[t] = s;
+    __syncthreads();
+    if (t < cw) {
+      float tot = 0.f;
+      for (int g = 0; g < G; ++g) tot += red[g * cw + t];
+      part[blockIdx.x * (long long)C + cb + t] = tot;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_colsum_final(const float* __restrict__ part, long long nb,
+                                                     int C, float* __restrict__ out) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (long long b = threadIdx.x; b < nb; b += 256) s += part[b * C + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+inline long long colsum_chunk(long long P) {
+  long long ch = (P + 1023) / 1024;   // about 1024 blocks
+  return ch < 64 ? 64 : ch;
+}
+
+// squared-error loss on channel 0 of pred [P][cs] against target [P]:
+// grad[p][c] = (c == 0) * 2 (pred - target) / P (bf16); per-block partial sums of
+// the squared error (fixed order) to part[block]
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ pred, int cs,
+                                            const float* __restrict__ target, long long P,
+                                            bfbits* __restrict__ grad, int gs,
+                                            float* __restrict__ part) {
+  __shared__ float red[256];
+  const long long p = blockIdx.x * 256LL + threadIdx.x;
+  float e2 = 0.f;
+  if (p < P) {
+    const float d = pred[p * cs] - target[p];
+    e2 = d * d;
+    for (int c = 0; c < gs; ++c) grad[p * gs + c] = c == 0 ? f2bf(2.f * d / (float)P) : (bfbits)0;
+  }
+  red[threadIdx.x] = e2;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma and Ba 2015) on FP32 master weights, bf16 copy refreshed
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, bfbits* __restrict__ wbf) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wbf) wbf[i] = f2bf(wi);
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ x, long long n, float scale,
+                              bfbits* __restrict__ y) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(scale * x[i]);
+}
+
+__global__ void k_bf16_to_f32(const bfbits* __restrict__ x, long long n, float scale,
+                              float* __restrict__ y) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = scale * bf2f(x[i]);
+}
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+}  // namespace
+
+extern "C" size_t bh_nn_gemm_workspace_size(int64_t M, int64_t N, int64_t K, int nsplit) {
+  if (nsplit <= 1 || M <= 0 || N <= 0 || K <= 0) return 0;
+  return (size_t)nsplit * (size_t)M * (size_t)N * sizeof(float);
+}
+
+extern "C" int bh_nn_gemm(const void* A, int ta, int64_t lda, const void* B, int tb, int64_t ldb,
+                          int64_t M, int64_t N, int64_t K, float alpha, const float* bias,
+                          int relu, void* C, int out_bf16, int64_t ldc, int nsplit, void* ws,
+                          size_t ws_bytes, bh_stream s) {
+  bh::clear_error();
+  if (M <= 0 || N <= 0) return BH_OK;
+  if (!A || !B || !C) BH_FAIL(BH_ERR_ARG, "bh_nn_gemm: null pointer");
+  if (K <= 0) BH_FAIL(BH_ERR_ARG, "bh_nn_gemm: K=%lld", (long long)K);
+  const int64_t ca = ta ? M : K, cb = tb ? K : N;   // contiguous extents
+  if (ca % 8 || cb % 8 || lda % 8 || ldb % 8 || !aligned16(A) || !aligned16(B))
+    BH_FAIL(BH_ERR_ARG, "bh_nn_gemm: contiguous extents and leading dimensions must be "
+                        "multiples of 8 and the operands 16-byte aligned");
+  if (lda < ca || ldb < cb || ldc < N) BH_FAIL(BH_ERR_ARG, "bh_nn_gemm: leading dimension");
+  if (nsplit < 1) nsplit = 1;
+  if (nsplit > 1 && (out_bf16 || bias || relu))
+    BH_FAIL(BH_ERR_ARG, "bh_nn_gemm: split-K gives FP32 output without epilogue");
+  long long kchunk = ((K + nsplit - 1) / nsplit + BK - 1) / BK * BK;
+  nsplit = (int)((K + kchunk - 1) / kchunk);
+  GemmArgs g;
+  g.A = (const bfbits*)A;
+  g.B = (const bfbits*)B;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.lda = lda;
+  g.ldb = ldb;
+  g.ldc = ldc;
+  g.ta = ta ? 1 : 0;
+  g.tb = tb ? 1 : 0;
+  g.out_bf16 = out_bf16 ? 1 : 0;
+  g.bias = bias;
+  g.relu = relu ? 1 : 0;
+  g.alpha = alpha;
+  g.kchunk = kchunk;
+  hipStream_t st = (hipStream_t)s;
+  const dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM), (unsigned)nsplit);
+  if (nsplit == 1) {
+    g.C = C;
+    g.split_stride = 0;
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, g);
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+  }
+  if (ldc != N) BH_FAIL(BH_ERR_ARG, "bh_nn_gemm: split-K needs ldc == N");
+  const size_t need = (size_t)nsplit * (size_t)M * (size_t)N * sizeof(float);
+  if (!ws || ws_bytes < need)
+    BH_FAIL(BH_ERR_ARG, "bh_nn_gemm: workspace of %zu bytes needed", need);
+  g.C = ws;
+  g.split_stride = M * N;
+  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, g);
+  BH_LAUNCH_CHECK();
+  const long long n = M * N;
+  hipLaunchKernelGGL(k_sum_partials, dim3(bh::grid1d(n, 256)), dim3(256), 0, st,
+                     (const float*)ws, nsplit, n, (float*)C);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+static int check_img(const char* name, int64_t B, int H, int W, int C) {
+  if (B <= 0 || H <= 0 || W <= 0 || C <= 0)
+    BH_FAIL(BH_ERR_ARG, "%s: shape (%lld, %d, %d, %d)", name, (long long)B, H, W, C);
+  return BH_OK;
+}
+
+extern "C" int bh_nn_im2col3(const void* X, int64_t B, int H, int W, int C, void* col,
+                             bh_stream s) {
+  bh::clear_error();
+  if (int rc = check_img("bh_nn_im2col3", B, H, W, C)) return rc;
+  if (!X || !col) BH_FAIL(BH_ERR_ARG, "bh_nn_im2col3: null pointer");
+  if (C % 8 || !aligned16(X) || !aligned16(col))
+    BH_FAIL(BH_ERR_ARG, "bh_nn_im2col3: C=%d must be a multiple of 8, buffers 16-byte aligned", C);
+  const long long n = B * (long long)H * W * 9LL * (C / 8);
+  hipLaunchKernelGGL(k_im2col3, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
+                     (const bfbits*)X, (long long)B, H, W, C, (bfbits*)col);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_nn_col2im3(const float* dcol, int64_t B, int H, int W, int C, const void* mask,
+                             void* dX, int out_bf16, bh_stream s) {
+  bh::clear_error();
+  if (int rc = check_img("bh_nn_col2im3", B, H, W, C)) return rc;
+  if (!dcol || !dX) BH_FAIL(BH_ERR_ARG, "bh_nn_col2im3: null pointer");
+  const long long n = B * (long long)H * W * C;
+  hipLaunchKernelGGL(k_col2im3, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s, dcol,
+                     (long long)B, H, W, C, (const bfbits*)mask, out_bf16 ? 1 : 0, dX);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_nn_pool2(const void* X, int64_t B, int H, int W, int C, void* Y, int backward,
+                           bh_stream s) {
+  bh::clear_error();
+  if (int rc = check_img("bh_nn_pool2", B, H, W, C)) return rc;
+  if (!X || !Y) BH_FAIL(BH_ERR_ARG, "bh_nn_pool2: null pointer");
+  if (H % 2 || W % 2) BH_FAIL(BH_ERR_ARG, "bh_nn_pool2: H and W must be even");
+  hipStream_t st = (hipStream_t)s;
+  if (!backward) {
+    const long long n = B * (long long)(H / 2) * (W / 2) * C;
+    hipLaunchKernelGGL(k_pool2, dim3(bh::grid1d(n, 256)), dim3(256), 0, st, (const bfbits*)X,
+                       (long long)B, H, W, C, (bfbits*)Y);
+  } else {
+    // X = dY (B, H/2, W/2, C), Y = dX (B, H, W, C)
+    const long long n = B * (long long)H * W * C;
+    hipLaunchKernelGGL(k_pool2_bwd, dim3(bh::grid1d(n, 256)), dim3(256), 0, st,
+                       (const bfbits*)X, (long long)B, H, W, C, (bfbits*)Y);
+  }
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_nn_up2(const void* X, int64_t B, int H, int W, int C, void* Y, int backward,
+                         bh_stream s) {
+  bh::clear_error();
+  if (int rc = check_img("bh_nn_up2", B, H, W, C)) return rc;
+  if (!X || !Y) BH_FAIL(BH_ERR_ARG, "bh_nn_up2: null pointer");
+  hipStream_t st = (hipStream_t)s;
+  if (!backward) {
+    // X (B, H, W, C) -> Y (B, 2H, 2W, C)
+    const long long n = B * 4LL * H * W * C;
+    hipLaunchKernelGGL(k_up2, dim3(bh::grid1d(n, 256)), dim3(256), 0, st, (const bfbits*)X,
+                       (long long)B, H, W, C, (bfbits*)Y);
+  } else {
+    // X = dY (B, 2H, 2W, C) -> Y = dX (B, H, W, C)
+    const long long n = B * (long long)H * W * C;
+    hipLaunchKernelGGL(k_up2_bwd, dim3(bh::grid1d(n, 256)), dim3(256), 0, st, (const bfbits*)X,
+                       (long long)B, H, W, C, (bfbits*)Y);
+  }
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_nn_add(const void* X, const void* Y, int64_t n, void* Z, bh_stream s) {
+  bh::clear_error();
+  if (n <= 0) return BH_OK;
+  if (!X || !Y || !Z) BH_FAIL(BH_ERR_ARG, "bh_nn_add: null pointer");
+  hipLaunchKernelGGL(k_add_bf16, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
+                     (const bfbits*)X, (const bfbits*)Y, (long long)n, (bfbits*)Z);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_nn_relu_bwd(const void* dY, const void* Yact, int64_t n, void* dZ, bh_stream s) {
+  bh::clear_error();
+  if (n <= 0) return BH_OK;
+  if (!dY || !Yact || !dZ) BH_FAIL(BH_ERR_ARG, "bh_nn_relu_bwd: null pointer");
+  hipLaunchKernelGGL(k_relu_bwd, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
+                     (const bfbits*)dY, (const bfbits*)Yact, (long long)n, (bfbits*)dZ);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" size_t bh_nn_colsum_workspace_size(int64_t P, int C) {
+  if (P <= 0 || C <= 0) return 0;
+  const long long ch = colsum_chunk(P);
+  return (size_t)((P + ch - 1) / ch) * (size_t)C * sizeof(float);
+}
+
+extern "C" int bh_nn_colsum(const void* X, int64_t P, int C, float* out, void* ws, size_t ws_bytes,
+                            bh_stream s) {
+  bh::clear_error();
+  if (C <= 0) return BH_OK;
+  if (!X || !out || P < 0) BH_FAIL(BH_ERR_ARG, "bh_nn_colsum: bad arguments");
+  hipStream_t st = (hipStream_t)s;
+  const long long ch = colsum_chunk(P);
+  const long long nb = (P + ch - 1) / ch;
+  const size_t need = (size_t)nb * (size_t)C * sizeof(float);
+  if (nb > 0 && (!ws || ws_bytes < need))
+    BH_FAIL(BH_ERR_ARG, "bh_nn_colsum: workspace of %zu bytes needed", need);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)nb), dim3(256), 0, st, (const bfbits*)X,
+                       (long long)P, C, ch, (float*)ws);
+  hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)C), dim3(256), 0, st, (const float*)ws, nb, C,
+                     out);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" size_t bh_nn_mse_workspace_size(int64_t P) {
+  return P <= 0 ? 0 : (size_t)((P + 255) / 256) * sizeof(float);
+}
+
+extern "C" int bh_nn_mse(const float* pred, int pred_stride, const float* target, int64_t P,
+                         void* grad, int grad_stride, float* part, size_t part_bytes,
+                         bh_stream s) {
+  bh::clear_error();
+  if (P <= 0) return BH_OK;
+  if (!pred || !target || !grad || !part || pred_stride < 1 || grad_stride < 1)
+    BH_FAIL(BH_ERR_ARG, "bh_nn_mse: bad arguments");
+  const long long nb = (P + 255) / 256;
+  if (part_bytes < (size_t)nb * sizeof(float))
+    BH_FAIL(BH_ERR_ARG, "bh_nn_mse: %lld partials need %zu bytes", nb, (size_t)nb * sizeof(float));
+  hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)s, pred, pred_stride,
+                     target, (long long)P, (bfbits*)grad, grad_stride, part);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr,
+                          float beta1, float beta2, float eps, int64_t step, void* w_bf16,
+                          bh_stream s) {
+  bh::clear_error();
+  if (n <= 0) return BH_OK;
+  if (!w || !g || !m || !v || step < 1) BH_FAIL(BH_ERR_ARG, "bh_nn_adam: bad arguments");
+  const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s, w, g, m, v,
+                     (long long)n, lr, beta1, beta2, eps, c1, c2, (bfbits*)w_bf16);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_nn_cast(const void* x, int from_bf16, int64_t n, float scale, void* y,
+                          bh_stream s) {
+  bh::clear_error();
+  if (n <= 0) return BH_OK;
+  if (!x || !y) BH_FAIL(BH_ERR_ARG, "bh_nn_cast: null pointer");
+  hipStream_t st = (hipStream_t)s;
+  if (from_bf16)
+    hipLaunchKernelGGL(k_bf16_to_f32, dim3(bh::grid1d(n, 256)), dim3(256), 0, st,
+                       (const bfbits*)x, (long long)n, scale, (float*)y);
+  else
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(bh::grid1d(n, 256)), dim3(256), 0, st,
+                       (const float*)x, (long long)n, scale, (bfbits*)y);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}

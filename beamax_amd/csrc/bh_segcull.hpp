@@ -90,6 +90,104 @@ __device__ inline void seg_interval(const SegDev& G, long long k, const double* 
   hi = fhi > (double)(n - 1) ? n - 1 : (fhi < 0.0 ? -1 : (int)fhi);
 }
 
+// LDS writes of a wave visible to all of its lanes (no workgroup barrier)
+__device__ inline void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One (record, 64-point group) pair of the FP32-split sums in closed form
+// along the group: lane l evaluates the point pref + l v, where
+//   cycles(l)   = A + l (B + l C)                          (FP64, exact algebra)
+//   exponent(l) = Ce (l - l0)^2 + emin   (log2 units; = delta^T Qi delta)
+// The envelope is expanded about its minimum along the line, so the FP32
+// evaluation adds two non-negative terms (no cancellation).  Stored negated
+// (the argument of exp2), with log2(2|A|) folded into the constant:
+//   term(l) = cos(2 pi cycles(l)) * 2^(nCe (l - l0)^2 + nlog).
+struct PairQ {
+  double A, B, C;
+  float l0, nCe, nlog, pad;
+};
+
+template <int D>
+__device__ inline int sym_off(int a, int b) {   // packed index of (a < b)
+  return D + (a * (2 * D - a - 1)) / 2 + (b - a - 1);
+}
+
+template <int D>
+__device__ inline PairQ pair_quad(const RecS<D>& R, const double* pref, const double* v) {
+  double d[D], Qd[D], Qv[D], Ed[D], Ev[D];
+  for (int a = 0; a < D; ++a) d[a] = pref[a] - R.x[a];
+  for (int a = 0; a < D; ++a) {
+    double qd = 0.0, qv = 0.0, ed = 0.0, ev = 0.0;
+    for (int b = 0; b < D; ++b) {
+      const double qr = a == b ? R.qr[a] : 0.5 * R.qr[a < b ? sym_off<D>(a, b) : sym_off<D>(b, a)];
+      const double qi =
+          a == b ? (double)R.qi[a] : 0.5 * (double)R.qi[a < b ? sym_off<D>(a, b) : sym_off<D>(b, a)];
+      qd += qr * d[b];
+      qv += qr * v[b];
+      ed += qi * d[b];
+      ev += qi * v[b];
+    }
+    Qd[a] = qd;
+    Qv[a] = qv;
+    Ed[a] = ed;
+    Ev[a] = ev;
+  }
+  PairQ c;
+  double A = R.ph0, B = 0.0, C = 0.0, Ae = 0.0, Be = 0.0, Ce = 0.0;
+  for (int a = 0; a < D; ++a) {
+    A += d[a] * (R.k[a] + Qd[a]);
+    B += v[a] * (R.k[a] + 2.0 * Qd[a]);
+    C += v[a] * Qv[a];
+    Ae += d[a] * Ed[a];
+    Be += 2.0 * v[a] * Ed[a];
+    Ce += v[a] * Ev[a];
+  }
+  c.A = A;
+  c.B = B;
+  c.C = C;
+  double emin = Ae, l0 = 0.0;
+  if (Ce > 1e-300) {
+    // l0 = -Be / (2 Ce): hardware reciprocal + one Newton step (l0 only
+    // places the expansion point; emin is then consistent with it)
+    const double x2 = 2.0 * Ce;
+    double r = __builtin_amdgcn_rcp(x2);
+    r = r * fma(-x2, r, 2.0);
+    l0 = -Be * r;
+    emin = fma(Be, l0, fma(Ce * l0, l0, Ae));   // exponent at l0
+  }
+  c.l0 = (float)l0;
+  c.nCe = (float)(-Ce);
+  c.nlog = (float)(-emin) + __builtin_amdgcn_logf(R.amp);
+  c.pad = 0.f;
+  return c;
+}
+
+// the same pair at lanes l (pa) and l + 64 (pb): two adjacent groups of a run
+__device__ inline void eval_quad2(const PairQ& c, double ld, float lf, float& pa, float& pb) {
+  const double cyc0 = fma(ld, fma(ld, c.C, c.B), c.A);
+  const double ld1 = ld + 64.0;
+  const double cyc1 = fma(ld1, fma(ld1, c.C, c.B), c.A);
+  const float fr0 = (float)__builtin_amdgcn_fract(cyc0);
+  const float fr1 = (float)__builtin_amdgcn_fract(cyc1);
+  const float t0 = lf - c.l0;
+  const float t1 = t0 + 64.f;
+  const float a0 = fmaf(t0 * c.nCe, t0, c.nlog);
+  const float a1 = fmaf(t1 * c.nCe, t1, c.nlog);
+  pa = fmaf(__builtin_amdgcn_cosf(fr0), __builtin_amdgcn_exp2f(a0), pa);
+  pb = fmaf(__builtin_amdgcn_cosf(fr1), __builtin_amdgcn_exp2f(a1), pb);
+}
+
+__device__ inline void eval_quad(const PairQ& c, double ld, float lf, float& part) {
+  const double cyc = fma(ld, fma(ld, c.C, c.B), c.A);
+  const float fr = (float)__builtin_amdgcn_fract(cyc);
+  const float t = lf - c.l0;
+  const float arg = fmaf(t * c.nCe, t, c.nlog);
+  part = fmaf(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_exp2f(arg), part);
+}
+
 int check_segments(const bh_segments* g, int ndim) {
   if (!g) BH_FAIL(BH_ERR_ARG, "segments: null");
   if (g->nseg <= 0 || g->ngroups <= 0 || g->S <= 0)

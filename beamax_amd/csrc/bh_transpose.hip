@@ -39,12 +39,6 @@ struct TArgs {
   long long ntg;
 };
 
-__device__ inline void wave_lds_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <int D, bool F64>
 __global__ __launch_bounds__(256) void k_fwd_transpose(TArgs A) {
   using Rec = typename RecOf<D, F64>::type;

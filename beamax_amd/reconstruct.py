@@ -41,13 +41,22 @@ def cgnr(op, data, *, x0=None, tol: float = 1e-3, maxiter: int = 500, precond=No
         x = torch.zeros(op.shape[1], dtype=torch.float64, device=d.device)
         r = d.clone()
     else:
-        x = x0.to(torch.float64).clone()
+        x = x0.to(torch.float64).reshape(op.shape[1]).clone()
         r = d - op.apply(x).to(torch.float64)
     g = op.transpose(r)                    # F^T r
-    g0 = np.sqrt(_dot(g, g))
-    hist = [1.0]
+    if x0 is None:
+        g0 = np.sqrt(_dot(g, g))
+    else:
+        # residuals stay relative to ||F^T d||, as from a zero start
+        gd = op.transpose(d)
+        g0 = np.sqrt(_dot(gd, gd))
+    hist = [np.sqrt(_dot(g, g)) / g0 if g0 > 0 else 0.0]
     if g0 == 0.0:
         return x, {"iterations": 0, "residuals": hist, "seconds": 0.0, "converged": True}
+    if hist[0] <= tol:
+        _sync(x)
+        return x, {"iterations": 0, "residuals": hist, "seconds": time.perf_counter() - t0,
+                   "converged": True}
     z = precond(g) if precond is not None else g
     p = z.clone()
     gz = _dot(g, z)

@@ -6,7 +6,9 @@ the points (sensors or image pixels) are grouped here once per geometry:
 points on the grid are split greedily into maximal runs along the last axis
 (length >= ``min_run``), then along the earlier axes, and whatever is left
 into runs of any length.  Each run is packed into groups of 64 points (one
-per lane); padding slots repeat a valid point and write nowhere.
+per lane); padding slots repeat a valid point and write nowhere.  Every run
+holds an even number of groups (an all-padding group completes an odd
+count), so the kernels can pair adjacent groups of one run.
 """
 
 from __future__ import annotations
@@ -42,10 +44,14 @@ class Segments:
             seg_n.append(n)
             seg_g0.append(g)
             ng = (n + 63) // 64
+            ng += ng % 2
             for q in range(ng):
                 m = members[q * 64:(q + 1) * 64]
-                slot_pts = np.repeat(pos[m[-1]][None], 64, axis=0)
-                slot_pts[:len(m)] = pos[m]
+                if len(m):
+                    slot_pts = np.repeat(pos[m[-1]][None], 64, axis=0)
+                    slot_pts[:len(m)] = pos[m]
+                else:   # padding group: the run continued past its end
+                    slot_pts = pos[members[0]][None] + (q * 64 + np.arange(64))[:, None] * v
                 slot_col = np.full(64, -1, dtype=np.int32)
                 slot_col[:len(m)] = m
                 pts.append(slot_pts)

@@ -22,6 +22,7 @@ import json
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -266,6 +267,29 @@ def recon_leg(device, tol=1e-3, seed=0):
     F.transpose(w)
     ev[2].record()
     torch.cuda.synchronize()
+    learned = None
+    wpath = Path(__file__).resolve().parent / "beamax_amd" / "weights" / "unet_recon256.safetensors"
+    if wpath.exists():
+        from beamax_amd.learned import UNet, warm_start
+        net = UNet.load(wpath, device=device)
+        warm_start(net, F, d, img)                       # warm-up (kernels, workspaces)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x0 = warm_start(net, F, d, img)
+        torch.cuda.synchronize()
+        t_init = time.perf_counter() - t1
+        x1, info1 = cgnr(F, d, x0=x0, tol=tol, maxiter=1000)
+        learned = {
+            "method": "CGNR from the U-Net estimate of p0 (adjoint image in, least-squares "
+                      "scaled), trained on phantom seeds 1000-5095",
+            "seconds": round(info1["seconds"] + t_init, 4),
+            "network_s": round(t_init, 4),
+            "iterations": info1["iterations"],
+            "converged": info1["converged"],
+            "start_residual": info1["residuals"][0],
+            "rel_err_vs_phantom": float(((x1 - p0).norm() / p0.norm()).item()),
+            "time_ratio_vs_plain": round((info1["seconds"] + t_init) / info["seconds"], 3),
+        }
     return {
         "seconds": round(info["seconds"], 4),
         "iterations": info["iterations"],
@@ -280,6 +304,7 @@ def recon_leg(device, tol=1e-3, seed=0):
         "workload": ("2-D 256^2 dx 1e-4, 3 levels (4,8,16) rho 2, four-edge 1020 sensors, "
                      f"{ts.size} steps, top_n 32768 frozen from the adjoint image; phantom "
                      "seed 0 + 1% noise"),
+        "learned": learned,
     }
 
 

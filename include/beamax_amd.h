@@ -412,6 +412,58 @@ int bh_mswpt_adjoint(const bh_tiling* t, const double* sum_gsquare, const double
                      const double* coeffs, int output_real, double* out, void* ws,
                      size_t ws_bytes, bh_stream s);
 
+/* ---- learned reconstruction network (no reference counterpart: the
+ * reference has no reconstruction solver; these serve the U-Net that gives
+ * CG on the normal equations its starting image).  Tensors are NHWC bf16
+ * (uint16 bit patterns) unless stated; device pointers, caller-allocated. */
+
+/* C[M][N] = alpha * sum_k A(m,k) B(k,n) (+ bias[n]) (ReLU if relu) with bf16
+ * MFMA and FP32 accumulation.  A stored [M][lda] (ta = 0) or [K][lda]
+ * (ta = 1); B stored [K][ldb] (tb = 0) or [N][ldb] (tb = 1); the contiguous
+ * extent and leading dimension of each operand multiples of 8, operands
+ * 16-byte aligned.  C is bf16 (out_bf16) or FP32 with leading dimension ldc.
+ * nsplit > 1 splits K (FP32 output, no bias/ReLU, ldc = N) through a
+ * workspace of bh_nn_gemm_workspace_size bytes, partials added in order. */
+size_t bh_nn_gemm_workspace_size(int64_t M, int64_t N, int64_t K, int nsplit);
+int bh_nn_gemm(const void* A, int ta, int64_t lda, const void* B, int tb, int64_t ldb, int64_t M,
+               int64_t N, int64_t K, float alpha, const float* bias, int relu, void* C,
+               int out_bf16, int64_t ldc, int nsplit, void* ws, size_t ws_bytes, bh_stream s);
+/* im2col of a 3x3 same-padded convolution: col[p][tap*C + c], tap = 3(dy+1) + dx+1,
+ * zero outside the image; C a multiple of 8. */
+int bh_nn_im2col3(const void* X, int64_t B, int H, int W, int C, void* col, bh_stream s);
+/* Its adjoint: dX = sum over taps of the FP32 column gradient dcol, zeroed
+ * where mask (bf16, optional) is not positive; bf16 or FP32 output. */
+int bh_nn_col2im3(const float* dcol, int64_t B, int H, int W, int C, const void* mask, void* dX,
+                  int out_bf16, bh_stream s);
+/* 2x2 average pooling of X (B, H, W, C) into Y (B, H/2, W/2, C); backward:
+ * X = dY (B, H/2, W/2, C) into Y = dX (B, H, W, C). */
+int bh_nn_pool2(const void* X, int64_t B, int H, int W, int C, void* Y, int backward, bh_stream s);
+/* nearest 2x upsampling of X (B, H, W, C) into Y (B, 2H, 2W, C); backward:
+ * X = dY (B, 2H, 2W, C) summed into Y = dX (B, H, W, C). */
+int bh_nn_up2(const void* X, int64_t B, int H, int W, int C, void* Y, int backward, bh_stream s);
+/* Z = X + Y (bf16, n elements). */
+int bh_nn_add(const void* X, const void* Y, int64_t n, void* Z, bh_stream s);
+/* dZ = dY where Yact > 0, else 0 (ReLU backward). */
+int bh_nn_relu_bwd(const void* dY, const void* Yact, int64_t n, void* dZ, bh_stream s);
+/* out[c] = sum_p X[p][c] (FP32, fixed order: row-chunk partials, then a
+ * fixed tree): bias gradients.  ws: bh_nn_colsum_workspace_size bytes. */
+size_t bh_nn_colsum_workspace_size(int64_t P, int C);
+int bh_nn_colsum(const void* X, int64_t P, int C, float* out, void* ws, size_t ws_bytes,
+                 bh_stream s);
+/* Squared error of channel 0 of pred (FP32, stride pred_stride) against
+ * target (P): grad (bf16, stride grad_stride, channel 0 = 2(pred - target)/P,
+ * other channels 0) and per-256-point partial sums of the squared error. */
+size_t bh_nn_mse_workspace_size(int64_t P);
+int bh_nn_mse(const float* pred, int pred_stride, const float* target, int64_t P, void* grad,
+              int grad_stride, float* part, size_t part_bytes, bh_stream s);
+/* One Adam step (Kingma and Ba 2015, bias-corrected, step >= 1) on FP32
+ * weights w with gradient g and moments m, v; w_bf16 (optional) receives the
+ * bf16 copy used by the GEMMs. */
+int bh_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+               float beta2, float eps, int64_t step, void* w_bf16, bh_stream s);
+/* y = scale * x, FP32 -> bf16 (from_bf16 = 0) or bf16 -> FP32 (1). */
+int bh_nn_cast(const void* x, int from_bf16, int64_t n, float scale, void* y, bh_stream s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,19 +61,20 @@ def test_batched_graph_replay_and_tiles(gpu):
             assert torch.equal(a, b), rep
         del junk
     # the graph reads the inputs it captured: doubling every p0 doubles the
-    # data exactly (same selection, amplitudes scaled by a power of two)
+    # data (same selection, amplitudes scaled by two; the FP32 terms carry
+    # log2 of the amplitude in their exponent, so to rounding, not bit for bit)
     for p in probs:
         p.p0.mul_(2.0)
     out = bf.replay()
     torch.cuda.synchronize()
     for a, b in zip(eager, out):
-        assert torch.equal(2.0 * a, b)
+        assert rel(_n(2.0 * a).astype(np.float64), _n(b).astype(np.float64)) < 1e-6
     for p in probs:
         p.p0.mul_(0.5)
     # every tile shape sums each point in the same order: identical results
-    best, times = bf.tune(tiles=(4, 8, 16, 32), reps=1)
-    assert best in (4, 8, 16, 32) and len(times) == 4
-    for gpb in (4, 32):
+    best, times = bf.tune(reps=1)
+    assert best in (8, 16, 32) and len(times) == 3
+    for gpb in (8, 32):
         bf.set_tile(gpb)
         out = bf.run()
         torch.cuda.synchronize()
