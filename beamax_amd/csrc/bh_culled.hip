@@ -292,8 +292,13 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
   }
 }
 
+// The FP32-split form runs at four waves per SIMD (its 128-VGPR budget costs
+// a few spilled registers outside the evaluation loop; the extra wave hides
+// more latency than the spills cost); the FP64 form keeps the compiler's
+// choice.
 template <int D, int SPT, bool F64, bool STATIC>
-__global__ __launch_bounds__(256) void k_culled_sum(CulledArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F64 ? 1 : 4)))
+void k_culled_sum(CulledArgs A) {
   culled_body<D, SPT, F64, STATIC>(A, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -305,7 +310,8 @@ struct BatchDesc {
 };
 
 template <int D, int SPT, bool F64>
-__global__ __launch_bounds__(256) void k_culled_batch(const BatchDesc* __restrict__ d,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F64 ? 1 : 4)))
+void k_culled_batch(const BatchDesc* __restrict__ d,
                                                       const long long* __restrict__ start,
                                                       int nsolve) {
   const long long b = blockIdx.x;
