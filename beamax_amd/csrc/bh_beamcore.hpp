@@ -122,7 +122,11 @@ __device__ inline void make_rec(const State<D>& S, double w, RecS<D>& R) {
       ++q;
     }
   R.amp = (float)(2.0 * cabs_(S.A));
-  R.ph0 = safe_angle(S.A) / bh::kTwoPi;
+  // arg A in FP32 (a few 1e-8 cycles, below the FP32 rounding of the reduced
+  // phase that the cosine takes); (0, 0) -> 0 as safe_angle
+  const float are = (float)S.A.re, aim = (float)S.A.im;
+  R.ph0 = (double)(atan2f(aim, (are == 0.f && aim == 0.f) ? 1e-12f : are) *
+                   (float)(1.0 / bh::kTwoPi));
 }
 
 template <int D>
