@@ -37,11 +37,11 @@ FLOP_PER_EVAL_3D = 43.0
 METRIC_3D = "Gbeam-evals/s MSGB fwd+adj (128^3 3-D, err<1e-5 vs FP64)"
 FP32_VALU_PEAK_TFLOPS = 157.3
 # HBM bytes per forward launch of the culled FP32-split kernel on this
-# workload at cull_log2 = 40: rocprofv3 FETCH_SIZE (244467.1 KiB raw, x2.0)
-# + WRITE_SIZE (77707.3 KiB raw, x1.0), both scaled by what the counters read
+# workload at cull_log2 = 40: rocprofv3 FETCH_SIZE (399233.7 KiB raw, x2.0)
+# + WRITE_SIZE (77705.0 KiB raw, x1.0), both scaled by what the counters read
 # for a 512 MiB copy (profiles/k_culled_sum_msgb2d_512.md).  Counters cannot
 # be read from inside the process, so other settings report null.
-MEASURED_FWD_TRAFFIC = {40.0: 244467.125 * 2.0 * 1024 + 77707.34375 * 1.0 * 1024}
+MEASURED_FWD_TRAFFIC = {40.0: 399233.6875 * 2.0 * 1024 + 77704.96875 * 1.0 * 1024}
 
 
 def build_problem(device, cull_log2=40.0):
@@ -491,9 +491,13 @@ def main():
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "flop_per_eval": flop,
             "evals_per_launch": fwd_exec,
-            "note": ("achieved counts executed lane evaluations of the launch (64 per "
-                     "evaluated beam x 64-point group pair); dense_equivalent_tflops counts "
-                     "every (beam, time, sensor) term"),
+            "note": ("achieved = executed lane evaluations of the launch (64 per "
+                     "evaluated beam x 64-point group pair; the FP32-split kernel evaluates "
+                     "two-group units, so a unit counts 128) x the direct formula's "
+                     "flop_per_eval, over the launch time; the kernel evaluates the same terms "
+                     "by a per-pair quadratic in the lane index, fewer operations than "
+                     "flop_per_eval; dense_equivalent_tflops counts every (beam, time, "
+                     "sensor) term"),
             "launch_ms": round(float(fwd_ms), 3),
         },
     }
