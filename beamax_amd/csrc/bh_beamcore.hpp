@@ -105,9 +105,9 @@ template <int D>
 __device__ inline void make_rec(const State<D>& S, double w, RecS<D>& R) {
   for (int a = 0; a < D; ++a) {
     R.x[a] = S.x[a];
-    R.k[a] = w * S.p[a] / bh::kTwoPi;
+    R.k[a] = w * S.p[a] * (1.0 / bh::kTwoPi);
   }
-  const double sr = w / (2.0 * bh::kTwoPi);
+  const double sr = w * (1.0 / (2.0 * bh::kTwoPi));
   const double si = 0.5 * w * bh::kLog2e;
   int q = 0;
   for (int a = 0; a < D; ++a) {
@@ -121,7 +121,7 @@ __device__ inline void make_rec(const State<D>& S, double w, RecS<D>& R) {
       R.qi[q] = (float)(si * (S.M[a][b].im + S.M[b][a].im));
       ++q;
     }
-  R.amp = (float)(2.0 * cabs_(S.A));
+  R.amp = (float)(2.0 * sqrt(S.A.re * S.A.re + S.A.im * S.A.im));
   // arg A in FP32 (a few 1e-8 cycles, below the FP32 rounding of the reduced
   // phase that the cosine takes); (0, 0) -> 0 as safe_angle
   const float are = (float)S.A.re, aim = (float)S.A.im;
@@ -304,15 +304,39 @@ namespace {
 //   M = diag(w) + kappa q q^T
 // (identical to diag(alpha) (A + u v^T)^-1 with A = I + s diag(alpha),
 // u = -s p / |p|^2, v = p o alpha), and the reference's amplitude formulas.
+// 1/x from the hardware reciprocal and two Newton steps (about 1 ulp; the
+// per-record closed forms below use it in place of IEEE division)
+__device__ inline double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * fma(-x, r, 2.0);
+  return r * fma(-x, r, 2.0);
+}
+
+// a / sqrt(z) (principal root) with the reciprocals of frcp: a conj(s) / |z|
+__device__ inline cd fdiv_csqrt(cd a, cd z) {
+  const double r = sqrt(z.re * z.re + z.im * z.im);
+  if (!(r > 0.0)) return a / csqrt_(z);
+  cd s;
+  if (z.re >= 0.0) {
+    const double t = sqrt(0.5 * (r + z.re));
+    s = mkc(t, z.im * frcp(2.0 * t));
+  } else {
+    const double t = sqrt(0.5 * (r - z.re));
+    s = mkc(fabs(z.im) * frcp(2.0 * t), copysign(t, z.im));
+  }
+  const double ir = frcp(r);
+  return mkc((a.re * s.re + a.im * s.im) * ir, (a.im * s.re - a.re * s.im) * ir);
+}
+
 template <int D>
 __device__ inline void hom_diag_state_fast(const double* x0, const double* p0, const cd* alpha,
                                            cd a0, double c0, double t, State<D>& S) {
   double np2 = 0.0;
   for (int a = 0; a < D; ++a) np2 += p0[a] * p0[a];
   const double np = sqrt(np2);
-  const double inp = 1.0 / np;
+  const double inp = frcp(np);
   const double s = (c0 * t) * inp;
-  const double sn = s / np2;
+  const double sn = s * (inp * inp);
   cd w[D], q[D];
   cd den = mkc(1.0);
   for (int a = 0; a < D; ++a) {
@@ -320,12 +344,12 @@ __device__ inline void hom_diag_state_fast(const double* x0, const double* p0, c
     S.p[a] = p0[a];
     // w = alpha / (1 + s alpha)
     const double br = 1.0 + s * alpha[a].re, bi = s * alpha[a].im;
-    const double r = 1.0 / (br * br + bi * bi);
+    const double r = frcp(br * br + bi * bi);
     w[a] = mkc((alpha[a].re * br + alpha[a].im * bi) * r, (alpha[a].im * br - alpha[a].re * bi) * r);
     q[a] = w[a] * p0[a];
     den = den - (sn * (p0[a] * p0[a])) * w[a];
   }
-  const double dr = 1.0 / (den.re * den.re + den.im * den.im);
+  const double dr = frcp(den.re * den.re + den.im * den.im);
   const cd kap = mkc(sn * den.re * dr, -sn * den.im * dr);
   for (int a = 0; a < D; ++a)
     for (int b = a; b < D; ++b) {
@@ -335,16 +359,17 @@ __device__ inline void hom_diag_state_fast(const double* x0, const double* p0, c
       S.M[b][a] = m;
     }
   if constexpr (D == 3) {
-    const double tsc = c0 * t / (np2 * np2);
+    const double i2 = inp * inp;
+    const double tsc = c0 * t * (i2 * i2);
     const cd a1 = alpha[0], a2 = alpha[1], a3 = alpha[2];
     const cd t1 = (c0 * t) * (a2 * a3) + (a2 + a3) * np;
     const cd t2 = (c0 * t) * (a1 * a3) + (a1 + a3) * np;
     const cd t3 = (c0 * t) * (a1 * a2) + (a1 + a2) * np;
     const cd dd = mkc(1.0) + tsc * ((p0[0] * p0[0]) * t1 + (p0[1] * p0[1]) * t2 + (p0[2] * p0[2]) * t3);
-    S.A = a0 / csqrt_(dd);
+    S.A = fdiv_csqrt(a0, dd);
   } else if constexpr (D == 2) {
-    const cd ap = ((p0[0] * p0[0]) * alpha[1] + (p0[1] * p0[1]) * alpha[0]) * (inp / np2);
-    S.A = a0 / csqrt_(mkc(1.0) + (c0 * t) * ap);
+    const cd ap = ((p0[0] * p0[0]) * alpha[1] + (p0[1] * p0[1]) * alpha[0]) * (inp * inp * inp);
+    S.A = fdiv_csqrt(a0, mkc(1.0) + (c0 * t) * ap);
   } else {
     S.A = a0;
   }

@@ -84,7 +84,8 @@ __device__ inline void seg_interval(const SegDev& G, long long k, const double* 
     return;
   }
   const double r = sqrt(disc);
-  const double ulo = (-Bq - r) / A, uhi = (-Bq + r) / A;
+  const double iA = frcp(A);
+  const double ulo = (-Bq - r) * iA, uhi = (-Bq + r) * iA;
   const double flo = floor(ulo) - 1.0, fhi = ceil(uhi) + 1.0;
   lo = flo < 0.0 ? 0 : (flo > (double)(n - 1) ? n : (int)flo);
   hi = fhi > (double)(n - 1) ? n - 1 : (fhi < 0.0 ? -1 : (int)fhi);
