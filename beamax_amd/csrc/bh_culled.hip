@@ -67,7 +67,10 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
   constexpr int UPW = SPT / U;   // units per wave (wave wv owns units wv + 4 i)
   __shared__ Rec rec[kRecChunk];
   __shared__ unsigned long long mask[NU][4];
-  __shared__ int gseg[GPB], gj0[GPB];
+  // the block's units grouped by run: per piece its run, first unit, number
+  // of units and the index of its first point within the run
+  __shared__ int4 runs[NU];
+  __shared__ int nruns;
   // FP32-split: per unit its first point and step; per wave the compacted
   // list of active (record, unit) pairs of the chunk and one pass of 64
   // pair coefficients
@@ -87,19 +90,31 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
   for (int i = 0; i < SPT; ++i) acc[i] = 0.0;
   const double ld = (double)lane;
   const float lf = (float)lane;
-  if (threadIdx.x < GPB && (int)threadIdx.x < ng) {
-    const long long g = gbase + threadIdx.x;
-    const int k = A.G.grp_seg[g];
-    gseg[threadIdx.x] = k;
-    gj0[threadIdx.x] = (int)((g - A.G.g0[k]) * 64);
-    if constexpr (!F64) {
-      if (threadIdx.x % U == 0) {
+  if (wv == 0) {
+    // lane u < nu: unit u; a piece starts where the run changes
+    int k = -1, kp = -1;
+    const bool in = lane < nu;
+    if (in) {
+      const long long g = gbase + (long long)lane * U;
+      k = A.G.grp_seg[g];
+      kp = lane > 0 ? A.G.grp_seg[g - U] : -1;
+      if constexpr (!F64) {
         for (int a = 0; a < D; ++a) {
-          gpt[threadIdx.x / U][a] = A.G.pts[g * 64 * D + a];
-          gstep[threadIdx.x / U][a] = A.G.v[(long long)k * D + a];
+          gpt[lane][a] = A.G.pts[g * 64 * D + a];
+          gstep[lane][a] = A.G.v[(long long)k * D + a];
         }
       }
     }
+    const bool first = in && k != kp;
+    const unsigned long long st = __ballot(first);
+    if (first) {
+      const unsigned long long above = lane < 63 ? st >> (lane + 1) : 0ull;
+      const int next = above ? lane + 1 + __builtin_ctzll(above) : nu;
+      const long long g = gbase + (long long)lane * U;
+      runs[__popcll(st & ((1ull << lane) - 1ull))] =
+          make_int4(k, lane, next - lane, (int)((g - A.G.g0[k]) * 64));
+    }
+    if (lane == 0) nruns = __popcll(st);
   }
   unsigned long long npairs = 0;
   const long long total = STATIC ? A.count : A.total;
@@ -146,26 +161,35 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
     } else {
       zero_rec<D>(R);
     }
-    // per-group activity masks by ballot (word = this wave's 64 records)
+    // per-unit activity masks by ballot (word = this wave's 64 records): one
+    // closed-form interval per run piece, then two compares per unit
     double Q[D][D];
-    if (live && A.E > 0.0) env_matrix<D>(R, Q);
-    long long cur_seg = -1;
-    int lo = 1, hi = 0;
-    for (int ul = 0; ul < nu; ++ul) {
-      const int gl = ul * U;
-      bool act = live;
-      if (live && A.E > 0.0) {
-        const int k = gseg[gl];
-        if (k != cur_seg) {
-          seg_interval<D>(A.G, k, R.x, Q, A.E, lo, hi);
-          cur_seg = k;
+    const bool cull = A.E > 0.0;
+    if (live && cull) env_matrix<D>(R, Q);
+    const int nr = nruns;
+    for (int r = 0; r < nr; ++r) {
+      const int4 ru = runs[r];
+      const int rk = __builtin_amdgcn_readfirstlane(ru.x);
+      const int ru0 = __builtin_amdgcn_readfirstlane(ru.y);
+      const int run = __builtin_amdgcn_readfirstlane(ru.z);
+      const int rj0 = __builtin_amdgcn_readfirstlane(ru.w);
+      int lo = 0x3fffffff, hi = -1;   // empty
+      if (live) {
+        if (cull) {
+          int l2, h2;
+          seg_interval<D>(A.G, rk, R.x, Q, A.E, l2, h2);
+          if (h2 >= l2) { lo = l2; hi = h2; }
+        } else {
+          lo = 0;
+          hi = 0x3fffffff;
         }
-        const int j0 = gj0[gl];
-        act = (hi >= lo) && (hi >= j0) && (lo <= j0 + 64 * U - 1);
       }
-      const unsigned long long m = __ballot(act);
-      if (lane == 0) mask[ul][wv] = m;
-      npairs += __popcll(m);
+      for (int i = 0; i < run; ++i) {
+        const int j0 = rj0 + i * 64 * U;
+        const unsigned long long m = __ballot(hi >= j0 && lo <= j0 + 64 * U - 1);
+        if (lane == 0) mask[ru0 + i][wv] = m;
+        npairs += __popcll(m);
+      }
     }
     __syncthreads();
     if constexpr (!F64) {

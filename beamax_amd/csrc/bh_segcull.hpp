@@ -181,14 +181,6 @@ __device__ inline void eval_quad2(const PairQ& c, double ld, float lf, float& pa
   pb = fmaf(__builtin_amdgcn_cosf(fr1), __builtin_amdgcn_exp2f(a1), pb);
 }
 
-__device__ inline void eval_quad(const PairQ& c, double ld, float lf, float& part) {
-  const double cyc = fma(ld, fma(ld, c.C, c.B), c.A);
-  const float fr = (float)__builtin_amdgcn_fract(cyc);
-  const float t = lf - c.l0;
-  const float arg = fmaf(t * c.nCe, t, c.nlog);
-  part = fmaf(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_exp2f(arg), part);
-}
-
 int check_segments(const bh_segments* g, int ndim) {
   if (!g) BH_FAIL(BH_ERR_ARG, "segments: null");
   if (g->nseg <= 0 || g->ngroups <= 0 || g->S <= 0)

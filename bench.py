@@ -36,6 +36,9 @@ FLOP_PER_EVAL_2D = 25.0
 FLOP_PER_EVAL_3D = 43.0
 METRIC_3D = "Gbeam-evals/s MSGB fwd+adj (128^3 3-D, err<1e-5 vs FP64)"
 FP32_VALU_PEAK_TFLOPS = 157.3
+# two transcendentals (cos, exp2) per evaluation at a quarter of the FP32
+# issue rate: 256 CUs x 64 lanes x 2.4 GHz / 4 / 2
+TRANS_BOUND_EVALS = 256 * 64 * 2.4e9 / 4 / 2
 # HBM bytes per forward launch of the culled FP32-split kernel on this
 # workload at cull_log2 = 40: rocprofv3 FETCH_SIZE (399233.7 KiB raw, x2.0)
 # + WRITE_SIZE (77705.0 KiB raw, x1.0), both scaled by what the counters read
@@ -499,6 +502,14 @@ def main():
                      "flop_per_eval; dense_equivalent_tflops counts every (beam, time, "
                      "sensor) term"),
             "launch_ms": round(float(fwd_ms), 3),
+            # the same launch against the transcendental issue bound: two
+            # (cos, exp2) per evaluation at quarter rate on 256 CUs x 64
+            # lanes x 2.4 GHz
+            "transcendental": {
+                "executed_evals_per_s": round(fwd_exec / (fwd_ms * 1e-3), 1),
+                "bound_evals_per_s": TRANS_BOUND_EVALS,
+                "frac": round(fwd_exec / (fwd_ms * 1e-3) / TRANS_BOUND_EVALS, 4),
+            },
         },
     }
     if not args.no_check:
