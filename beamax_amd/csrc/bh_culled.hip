@@ -236,8 +236,9 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
         }
         wave_lds_fence();
         for (int i = 0; i < UPW; ++i) {
-          const int lo = (gs[i] > b ? gs[i] : b) - b;
-          const int hi = (gs[i + 1] < b + 64 ? gs[i + 1] : b + 64) - b;
+          // wave-uniform bounds: a scalar loop
+          const int lo = __builtin_amdgcn_readfirstlane((gs[i] > b ? gs[i] : b) - b);
+          const int hi = __builtin_amdgcn_readfirstlane((gs[i + 1] < b + 64 ? gs[i + 1] : b + 64) - b);
           if (lo >= hi) continue;
           // two pairs per step, each at lanes l and l + 64
           int sl = lo;

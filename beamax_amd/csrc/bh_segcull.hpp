@@ -103,12 +103,13 @@ __device__ inline void wave_lds_fence() {
 //   cycles(l)   = A + l (B + l C)                          (FP64, exact algebra)
 //   exponent(l) = Ce (l - l0)^2 + emin   (log2 units; = delta^T Qi delta)
 // The envelope is expanded about its minimum along the line, so the FP32
-// evaluation adds two non-negative terms (no cancellation).  Stored negated
-// (the argument of exp2), with log2(2|A|) folded into the constant:
-//   term(l) = cos(2 pi cycles(l)) * 2^(nCe (l - l0)^2 + nlog).
+// evaluation adds two non-negative terms (no cancellation).  It is stored as
+// s = sqrt(Ce) and -l0 s, so that u = l s - l0 s is one FMA and the exponent
+// another; negated (the argument of exp2), with log2(2|A|) folded in:
+//   term(l) = cos(2 pi cycles(l)) * 2^(nlog - u(l)^2).
 struct PairQ {
   double A, B, C;
-  float l0, nCe, nlog, pad;
+  float s, nl0s, nlog, pad;
 };
 
 template <int D>
@@ -159,8 +160,9 @@ __device__ inline PairQ pair_quad(const RecS<D>& R, const double* pref, const do
     l0 = -Be * r;
     emin = fma(Be, l0, fma(Ce * l0, l0, Ae));   // exponent at l0
   }
-  c.l0 = (float)l0;
-  c.nCe = (float)(-Ce);
+  const float sc = __builtin_amdgcn_sqrtf((float)(Ce > 0.0 ? Ce : 0.0));
+  c.s = sc;
+  c.nl0s = (float)(-l0 * (double)sc);
   c.nlog = (float)(-emin) + __builtin_amdgcn_logf(R.amp);
   c.pad = 0.f;
   return c;
@@ -173,10 +175,10 @@ __device__ inline void eval_quad2(const PairQ& c, double ld, float lf, float& pa
   const double cyc1 = fma(ld1, fma(ld1, c.C, c.B), c.A);
   const float fr0 = (float)__builtin_amdgcn_fract(cyc0);
   const float fr1 = (float)__builtin_amdgcn_fract(cyc1);
-  const float t0 = lf - c.l0;
-  const float t1 = t0 + 64.f;
-  const float a0 = fmaf(t0 * c.nCe, t0, c.nlog);
-  const float a1 = fmaf(t1 * c.nCe, t1, c.nlog);
+  const float u0 = fmaf(lf, c.s, c.nl0s);
+  const float u1 = fmaf(64.f, c.s, u0);
+  const float a0 = fmaf(-u0, u0, c.nlog);
+  const float a1 = fmaf(-u1, u1, c.nlog);
   pa = fmaf(__builtin_amdgcn_cosf(fr0), __builtin_amdgcn_exp2f(a0), pa);
   pb = fmaf(__builtin_amdgcn_cosf(fr1), __builtin_amdgcn_exp2f(a1), pb);
 }
