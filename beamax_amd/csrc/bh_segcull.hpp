@@ -101,6 +101,13 @@ __device__ inline void wave_lds_fence() {
 // One (record, 64-point group) pair of the FP32-split sums in closed form
 // along the group: lane l evaluates the point pref + l v, where
 //   cycles(l)   = A + l (B + l C)                          (FP64, exact algebra)
+// l is an integer, so A, B and C are stored reduced modulo one (|cycles| <
+// 8200 over two groups) and A carries kPhaseBias = 1.5 2^20: every value of
+// the polynomial then lies in [2^20, 2^21), where the FP64 spacing is 2^-32,
+// and the low 32 bits of the result are the fractional cycle in units of
+// 2^-32.  Their top 23 bits under the exponent of 1.0 (one v_alignbit_b32)
+// give 1 + frac in FP32 for v_cos_f32, which reduces revolutions itself; the
+// extra 2^-24 in the bias turns that truncation into rounding.
 //   exponent(l) = Ce (l - l0)^2 + emin   (log2 units; = delta^T Qi delta)
 // The envelope is expanded about its minimum along the line, so the FP32
 // evaluation adds two non-negative terms (no cancellation).  It is stored as
@@ -111,6 +118,8 @@ struct PairQ {
   double A, B, C;
   float s, nl0s, nlog, pad;
 };
+
+constexpr double kPhaseBias = 1572864.0 + 0x1p-24;
 
 template <int D>
 __device__ inline int sym_off(int a, int b) {   // packed index of (a < b)
@@ -147,9 +156,9 @@ __device__ inline PairQ pair_quad(const RecS<D>& R, const double* pref, const do
     Be += 2.0 * v[a] * Ed[a];
     Ce += v[a] * Ev[a];
   }
-  c.A = A;
-  c.B = B;
-  c.C = C;
+  c.A = __builtin_amdgcn_fract(A) + kPhaseBias;
+  c.B = B - rint(B);
+  c.C = C - rint(C);
   double emin = Ae, l0 = 0.0;
   if (Ce > 1e-300) {
     // l0 = -Be / (2 Ce): hardware reciprocal + one Newton step (l0 only
@@ -173,8 +182,10 @@ __device__ inline void eval_quad2(const PairQ& c, double ld, float lf, float& pa
   const double cyc0 = fma(ld, fma(ld, c.C, c.B), c.A);
   const double ld1 = ld + 64.0;
   const double cyc1 = fma(ld1, fma(ld1, c.C, c.B), c.A);
-  const float fr0 = (float)__builtin_amdgcn_fract(cyc0);
-  const float fr1 = (float)__builtin_amdgcn_fract(cyc1);
+  const float fr0 = __uint_as_float(
+      __builtin_amdgcn_alignbit(0x7Fu, (unsigned)__double_as_longlong(cyc0), 9u));
+  const float fr1 = __uint_as_float(
+      __builtin_amdgcn_alignbit(0x7Fu, (unsigned)__double_as_longlong(cyc1), 9u));
   const float u0 = fmaf(lf, c.s, c.nl0s);
   const float u1 = fmaf(64.f, c.s, u0);
   const float a0 = fmaf(-u0, u0, c.nlog);
