@@ -121,7 +121,15 @@ __device__ inline void make_rec(const State<D>& S, double w, RecS<D>& R) {
       R.qi[q] = (float)(si * (S.M[a][b].im + S.M[b][a].im));
       ++q;
     }
-  R.amp = (float)(2.0 * sqrt(S.A.re * S.A.re + S.A.im * S.A.im));
+  // 2|A| in FP32: |A|^2 clamped to [1e-100, 1e100] (the FP32 result is 0 or
+  // infinite outside) and its root from the hardware estimate with one
+  // Newton step
+  {
+    const double a2 = fmin(fmax(S.A.re * S.A.re + S.A.im * S.A.im, 1e-100), 1e100);
+    const double y = __builtin_amdgcn_rsq(a2);
+    const double r0 = a2 * y;
+    R.amp = (float)(2.0 * fma(0.5 * y, fma(-r0, r0, a2), r0));
+  }
   // arg A in FP32 (a few 1e-8 cycles, below the FP32 rounding of the reduced
   // phase that the cosine takes); (0, 0) -> 0 as safe_angle
   const float are = (float)S.A.re, aim = (float)S.A.im;
@@ -316,14 +324,12 @@ __device__ inline double frcp(double x) {
 __device__ inline cd fdiv_csqrt(cd a, cd z) {
   const double r = sqrt(z.re * z.re + z.im * z.im);
   if (!(r > 0.0)) return a / csqrt_(z);
-  cd s;
-  if (z.re >= 0.0) {
-    const double t = sqrt(0.5 * (r + z.re));
-    s = mkc(t, z.im * frcp(2.0 * t));
-  } else {
-    const double t = sqrt(0.5 * (r - z.re));
-    s = mkc(fabs(z.im) * frcp(2.0 * t), copysign(t, z.im));
-  }
+  // principal root: t = sqrt((r + |Re z|) / 2) and u = Im z / (2 t) give
+  // (t, u) for Re z >= 0 and (|u|, t sign(Im z)) otherwise (no branch)
+  const double t = sqrt(0.5 * (r + fabs(z.re)));
+  const double u = z.im * frcp(2.0 * t);
+  const bool right = z.re >= 0.0;
+  const cd s = mkc(right ? t : fabs(u), right ? u : copysign(t, z.im));
   const double ir = frcp(r);
   return mkc((a.re * s.re + a.im * s.im) * ir, (a.im * s.re - a.re * s.im) * ir);
 }

@@ -186,7 +186,9 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
       }
       for (int i = 0; i < run; ++i) {
         const int j0 = rj0 + i * 64 * U;
-        const unsigned long long m = __ballot(hi >= j0 && lo <= j0 + 64 * U - 1);
+        // hi >= j0 and lo <= j0 + 64 U - 1 as lane masks (v_cmp into SGPRs)
+        const unsigned long long m = __builtin_amdgcn_sicmp(hi, j0, 39) &
+                                     __builtin_amdgcn_sicmp(lo, j0 + 64 * U - 1, 41);
         if (lane == 0) mask[ru0 + i][wv] = m;
         npairs += __popcll(m);
       }

@@ -83,12 +83,28 @@ __device__ inline void seg_interval(const SegDev& G, long long k, const double* 
     if (disc == disc) { lo = 1; hi = 0; } else { lo = 0; hi = n - 1; }   // NaN: keep all
     return;
   }
-  const double r = sqrt(disc);
-  const double iA = frcp(A);
+  // sqrt(disc) and 1/A from the hardware estimates with one Newton step
+  // each (relative error below 1e-13 for operands inside [1e-200, 1e200];
+  // outside it the exact forms); the ends are widened by one point plus
+  // 1e-12 of the magnitudes that form them, so the interval always holds
+  // every point whose exponent is <= E
+  double r, iA;
+  if (disc > 1e-200 && disc < 1e200 && A > 1e-200 && A < 1e200) {
+    const double y = __builtin_amdgcn_rsq(disc);
+    const double r0 = disc * y;
+    r = fma(0.5 * y, fma(-r0, r0, disc), r0);
+    const double q = __builtin_amdgcn_rcp(A);
+    iA = q * fma(-A, q, 2.0);
+  } else {
+    r = sqrt(disc);
+    iA = 1.0 / A;
+  }
   const double ulo = (-Bq - r) * iA, uhi = (-Bq + r) * iA;
-  const double flo = floor(ulo) - 1.0, fhi = ceil(uhi) + 1.0;
-  lo = flo < 0.0 ? 0 : (flo > (double)(n - 1) ? n : (int)flo);
-  hi = fhi > (double)(n - 1) ? n - 1 : (fhi < 0.0 ? -1 : (int)fhi);
+  const double slack = 1e-12 * (fabs(Bq) + r) * iA;
+  const double flo = floor(ulo - slack) - 1.0, fhi = ceil(uhi + slack) + 1.0;
+  // flo and fhi are whole numbers: clamp to [0, n] and [-1, n - 1]
+  lo = (int)fmin(fmax(flo, 0.0), (double)n);
+  hi = (int)fmax(fmin(fhi, (double)(n - 1)), -1.0);
 }
 
 // LDS writes of a wave visible to all of its lanes (no workgroup barrier)
