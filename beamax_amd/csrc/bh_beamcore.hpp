@@ -357,9 +357,12 @@ __device__ inline void hom_diag_state_fast(const double* x0, const double* p0, c
   }
   const double dr = frcp(den.re * den.re + den.im * den.im);
   const cd kap = mkc(sn * den.re * dr, -sn * den.im * dr);
+  // M = diag(w) + kap q q^T, with kap q formed once per row
+  cd kq[D];
+  for (int a = 0; a < D; ++a) kq[a] = kap * q[a];
   for (int a = 0; a < D; ++a)
     for (int b = a; b < D; ++b) {
-      cd m = kap * (q[a] * q[b]);
+      cd m = kq[a] * q[b];
       if (a == b) m = m + w[a];
       S.M[a][b] = m;
       S.M[b][a] = m;

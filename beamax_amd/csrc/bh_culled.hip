@@ -209,7 +209,7 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
           const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
           m = ((unsigned long long)hi32 << 32) | lo32;
           if (!m) continue;
-          if ((m >> lane) & 1ull) {
+          if (__builtin_amdgcn_inverse_ballot_w64(m)) {   // this lane's bit of m
             const int pos = __builtin_amdgcn_mbcnt_hi(hi32, __builtin_amdgcn_mbcnt_lo(lo32, 0u));
             plist[wv][base + pos] = (unsigned char)(q * 64 + lane);
           }
