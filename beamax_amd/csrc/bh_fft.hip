@@ -1,9 +1,10 @@
 // Batched strided complex FFT (FP64), mixed radix Stockham autosort.
 //
-// n <= 1024: one LDS-resident pass per launch; each workgroup loads tpb whole
-// transforms (coalesced along whichever axis is contiguous), runs every radix
-// stage between two LDS buffers with a per-block twiddle table, and stores.
-// n > 1024: the same stages run as one global-memory pass per launch.
+// n <= 2560: one LDS-resident pass per launch; each workgroup loads tpb whole
+// transforms (several up to 1024 points, one above; coalesced along whichever
+// axis is contiguous), runs every radix stage between two LDS buffers with a
+// per-block twiddle table, and stores.
+// n > 2560: the same stages run as one global-memory pass per launch.
 //
 // Stage s with radix R over sub-length Ns produces output o from
 //   k = o mod Ns, q = (o / Ns) mod R, j = (o / (Ns R)) Ns + k,
@@ -193,10 +194,20 @@ int fft_strided(double2* data, int64_t n, int64_t batch, int64_t inner, int64_t 
   a.dir = dir;
   a.scale = scale;
   if (n <= kFftLdsMax) {
-    a.tpb = (int)(kFftLdsMax / n);
+    a.tpb = (int)(kFftLdsShare / n);
     if (a.tpb < 1) a.tpb = 1;
     if (a.tpb > batch) a.tpb = (int)batch;
     const size_t lds = (size_t)(2 * a.tpb * n + n) * sizeof(double2);
+    if (lds > 65536) {
+      // above the default dynamic LDS limit of a launch
+      static bool raised = false;
+      if (!raised) {
+        if (hipFuncSetAttribute((const void*)k_fft_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)((2 * kFftLdsMax + kFftLdsMax) * sizeof(double2))) != hipSuccess)
+          BH_FAIL(BH_ERR_HIP, "fft: cannot raise the LDS limit of k_fft_lds");
+        raised = true;
+      }
+    }
     const unsigned grid = (unsigned)((batch + a.tpb - 1) / a.tpb);
     hipLaunchKernelGGL(k_fft_lds, dim3(grid), dim3(256), lds, s, a);
     BH_LAUNCH_CHECK();

@@ -6,8 +6,12 @@
 namespace bh {
 
 // Batched strided 1-D complex FFT in place (see bh_fft in the public header).
-// ws may be null when n <= kFftLdsMax (LDS-resident path needs no workspace).
-constexpr int64_t kFftLdsMax = 1024;
+// ws may be null when n <= kFftLdsMax (LDS-resident path needs no workspace):
+// up to 1024 points several transforms share a workgroup; above, one
+// transform per workgroup with its two buffers and twiddles in up to 120 KiB
+// of the CU's 160 KiB of LDS.
+constexpr int64_t kFftLdsMax = 2560;
+constexpr int64_t kFftLdsShare = 1024;
 size_t fft_workspace_bytes(int64_t n, int64_t batch);
 int fft_strided(double2* data, int64_t n, int64_t batch, int64_t inner, int64_t inner_stride,
                 int64_t outer_stride, int64_t elem_stride, int dir, double scale, void* ws,

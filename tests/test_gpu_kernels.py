@@ -36,7 +36,7 @@ def rel(a, b):
 # ---------------------------------------------------------------------------
 # FFT
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("n", [1, 2, 4, 6, 12, 64, 77, 512, 1024, 1207, 2414])
+@pytest.mark.parametrize("n", [1, 2, 4, 6, 12, 64, 77, 512, 1024, 1207, 2414, 2560, 3000])
 def test_fft_columns_match_numpy(gpu, n):
     import torch
     from beamax_amd import _lib
