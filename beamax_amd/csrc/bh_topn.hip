@@ -195,11 +195,9 @@ __global__ void k_pad(unsigned long long* key, long long* idx, long long k, long
   }
 }
 
-// Full bitonic stages k = 2..kmax (kmax <= tile) inside 2048-element LDS tiles,
-// or (merge_only) the j < tile steps of stage `kstage`.
+// Bitonic sort of each 2048-element tile in LDS (ascending by pair_less).
 __global__ __launch_bounds__(256) void k_bitonic_tile(unsigned long long* key, long long* idx,
-                                                      long long P, int merge_only,
-                                                      long long kstage) {
+                                                      long long P) {
   __shared__ unsigned long long sk[kSortTile];
   __shared__ long long si[kSortTile];
   const long long base = (long long)blockIdx.x * kSortTile;
@@ -209,16 +207,12 @@ __global__ __launch_bounds__(256) void k_bitonic_tile(unsigned long long* key, l
     si[i] = idx[base + i];
   }
   __syncthreads();
-  long long k0 = merge_only ? kstage : 2;
-  long long k1 = merge_only ? kstage : tile;
-  for (long long kk = k0; kk <= k1; kk <<= 1) {
-    long long jstart = merge_only ? (tile >> 1) : (kk >> 1);
-    for (long long j = jstart; j > 0; j >>= 1) {
+  for (int kk = 2; kk <= tile; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
       for (int t = threadIdx.x; t < tile / 2; t += blockDim.x) {
-        const int lo = (int)(((t / j) * 2 * j) + (t % j));
-        const int hi = lo + (int)j;
-        const long long gpos = base + lo;
-        const bool up = ((gpos & kk) == 0);
+        const int lo = ((t / j) * 2 * j) + (t % j);
+        const int hi = lo + j;
+        const bool up = (lo & kk) == 0;
         const bool sw = up ? pair_less(sk[hi], si[hi], sk[lo], si[lo])
                            : pair_less(sk[lo], si[lo], sk[hi], si[hi]);
         if (sw) {
@@ -235,20 +229,35 @@ __global__ __launch_bounds__(256) void k_bitonic_tile(unsigned long long* key, l
   }
 }
 
-__global__ void k_bitonic_global(unsigned long long* key, long long* idx, long long P,
-                                 long long kk, long long j) {
+// One merge level: adjacent sorted runs of length L into runs of 2L, one
+// thread per element.  An element of the first run of a pair lands at its
+// index plus the number of second-run elements below it; an element of the
+// second run at its index plus the number of first-run elements not above it
+// (a stable merge, so the equal padding pairs keep distinct slots).
+__global__ void k_merge_runs(const unsigned long long* __restrict__ key,
+                             const long long* __restrict__ idx,
+                             unsigned long long* __restrict__ okey, long long* __restrict__ oidx,
+                             long long P, long long L) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= P / 2) return;
-  const long long lo = (t / j) * 2 * j + (t % j);
-  const long long hi = lo + j;
-  const bool up = ((lo & kk) == 0);
-  const unsigned long long kl = key[lo], kh = key[hi];
-  const long long il = idx[lo], ih = idx[hi];
-  const bool sw = up ? pair_less(kh, ih, kl, il) : pair_less(kl, il, kh, ih);
-  if (sw) {
-    key[lo] = kh; key[hi] = kl;
-    idx[lo] = ih; idx[hi] = il;
+  if (t >= P) return;
+  const long long run = t / L;
+  const long long pos = t - run * L;
+  const long long a0 = (run & ~1ll) * L;
+  const bool first = (run & 1) == 0;
+  const long long o = first ? a0 + L : a0;
+  const unsigned long long k = key[t];
+  const long long ix = idx[t];
+  long long lo = 0, hi = L;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    const unsigned long long km = key[o + mid];
+    const long long im = idx[o + mid];
+    const bool before = first ? pair_less(km, im, k, ix) : !pair_less(k, ix, km, im);
+    if (before) lo = mid + 1;
+    else hi = mid;
   }
+  okey[a0 + pos + lo] = k;
+  oidx[a0 + pos + lo] = ix;
 }
 
 __global__ void k_gather_out(const double2* __restrict__ c, const long long* __restrict__ sidx,
@@ -271,6 +280,8 @@ struct TopnWs {
   unsigned long long* key;
   unsigned long long* okey;
   long long* oidx;
+  unsigned long long* okey2;   // merge ping-pong
+  long long* oidx2;
   long long* cnt;
   unsigned int* hist;
   SelState* st;
@@ -290,6 +301,8 @@ TopnWs carve(void* ws, long long n, long long k) {
   w.key = (unsigned long long*)take(n * 8);
   w.okey = (unsigned long long*)take(P * 8);
   w.oidx = (long long*)take(P * 8);
+  w.okey2 = (unsigned long long*)take(P * 8);
+  w.oidx2 = (long long*)take(P * 8);
   w.cnt = (long long*)take(2 * nblk * 8);
   w.hist = (unsigned int*)take(256 * 4);
   w.st = (SelState*)take(sizeof(SelState));
@@ -357,17 +370,22 @@ extern "C" int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out,
   hipLaunchKernelGGL(k_compact, dim3((unsigned)nblk), dim3(kSelBlock), 0, st, w.key,
                      (long long)n, w.st, w.cnt, w.okey, w.oidx);
   BH_LAUNCH_CHECK();
-  // bitonic sort of P (power of two >= 2048) pairs, ascending
+  // sort of P (power of two >= 2048) pairs, ascending: bitonic within
+  // 2048-element tiles, then log2(P / 2048) merge levels
   const unsigned tiles = (unsigned)(P / kSortTile);
-  hipLaunchKernelGGL(k_bitonic_tile, dim3(tiles), dim3(256), 0, st, w.okey, w.oidx, P, 0, 0ll);
-  for (long long kk = 2 * kSortTile; kk <= P; kk <<= 1) {
-    for (long long j = kk >> 1; j >= kSortTile; j >>= 1)
-      hipLaunchKernelGGL(k_bitonic_global, dim3(bh::grid1d(P / 2, 256)), dim3(256), 0, st,
-                         w.okey, w.oidx, P, kk, j);
-    hipLaunchKernelGGL(k_bitonic_tile, dim3(tiles), dim3(256), 0, st, w.okey, w.oidx, P, 1, kk);
+  hipLaunchKernelGGL(k_bitonic_tile, dim3(tiles), dim3(256), 0, st, w.okey, w.oidx, P);
+  unsigned long long* ka = w.okey;
+  long long* ia = w.oidx;
+  unsigned long long* kb = w.okey2;
+  long long* ib = w.oidx2;
+  for (long long L = kSortTile; L < P; L <<= 1) {
+    hipLaunchKernelGGL(k_merge_runs, dim3(bh::grid1d(P, 256)), dim3(256), 0, st, ka, ia, kb, ib,
+                       P, L);
+    unsigned long long* tk = ka; ka = kb; kb = tk;
+    long long* ti = ia; ia = ib; ib = ti;
   }
   hipLaunchKernelGGL(k_gather_out, dim3(bh::grid1d(k, 256)), dim3(256), 0, st,
-                     (const double2*)c, w.oidx, (long long)k, (long long*)idx_out,
+                     (const double2*)c, ia, (long long)k, (long long*)idx_out,
                      (double2*)val_out);
   BH_LAUNCH_CHECK();
   return BH_OK;
