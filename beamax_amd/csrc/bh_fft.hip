@@ -41,7 +41,7 @@ __device__ inline long long fft_base(long long t, const FftArgs& a) {
   return (t / a.inner) * a.outer_stride + (t % a.inner) * a.inner_stride;
 }
 
-__global__ __launch_bounds__(256) void k_fft_lds(FftArgs a) {
+__global__ __launch_bounds__(1024) void k_fft_lds(FftArgs a) {
   extern __shared__ double2 sm[];
   const int n = a.n, tpb = a.tpb, tot = tpb * n;
   double2* b0 = sm;
@@ -209,7 +209,10 @@ int fft_strided(double2* data, int64_t n, int64_t batch, int64_t inner, int64_t 
       }
     }
     const unsigned grid = (unsigned)((batch + a.tpb - 1) / a.tpb);
-    hipLaunchKernelGGL(k_fft_lds, dim3(grid), dim3(256), lds, s, a);
+    // a workgroup that holds one long transform takes 16 waves, so that the
+    // CU it fills alone still has four waves per SIMD to hide LDS latency
+    const unsigned threads = (a.tpb == 1 && n >= 1024) ? 1024 : 256;
+    hipLaunchKernelGGL(k_fft_lds, dim3(grid), dim3(threads), lds, s, a);
     BH_LAUNCH_CHECK();
     return BH_OK;
   }

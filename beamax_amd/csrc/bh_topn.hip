@@ -9,8 +9,9 @@
 //    many of the entries equal to T are needed (the largest indices win).
 // 3. Stable compaction of the winners (per-block counts, one scan block,
 //    per-block ordered writes).
-// 4. Bitonic sort of the winners on (key, index): 2048-element LDS tiles, then
-//    global merge passes for the long strides.
+// 4. Sort of the winners on (key, index): bitonic within 512-element LDS
+//    tiles, then stable merge levels (one thread per element, its output slot
+//    from a binary search in the partner run).
 // 5. Gather of indices and values.
 #include "bh_internal.hpp"
 
@@ -19,7 +20,7 @@ namespace {
 constexpr int kSelBlock = 256;
 constexpr int kItems = 16;                       // keys per thread per block pass
 constexpr int kChunk = kSelBlock * kItems;       // keys per block
-constexpr int kSortTile = 2048;
+constexpr int kSortTile = 512;
 
 struct SelState {
   unsigned long long prefix;   // fixed high bits of T
@@ -117,16 +118,37 @@ __global__ __launch_bounds__(kSelBlock) void k_count(const unsigned long long* _
   }
 }
 
-// single block: exclusive scans of the per-block counts (gt and eq) in place
-__global__ void k_scan_counts(long long* cnt, long long nblk) {
-  if (threadIdx.x != 0) return;
-  long long sg = 0, se = 0;
-  for (long long b = 0; b < nblk; ++b) {
-    const long long g = cnt[2 * b], e = cnt[2 * b + 1];
-    cnt[2 * b] = sg;
-    cnt[2 * b + 1] = se;
-    sg += g;
-    se += e;
+// one block of 1024 threads: exclusive scans of the per-block counts (gt and
+// eq) in place; thread t owns a contiguous slice of the blocks, the slice
+// sums are scanned in LDS
+__global__ __launch_bounds__(1024) void k_scan_counts(long long* cnt, long long nblk) {
+  __shared__ long long sg[1024], se[1024];
+  const int t = threadIdx.x;
+  const long long per = (nblk + 1023) / 1024;
+  const long long b0 = t * per;
+  const long long b1 = b0 + per < nblk ? b0 + per : nblk;
+  long long g = 0, e = 0;
+  for (long long b = b0; b < b1; ++b) {
+    g += cnt[2 * b];
+    e += cnt[2 * b + 1];
+  }
+  sg[t] = g;
+  se[t] = e;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const long long ag = t >= off ? sg[t - off] : 0, ae = t >= off ? se[t - off] : 0;
+    __syncthreads();
+    sg[t] += ag;
+    se[t] += ae;
+    __syncthreads();
+  }
+  long long rg = sg[t] - g, re = se[t] - e;
+  for (long long b = b0; b < b1; ++b) {
+    const long long gg = cnt[2 * b], ee = cnt[2 * b + 1];
+    cnt[2 * b] = rg;
+    cnt[2 * b + 1] = re;
+    rg += gg;
+    re += ee;
   }
 }
 
@@ -195,7 +217,7 @@ __global__ void k_pad(unsigned long long* key, long long* idx, long long k, long
   }
 }
 
-// Bitonic sort of each 2048-element tile in LDS (ascending by pair_less).
+// Bitonic sort of each 512-element tile in LDS (ascending by pair_less).
 __global__ __launch_bounds__(256) void k_bitonic_tile(unsigned long long* key, long long* idx,
                                                       long long P) {
   __shared__ unsigned long long sk[kSortTile];
@@ -363,15 +385,15 @@ extern "C" int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out,
   }
   hipLaunchKernelGGL(k_count, dim3((unsigned)nblk), dim3(kSelBlock), 0, st, w.key, (long long)n,
                      w.st, w.cnt);
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(64), 0, st, w.cnt, nblk);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.cnt, nblk);
   const long long P = pow2_at_least(k < kSortTile ? kSortTile : k);
   hipLaunchKernelGGL(k_pad, dim3(bh::grid1d(P - k, 256)), dim3(256), 0, st, w.okey, w.oidx,
                      (long long)k, P);
   hipLaunchKernelGGL(k_compact, dim3((unsigned)nblk), dim3(kSelBlock), 0, st, w.key,
                      (long long)n, w.st, w.cnt, w.okey, w.oidx);
   BH_LAUNCH_CHECK();
-  // sort of P (power of two >= 2048) pairs, ascending: bitonic within
-  // 2048-element tiles, then log2(P / 2048) merge levels
+  // sort of P (power of two >= 512) pairs, ascending: bitonic within
+  // 512-element tiles, then log2(P / 512) merge levels
   const unsigned tiles = (unsigned)(P / kSortTile);
   hipLaunchKernelGGL(k_bitonic_tile, dim3(tiles), dim3(256), 0, st, w.okey, w.oidx, P);
   unsigned long long* ka = w.okey;
