@@ -91,3 +91,33 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     assert set(_lib.SIGNATURES) <= names
     assert lib.bh_version() == 1
+
+
+def test_biased_phase_fraction_extraction():
+    """The FP32-split sums take the fractional cycle of A + l (B + l C) from
+    the low 32 bits of the biased FP64 value (A reduced modulo one plus
+    1.5 2^20 + 2^-24, B and C reduced to the nearest integer, l < 128) and
+    move their top 23 bits under the exponent of 1.0.  Restated here in
+    numpy and checked against exact rational arithmetic: the value stays in
+    [2^20, 2^21) and the cycle is rounded to within 2^-24 (plus the 2^-32
+    spacing of the FP64 value)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(7)
+    bias = 1572864.0 + 2.0 ** -24
+    worst = 0.0
+    for _ in range(3000):
+        A = rng.uniform(-5e4, 5e4)
+        B = rng.uniform(-300.0, 300.0) * rng.choice([1.0, 1e-3])
+        C = rng.uniform(-2.0, 2.0) * rng.choice([1.0, 1e-4])
+        l = float(rng.integers(0, 128))
+        Ap, Bp, Cp = A - np.floor(A), B - np.rint(B), C - np.rint(C)
+        y = np.float64(l * (Bp + l * Cp)) + np.float64(Ap + bias)
+        assert 2.0 ** 20 <= y < 2.0 ** 21
+        low = int(np.array([y]).view(np.uint64)[0]) & 0xFFFFFFFF
+        x = np.array([(0x7F << 23) | (low >> 9)], dtype=np.uint32).view(np.float32)[0]
+        assert 1.0 <= x < 2.0
+        exact = Fraction(A) + Fraction(l) * (Fraction(B) + Fraction(l) * Fraction(C))
+        frac = float(exact - (exact.numerator // exact.denominator))
+        d = abs((float(x) - 1.0) - frac)
+        worst = max(worst, min(d, 1.0 - d))
+    assert worst <= 2.0 ** -24 + 2.0 ** -30
