@@ -68,26 +68,30 @@ __global__ __launch_bounds__(kSelBlock) void k_hist(const unsigned long long* __
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
-// one block of 256: choose the digit of the k-th largest key, clear the histogram
-__global__ void k_pick(SelState* st, unsigned int* hist, int shift) {
-  __shared__ unsigned int h[256];
+// one block of 256: choose the digit of the k-th largest key (the largest
+// digit whose suffix count reaches `need`, from a parallel suffix scan),
+// clear the histogram
+__global__ __launch_bounds__(256) void k_pick(SelState* st, unsigned int* hist, int shift) {
+  __shared__ long long sfx[256];
   const int t = threadIdx.x;
-  h[t] = hist[t];
+  const long long hv = hist[t];
+  const long long need = st->need;
+  sfx[t] = hv;
   __syncthreads();
-  if (t == 0) {
-    long long need = st->need;
-    long long above = 0;
-    int digit = 0;
-    for (int b = 255; b >= 0; --b) {
-      if (above + (long long)h[b] >= need) { digit = b; break; }
-      above += h[b];
-    }
-    st->prefix |= ((unsigned long long)digit) << shift;
+  for (int off = 1; off < 256; off <<= 1) {
+    const long long v = t + off < 256 ? sfx[t + off] : 0;
+    __syncthreads();
+    sfx[t] += v;
+    __syncthreads();
+  }
+  // sfx is non-increasing in the digit: exactly one thread holds the digit
+  const long long above = sfx[t] - hv;   // keys with a larger digit
+  if (sfx[t] >= need && above < need) {
+    st->prefix |= ((unsigned long long)t) << shift;
     st->mask |= 0xffull << shift;
     st->need = need - above;
-    st->eq_total = h[digit];
+    st->eq_total = hv;
   }
-  __syncthreads();
   hist[t] = 0u;
 }
 
