@@ -40,11 +40,11 @@ FP32_VALU_PEAK_TFLOPS = 157.3
 # issue rate: 256 CUs x 64 lanes x 2.4 GHz / 4 / 2
 TRANS_BOUND_EVALS = 256 * 64 * 2.4e9 / 4 / 2
 # HBM bytes per forward launch of the culled FP32-split kernel on this
-# workload at cull_log2 = 40: rocprofv3 FETCH_SIZE (386661.1 KiB raw, x2.0)
-# + WRITE_SIZE (77711.2 KiB raw, x1.0), both scaled by what the counters read
+# workload at cull_log2 = 40: rocprofv3 FETCH_SIZE (72047.4 KiB raw, x2.0)
+# + WRITE_SIZE (155403.7 KiB raw, x1.0), both scaled by what the counters read
 # for a 512 MiB copy (profiles/k_culled_sum_msgb2d_512.md).  Counters cannot
 # be read from inside the process, so other settings report null.
-MEASURED_FWD_TRAFFIC = {40.0: 386661.125 * 2.0 * 1024 + 77711.21875 * 1.0 * 1024}
+MEASURED_FWD_TRAFFIC = {40.0: 72047.4375 * 2.0 * 1024 + 155403.6875 * 1.0 * 1024}
 
 
 def build_problem(device, cull_log2=40.0):
