@@ -2,7 +2,7 @@
 
 Tolerances: integer/index work is compared exactly; FP64 kernels to ~1e-12
 relative (different but correctly-rounded op order); the FP32-split summation
-to rel-L2 <= 1e-5 against the FP64 kernel (the north-star gate).
+to rel-L2 <= 1e-5 against the FP64 kernel (the accuracy gate of the FP32-split operator).
 """
 
 import ctypes as C
