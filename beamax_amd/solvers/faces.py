@@ -131,12 +131,24 @@ class FourEdgeAdjoint:
         return out
 
     def beams(self, data) -> StaticBeams:
+        """The four faces' boundary beams, each face prepared on a stream of
+        its own (their small crop/MSWPT/top_n/parameter kernels overlap);
+        the caller's stream waits for all four before the concatenation."""
+        import torch
+        main = torch.cuda.current_stream(self.device)
+        data = data.to(torch.float64).contiguous()   # once, on the caller's stream
+        if getattr(self, "_streams", None) is None:
+            self._streams = [torch.cuda.Stream(device=self.device) for _ in self.faces]
         tabs = []
-        for f in self.faces:
-            face_data = self.crop(data, f)
-            tabs.append(self.solver.prepare_adjoint_beams(
-                face_data, f["sources"], f["data_domain"], f["wpt"], self.c,
-                swap_axes=f["swap"], plane=f["plane"], medium=self.medium))
+        for f, st in zip(self.faces, self._streams):
+            st.wait_stream(main)
+            with torch.cuda.stream(st):
+                face_data = self.crop(data, f)
+                tabs.append(self.solver.prepare_adjoint_beams(
+                    face_data, f["sources"], f["data_domain"], f["wpt"], self.c,
+                    swap_axes=f["swap"], plane=f["plane"], medium=self.medium))
+        for st in self._streams:
+            main.wait_stream(st)
         return StaticBeams.concat(tabs)
 
     def __call__(self, data):
