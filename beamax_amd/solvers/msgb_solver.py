@@ -327,6 +327,19 @@ class MSGBSolver:
 
     def _segments(self, points, dx, device):
         from ..segments import Segments
+        if hasattr(points, "data_ptr"):
+            # a device tensor (the image grid of the adjoint): keyed by the
+            # tensor itself and its in-place version, so a repeated call
+            # neither copies it to the host nor hashes it (the copy would
+            # stall the stream between the preparation and the sum)
+            key = ("tensor", id(points), points.data_ptr(), points._version,
+                   tuple(points.shape), tuple(dx), str(device))
+            hit = self._seg_cache.get(key)
+            if hit is None:
+                pn = np.ascontiguousarray(_np(points), dtype=np.float64).reshape(-1, len(dx))
+                hit = (Segments(pn, dx, device), points)   # holds the tensor: its id stays unique
+                self._seg_cache[key] = hit
+            return hit[0]
         key = (points.shape, hash(np.ascontiguousarray(points, dtype=np.float64).tobytes()),
                tuple(dx), str(device))
         seg = self._seg_cache.get(key)
@@ -687,9 +700,11 @@ class MSGBSolver:
         dev = beams.x.device
         d = beams.d
         if self.cull_log2 > 0 and dx is not None:
-            pn = _np(points) if isinstance(points, torch.Tensor) else points
-            pn = np.ascontiguousarray(pn, dtype=np.float64).reshape(-1, d)
-            seg = self._segments(pn, dx, dev)
+            if isinstance(points, torch.Tensor):
+                seg = self._segments(points, dx, dev)
+            else:
+                seg = self._segments(np.ascontiguousarray(points, dtype=np.float64).reshape(-1, d),
+                                     dx, dev)
             if seg.efficient:
                 dt = torch.float64 if self.precision == "fp64" else torch.float32
                 if out is None:
