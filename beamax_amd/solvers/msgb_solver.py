@@ -325,6 +325,23 @@ class MSGBSolver:
                   idx.numel(), beams.struct(), _lib.stream_handle())
         return beams, idx, val
 
+    def _ts_device(self, ts, device):
+        """The time grid on the device, cached by its contents: a repeated
+        call makes no host-to-device copy (a pageable copy would wait for the
+        stream's earlier work before the sum could be launched)."""
+        import torch
+        if isinstance(ts, torch.Tensor) and ts.device == torch.device(device) \
+                and ts.dtype == torch.float64:
+            return ts.contiguous()
+        t = np.ascontiguousarray(_np(ts), dtype=np.float64)
+        key = (t.shape, hash(t.tobytes()), str(device))
+        cache = self.__dict__.setdefault("_ts_cache", {})
+        hit = cache.get(key)
+        if hit is None:
+            hit = torch.as_tensor(t, device=device)
+            cache[key] = hit
+        return hit
+
     def _segments(self, points, dx, device):
         from ..segments import Segments
         if hasattr(points, "data_ptr"):
@@ -362,7 +379,7 @@ class MSGBSolver:
                                           "boundaries is not on the GPU path")
             pts = np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d)
             seg = self._segments(pts, domain.dx, dev)
-            tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
+            tsd = self._ts_device(ts, dev)
             f64 = self.precision == "fp64"
             if self.use_real:
                 dt = torch.float64 if f64 else torch.float32
@@ -384,7 +401,7 @@ class MSGBSolver:
             pts = np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d)
             seg = self._segments(pts, domain.dx, dev)
             if seg.efficient:
-                tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
+                tsd = self._ts_device(ts, dev)
                 dt = torch.float64 if self.precision == "fp64" else torch.float32
                 if out is None:
                     out = torch.empty((tsd.numel(), seg.S), dtype=dt, device=dev)
@@ -399,7 +416,7 @@ class MSGBSolver:
                 return out
         pos = torch.as_tensor(np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d),
                               device=dev)
-        tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
+        tsd = self._ts_device(ts, dev)
         nt, S = tsd.numel(), pos.shape[0]
         dt = torch.float64 if self.precision == "fp64" else torch.float32
         if out is None:
@@ -471,7 +488,7 @@ class MSGBSolver:
         if sp is None:
             from ..utils.interp import GridSpeed
             sp = GridSpeed(np.full((1,) * d, float(domain.c)))
-        tsd = torch.as_tensor(np.ascontiguousarray(_np(ts), dtype=np.float64), device=dev)
+        tsd = self._ts_device(ts, dev)
         nt = tsd.numel()
         total = 2 * beams.count
         ns = 2 * d + 2 * d * d + 2
