@@ -409,10 +409,11 @@ struct Shape {
 
 inline Shape culled_shape(long long G, long long ny, long long nchunk, bool have_part,
                           int min_gpb, int max_split) {
-  // about 8 rounds of 4 blocks on each of 256 CUs, so that the partly
-  // filled last round is a small share (2414 rows: split 4 gives 30.4 ms,
-  // split 2 31.2 ms, split 1 32.5 ms on the 512^2 forward)
-  const long long target = 8192;
+  // about 20 rounds of 4 blocks on each of 256 CUs, so that the blocks
+  // still running at the end are a small share of the launch (2414 rows,
+  // 2^-40 cut: split 4 gives 30.4 ms, split 2 31.2 ms, split 1 32.5 ms on
+  // the 512^2 forward; at 2^-32, split 8 26.65 ms against 26.83 ms at 4)
+  const long long target = 20480;
   Shape sh{min_gpb, 1};
   const int opts[4] = {32, 16, 8, 4};
   for (int o : opts) {
