@@ -40,13 +40,12 @@ FP32_VALU_PEAK_TFLOPS = 157.3
 # issue rate: 256 CUs x 64 lanes x 2.4 GHz / 4 / 2
 TRANS_BOUND_EVALS = 256 * 64 * 2.4e9 / 4 / 2
 # HBM bytes per forward launch of the culled FP32-split kernel on this
-# workload: rocprofv3 FETCH_SIZE (KiB raw, x2.0) + WRITE_SIZE (KiB raw, x1.0),
-# both scaled by what the counters read for a 512 MiB copy
-# (profiles/k_culled_sum_msgb2d_512.md), at cull_log2 = 32 (72778.8 and
-# 155402.3 KiB) and 40 (72047.4 and 155403.7 KiB).  Counters cannot be read
-# from inside the process, so other settings report null.
-MEASURED_FWD_TRAFFIC = {32.0: 72778.8125 * 2.0 * 1024 + 155402.34375 * 1.0 * 1024,
-                        40.0: 72047.4375 * 2.0 * 1024 + 155403.6875 * 1.0 * 1024}
+# workload (8 beam ranges): rocprofv3 FETCH_SIZE (KiB raw, x2.0) + WRITE_SIZE
+# (KiB raw, x1.0) of the steady-state dispatch, both scaled by what the
+# counters read for a 512 MiB copy (profiles/k_culled_sum_msgb2d_512.md), at
+# cull_log2 = 32 (46232.1 and 308388.5 KiB).  Counters cannot be read from
+# inside the process, so other settings report null.
+MEASURED_FWD_TRAFFIC = {32.0: 46232.0625 * 2.0 * 1024 + 308388.53125 * 1.0 * 1024}
 
 
 def build_problem(device, cull_log2=32.0):
