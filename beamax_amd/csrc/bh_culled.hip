@@ -158,7 +158,7 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
           }
           St.A = mkc(y[2 * D + 2 * D * D], y[2 * D + 2 * D * D + 1]);
         } else {
-          hom_diag_state_fast<D>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]),
+          hom_diag_state_fast<D, !F64>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]),
                                  neg ? -A.c : A.c, tv, St);
         }
         make_rec<D>(St, A.b.omega[j], R);
