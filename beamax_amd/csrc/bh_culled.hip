@@ -20,7 +20,10 @@
 namespace {
 
 constexpr int kRecChunk = 256;
-constexpr int kStaticSplit = 16;   // beam ranges of a precomputed (image) sum
+constexpr int kStaticSplit = 64;   // beam ranges of a precomputed (image) sum
+// blocks a culled launch aims at: about 20 rounds of 4 blocks on each of 256
+// CUs, so that the blocks still running at the end are a small share
+constexpr long long kTargetBlocks = 20480;
 
 struct CulledArgs {
   // forward source: mirror != 0 -> beams [0, count) with mode +1 and their
@@ -409,11 +412,10 @@ struct Shape {
 
 inline Shape culled_shape(long long G, long long ny, long long nchunk, bool have_part,
                           int min_gpb, int max_split) {
-  // about 20 rounds of 4 blocks on each of 256 CUs, so that the blocks
-  // still running at the end are a small share of the launch (2414 rows,
-  // 2^-40 cut: split 4 gives 30.4 ms, split 2 31.2 ms, split 1 32.5 ms on
-  // the 512^2 forward; at 2^-32, split 8 26.65 ms against 26.83 ms at 4)
-  const long long target = 20480;
+  // kTargetBlocks (2414 rows, 2^-40 cut: split 4 gives 30.4 ms, split 2
+  // 31.2 ms, split 1 32.5 ms on the 512^2 forward; at 2^-32, split 8 26.65
+  // ms against 26.83 ms at 4)
+  const long long target = kTargetBlocks;
   Shape sh{min_gpb, 1};
   const int opts[4] = {32, 16, 8, 4};
   for (int o : opts) {
@@ -437,8 +439,10 @@ int launch_culled(CulledArgs a, long long ny, long long nchunk, void* part_ws, s
                   hipStream_t s) {
   const long long G = a.G.ngroups;
   const size_t per_split = (size_t)ny * a.G.S * sizeof(double);
-  // precomputed (image) sums: 16 groups per block and up to 16 beam ranges
-  // (measured: 4.76 ms against 5.00 ms with 8 and 8 on the 512^2 adjoint)
+  // precomputed (image) sums: 16 groups per block and up to 64 beam ranges
+  // (the 512^2 adjoint: 2.46 ms against 2.74 ms with 16 ranges, 2.56 ms
+  // with 32; earlier, with the records in LDS, 16 groups and 16 ranges took
+  // 4.76 ms against 5.00 ms with 8 and 8)
   const Shape sh = culled_shape(G, ny, nchunk, part_ws != nullptr && part_bytes >= 2 * per_split,
                                 F64 ? 4 : (STATIC ? 16 : 8), STATIC ? kStaticSplit : 8);
   a.split = sh.split;
@@ -556,7 +560,13 @@ extern "C" int bh_forward_sum_modes(const bh_fwd_beams* b, int ndim, int64_t npo
 extern "C" size_t bh_static_sum_culled_workspace_size(int64_t count, int ndim, int64_t P) {
   if (count <= 0 || P <= 0) return 0;
   const size_t rec = (bh_static_sum_workspace_size(count, ndim) + 255) / 256 * 256;
-  return rec + (size_t)kStaticSplit * P * sizeof(double);
+  // the launch has at least P / 2048 blocks per beam range (at most 32
+  // groups of 64 points per block), so it splits at most
+  // ceil(kTargetBlocks / (P / 2048)) ways
+  const long long blocks = P / 2048 > 1 ? P / 2048 : 1;
+  long long split = (kTargetBlocks + blocks - 1) / blocks;
+  if (split > kStaticSplit) split = kStaticSplit;
+  return rec + (size_t)split * P * sizeof(double);
 }
 
 extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh_segments* seg,
