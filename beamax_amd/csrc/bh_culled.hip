@@ -68,7 +68,12 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
   constexpr int U = F64 ? 1 : 2;
   static_assert(F64 || SPT >= 2, "the FP32-split form needs at least 8 groups per block");
   constexpr int NU = GPB / U;    // units per block
-  constexpr int UPW = SPT / U;   // units per wave (wave wv owns units wv + 4 i)
+  constexpr int UPW = SPT / U;   // units per wave
+  // FP32-split: wave wv owns units 4 i + ((wv + i) mod 4), i < UPW -- in
+  // every four consecutive units one, at a position that rotates with i, so
+  // that a wave does not get the same place (say, the corner units) of every
+  // run; the F64 form: groups wv + 4 i
+  auto unit_of = [wv = (int)(threadIdx.x >> 6)](int i) { return 4 * i + ((wv + i) & 3); };
   // precomputed FP32-split records stay in global memory (L2): the masks
   // use each thread's own record from registers, the pair coefficients read
   // theirs from the table, and the block's LDS drops to 13 KB (measured on
@@ -206,12 +211,12 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
     __syncthreads();
     if constexpr (!F64) {
       // 1. compact the wave's active pairs: unit by unit (its units
-      //    wv + 4 i), record order within a unit
+      //    unit_of(i)), record order within a unit
       int gs[UPW + 1];
       int base = 0;
       for (int i = 0; i < UPW; ++i) {
         gs[i] = base;
-        const int ul = wv + 4 * i;
+        const int ul = unit_of(i);
         if (ul >= nu) continue;
         for (int q = 0; q < 4; ++q) {
           unsigned long long m = mask[ul][q];
@@ -243,7 +248,7 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
         if (e < base) {
           int gi = 0;
           for (int i = 1; i < UPW; ++i) gi = (e >= gs[i]) ? i : gi;
-          const int ul = wv + 4 * gi;
+          const int ul = unit_of(gi);
           if constexpr (GREC)
             coef[wv][lane] = pair_quad<D>(((const Rec*)A.recs)[cb + plist[wv][e]], gpt[ul],
                                           gstep[ul]);
@@ -318,7 +323,8 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
   }
   if (A.pairs && lane == 0) atomicAdd(A.pairs, npairs * U);
   for (int i = 0; i < SPT; ++i) {
-    const int gl = (wv + 4 * (i / U)) * U + i % U;   // unit wv + 4 (i / U), group i % U
+    // unit unit_of(i / U), group i % U (F64: group wv + 4 i)
+    const int gl = F64 ? wv + 4 * i : unit_of(i / U) * U + i % U;
     if (gl >= ng) continue;
     const int col = A.G.col[(gbase + gl) * 64 + lane];
     if (col < 0) continue;
