@@ -80,7 +80,9 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
   // the 512^2 adjoint: 4.44 against 4.84 ms with the records in LDS)
   constexpr bool GREC = STATIC && !F64;
   __shared__ Rec rec[GREC ? 1 : kRecChunk];
-  __shared__ unsigned long long mask[NU][4];
+  // masks of the chunk being evaluated and of the next one (a wave builds
+  // the next chunk's masks while the other waves still evaluate)
+  __shared__ unsigned long long mask[2][NU][4];
   // the block's units grouped by run: per piece its run, first unit, number
   // of units and the index of its first point within the run
   __shared__ int4 runs[NU];
@@ -130,16 +132,23 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
     }
     if (lane == 0) nruns = __popcll(st);
   }
+  __syncthreads();   // the runs table and the group tables
   unsigned long long npairs = 0;
   const long long total = STATIC ? A.count : A.total;
   const long long nchunk = (total + kRecChunk - 1) / kRecChunk;
   const long long c_lo = nchunk * bz / A.split;
   const long long c_hi = nchunk * (bz + 1) / A.split;
+  // Per chunk: each thread builds its record in registers and its wave the
+  // chunk's masks (into the mask buffer of the chunk's parity), both before
+  // the barrier that waits for the previous chunk's evaluation, so a wave
+  // that finished early fills the wait with this work; then the records go
+  // to LDS and, after a second barrier, the waves evaluate.  Two chunks
+  // apart the mask buffer is rewritten only after every wave has passed the
+  // second barrier of the chunk in between, so no wave still reads it.
   for (long long cb = c_lo * kRecChunk; cb < c_hi * kRecChunk && cb < total; cb += kRecChunk) {
-    __syncthreads();   // previous chunk's records and masks fully consumed
+    const int par = (int)((cb / kRecChunk) & 1);
     const long long bi = cb + threadIdx.x;
-    Rec Rreg;
-    Rec& R = GREC ? Rreg : rec[threadIdx.x];
+    Rec R;
     bool live = bi < total;
     if (live) {
       if constexpr (STATIC) {
@@ -204,11 +213,15 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
         // hi >= j0 and lo <= j0 + 64 U - 1 as lane masks (v_cmp into SGPRs)
         const unsigned long long m = __builtin_amdgcn_sicmp(hi, j0, 39) &
                                      __builtin_amdgcn_sicmp(lo, j0 + 64 * U - 1, 41);
-        if (lane == 0) mask[ru0 + i][wv] = m;
+        if (lane == 0) mask[par][ru0 + i][wv] = m;
         npairs += __popcll(m);
       }
     }
-    __syncthreads();
+    if constexpr (!GREC) {
+      __syncthreads();   // every wave is done with the previous chunk's records
+      rec[threadIdx.x] = R;
+    }
+    __syncthreads();   // this chunk's records and masks are complete
     if constexpr (!F64) {
       // 1. compact the wave's active pairs: unit by unit (its units
       //    unit_of(i)), record order within a unit
@@ -219,7 +232,7 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
         const int ul = unit_of(i);
         if (ul >= nu) continue;
         for (int q = 0; q < 4; ++q) {
-          unsigned long long m = mask[ul][q];
+          unsigned long long m = mask[par][ul][q];
           const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)m);
           const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
           m = ((unsigned long long)hi32 << 32) | lo32;
@@ -288,7 +301,7 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
         for (int a = 0; a < D; ++a) pti[a] = A.G.pts[slot * D + a];
       }
       for (int q = 0; q < 4; ++q) {
-        unsigned long long m = mask[gl][q];
+        unsigned long long m = mask[par][gl][q];
         const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)m);
         const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(m >> 32));
         m = ((unsigned long long)hi32 << 32) | lo32;
