@@ -149,6 +149,7 @@ class FourEdgeAdjoint:
                     swap_axes=f["swap"], plane=f["plane"], medium=self.medium))
         for st in self._streams:
             main.wait_stream(st)
+        self.face_beams = tabs   # per-face tables (selection diagnostics)
         return StaticBeams.concat(tabs)
 
     def __call__(self, data):

@@ -52,6 +52,22 @@ def mT_inverse(xdot, mT_st, pdot=None):
     return np.concatenate([top, bot], axis=1)
 
 
+def start_time(k_t, support_t):
+    """tr_solver_utils.py:308-309: t_start = k_t / (rho bl_t), the time-axis
+    support (rho bl_t, in seconds) dividing the coefficient's time index."""
+    return k_t / support_t
+
+
+def detector_coordinate(k, support):
+    """tr_solver_utils.py:311-316: x_T = k / (rho bl) on each detector axis."""
+    return k / support
+
+
+def boundary_amplitude(L, support, data_N, sigma):
+    """tr_solver_utils.py:409-416: prod_i sqrt(pi L_i / (rho bl_i N_i)) sigma_i."""
+    return np.prod(np.sqrt((np.pi * L[None, :]) / (support * data_N[None, :])) * sigma, axis=1)
+
+
 def _isclose(a, b):
     return abs(a - b) <= 1e-8 + 1e-5 * abs(b)
 
@@ -86,9 +102,10 @@ def tr_parameters(wpt: MSWPT, idx, data_N, data_dx, src_positions, src_N, src_dx
     sig = bl / 2.0
     signum = -np.sign(ch[:, 0])
     om = np.maximum(np.abs(tau), 1e-6)[:, 0]
-    t_start = multi[1, :] / Ll[:, 0]
+    t_start = start_time(multi[1, :], Ll[:, 0])
     xstar_idx = multi[1:, :] if d_sp == 1 else multi[2:, :]
-    xstar = xstar_idx.T / Ll[:, 1:] if d_sp > 1 else xstar_idx.T / Ll[:, 0:1]
+    xstar = (detector_coordinate(xstar_idx.T, Ll[:, 1:]) if d_sp > 1
+             else detector_coordinate(xstar_idx.T, Ll[:, 0:1]))
     B = len(idx)
     mask_tan = np.arange(d_sp) != normal_axis
     tan_index = np.where(mask_tan, np.cumsum(mask_tan) - 1, 0)
@@ -140,8 +157,7 @@ def tr_parameters(wpt: MSWPT, idx, data_N, data_dx, src_positions, src_N, src_dx
     else:
         with np.errstate(divide="ignore", invalid="ignore"):
             Mts = mT_inverse(xdot, Minit, pdot)
-    ats = np.prod(np.sqrt((np.pi * L[None, :]) / (Ll * np.asarray(data_N, float)[None, :])) * sig,
-                  axis=1)
+    ats = boundary_amplitude(L, Ll, np.asarray(data_N, float), sig)
     ats = ats * frame_phase(t, box, multi[1:, :].T, wpt.redundancy)
     graz = np.abs(pts[:, normal_axis]) == 0.0
     pts = np.where(graz[:, None], 1.0, pts)
@@ -209,12 +225,17 @@ def tr_field_sum(x, p, M, A, omega, points):
 
 def adjoint_pipeline(data, dt, data_wpt: MSWPT, data_N, data_dx, src_positions, src_N,
                      src_dx, c, points, thr, guard=5e-2, window=None, src_speed=None,
-                     speed=None):
+                     speed=None, selection=None, field_sum=None):
     """MSGBSolver.adjoint (msgb_solver.py:1299-1386): solve_hom_TR in a
     homogeneous medium; with ``src_speed`` / ``speed`` (the detector's and
     the image's ``oracle.medium.Speed``) the source weight c(sources)^2,
     the parameters at c(x_T), the ODE propagation and the c(x)^-2 image
-    weight of the heterogeneous path."""
+    weight of the heterogeneous path.
+
+    ``selection``: coefficient indices to use instead of this function's own
+    top_n (a caller comparing against another top_n whose order of exactly
+    conjugate pairs at rank N differs); ``field_sum``: a replacement for
+    ``tr_field_sum`` with the same signature (the C++ sum, for speed)."""
     from .topn import top_n
     points = np.asarray(points, float)
     src_positions = np.asarray(src_positions, float)
@@ -224,8 +245,12 @@ def adjoint_pipeline(data, dt, data_wpt: MSWPT, data_N, data_dx, src_positions, 
         c_src = src_speed(src_positions)[0]
     src = adjoint_source(data, dt, c_src, window)
     coeffs = data_wpt.forward(src, "spatial")
-    n_eff = min(int(thr), data_wpt.total_coeffs)
-    idx, val = top_n(coeffs, n_eff)
+    if selection is None:
+        n_eff = min(int(thr), data_wpt.total_coeffs)
+        idx, val = top_n(coeffs, n_eff)
+    else:
+        idx = np.asarray(selection, dtype=np.int64)
+        val = coeffs[idx]
     pts, Mts, xts, om, ats, signum, t_start = adj_parameters(
         data_wpt, idx, data_N, data_dx, src_positions, src_N, src_dx, c, guard, src_speed)
     ats = ats * val
@@ -235,8 +260,94 @@ def adjoint_pipeline(data, dt, data_wpt: MSWPT, data_N, data_dx, src_positions, 
     else:
         x, p, M, A = propagate_tr(xts, pts, Mts, ats, signum, t_start, speed)
         w = 1.0 / speed(points)[0] ** 2
-    q = tr_field_sum(x, p, M, A, om, points)
+    q = (tr_field_sum if field_sum is None else field_sum)(x, p, M, A, om, points)
     return q * w, (idx, val, x, p, M, A, om)
+
+
+def fourier_crop_time(data, nt_data):
+    """prepare_data_domain_for_msgb's crop (examples/reconstruction/
+    2d_time_reversal_and_adjoint.py:105-140): unitary FFT over every axis,
+    keep the middle ``nt_data`` rows of the shifted spectrum, unitary IFFT,
+    real part."""
+    F = np.fft.fftshift(np.fft.fftn(np.asarray(data, float), norm="ortho"))
+    mid = F.shape[0] // 2
+    F = F[mid - nt_data // 2: mid + nt_data // 2]
+    return np.fft.ifftn(np.fft.ifftshift(F), norm="ortho").real
+
+
+def four_edge_faces(N, dx):
+    """The four planar faces of a 2-D four-edge detector.
+
+    The sensors are the mask's nonzeros in row-major order
+    (geometry.py:575-590).  Each face lists the sensor numbers on one edge in
+    increasing tangential coordinate and a weight 1 / (number of edges the
+    sensor lies on), so a corner counts once over its two faces.  Column
+    faces (normal = axis 1) carry ``swap``: the reference's
+    ``compute_mT_linear_system`` treats axis 0 as the detector normal
+    (tr_solver_utils.py:143-149), so they are solved with the two spatial
+    axes exchanged."""
+    n0, n1 = N
+    mask = np.zeros(N, dtype=bool)
+    mask[0, :] = mask[-1, :] = mask[:, 0] = mask[:, -1] = True
+    rc = np.argwhere(mask)                      # row-major order = sensor order
+    edges = [(rc[:, 0] == 0), (rc[:, 0] == n0 - 1), (rc[:, 1] == 0), (rc[:, 1] == n1 - 1)]
+    count = np.sum(edges, axis=0)
+    faces = []
+    for e, (on, swap) in enumerate(zip(edges, (False, False, True, True))):
+        members = np.flatnonzero(on)
+        tan = rc[members, 0] if swap else rc[members, 1]
+        members = members[np.argsort(tan, kind="stable")]
+        faces.append(dict(members=members, weights=1.0 / count[members], swap=swap,
+                          far=e in (1, 3)))
+    return faces
+
+
+def four_edge_adjoint(data, N, dx, c, ts, nt_data, thr, guard=5e-2, selections=None,
+                      field_sum=None, num_boxes_levels=(4, 8), pixels=None):
+    """Principal-symbol adjoint of 2-D four-edge data (Nt, S) as the sum of
+    four single-face adjoints (MSGBSolver.adjoint, msgb_solver.py:1299-1386,
+    applied once per face).
+
+    Per face: the weighted columns, Fourier-cropped to ``nt_data`` time
+    samples; a data domain (nt_data, n_face) with dt' = span / (nt_data - 1)
+    and a 2-level MSWPT of aspect (nt_data // n, n_face // n), rho = 2,
+    rectangular_mirror (the example's recipe, 2d_time_reversal_and_adjoint.py:
+    141-165); the single-face adjoint on the image grid of the face's frame
+    (axes exchanged for a column face); the face image transposed back to the
+    physical frame.  ``selections``: per-face coefficient indices to use
+    instead of the oracle's own top_n; ``pixels``: row-major indices of the
+    image points to evaluate (all by default).  Returns (image, per-face
+    beams): the image has shape N, or one value per listed pixel."""
+    from .decomposition import Tiling
+    n0, n1 = N
+    ts = np.asarray(ts, float)
+    dt_data = float(ts[-1] - ts[0]) / (nt_data - 1)
+    flat = np.arange(n0 * n1) if pixels is None else np.asarray(pixels, dtype=np.int64)
+    ii, jj = np.unravel_index(flat, N)
+    phys = np.stack([ii * dx[0], jj * dx[1]], axis=1)
+    image = np.zeros(flat.size)
+    info = []
+    for k, f in enumerate(four_edge_faces(N, dx)):
+        sub = np.asarray(data, float)[:, f["members"]] * f["weights"][None, :]
+        face_data = fourier_crop_time(sub, nt_data)
+        nf = face_data.shape[1]
+        Nf = (n1, n0) if f["swap"] else (n0, n1)
+        dxf = (dx[1], dx[0]) if f["swap"] else tuple(dx)
+        normal = (Nf[0] - 1) * dxf[0] if f["far"] else 0.0
+        src = np.stack([np.full(nf, normal), np.arange(nf) * dxf[1]], axis=1)
+        nmin = min(nt_data, nf)
+        tiling = Tiling(2, (nt_data, nf), num_boxes_levels, (nt_data // nmin, nf // nmin))
+        wpt = MSWPT(tiling, 2, "rectangular_mirror")
+        # the image points in the face's frame: coordinates exchanged for a
+        # column face, so its image lands back on the physical pixels
+        pts = np.ascontiguousarray(phys[:, ::-1] if f["swap"] else phys)
+        sel = None if selections is None else selections[k]
+        q, beams = adjoint_pipeline(face_data, dt_data, wpt, (nt_data, nf), (dt_data, dxf[1]),
+                                    src, Nf, dxf, c, pts, thr, guard, selection=sel,
+                                    field_sum=field_sum)
+        image += q
+        info.append(beams)
+    return (image.reshape(N) if pixels is None else image), info
 
 
 def time_reversal_pipeline(data, data_wpt: MSWPT, data_N, data_dx, src_positions, src_N, src_dx,

@@ -85,6 +85,53 @@ def compute_frames(tiling, box, k, redundancy, windowing="rectangular"):
     return norm * ph * np.exp(-2 * np.pi * 1j * scaled) * g
 
 
+def sum_gsquare_windowed(tiling: Tiling, redundancy, windowing):
+    """transforms.py:511-555: sum over every box, in box order, of the
+    squared windowed filter.  Each filter is evaluated only where its window
+    can be non-zero (offsets |o| <= ceil(ws) + 1 from its centre on each axis,
+    wrapped): elsewhere it is exactly zero, so the sum equals the full-grid
+    sum of ``windowed_gaussian`` squares bit for bit (the same operations on
+    the same values, in the same order)."""
+    N = np.asarray(tiling.N)
+    Nf = N.astype(float)
+    d = tiling.ndim
+    asp = np.asarray(tiling.aspect)
+    sg = np.zeros(tiling.N)
+    for b in range(tiling.total_num_boxes):
+        lvl = int(tiling.find_level(b))
+        bl = tiling.box_lengths[lvl]
+        sigma = (bl // 2) * asp
+        ws = bl * asp * (redundancy / 2)
+        centre = tiling.centres[b]
+        idx, sq, win = [], [], []
+        for a in range(d):
+            r = int(np.ceil(ws[a])) + 1
+            m = np.unique((centre[a] + N[a] // 2 + np.arange(-r, r + 1)) % N[a])
+            mg = m - N[a] // 2                      # fourier_meshgrid value at index m
+            diff = np.mod(mg - centre[a] + Nf[a] / 2, Nf[a]) - Nf[a] / 2
+            shape = [1] * d
+            shape[a] = m.size
+            idx.append(m)
+            sq.append(np.square(diff / sigma[a]).reshape(shape))
+            if windowing == "rectangular":
+                w = (-ws[a] <= diff) & (diff < ws[a])
+            elif windowing == "rectangular_mirror":
+                w = np.abs(diff) < ws[a]
+            else:
+                raise ValueError(windowing)
+            win.append(w.reshape(shape))
+        tot = sq[0]
+        for a in range(1, d):
+            tot = tot + sq[a]
+        allw = win[0]
+        for a in range(1, d):
+            allw = allw & win[a]
+        g = np.exp(-tot) * allw.astype(float)
+        ix = np.ix_(*idx)
+        sg[ix] = sg[ix] + np.square(g)
+    return sg
+
+
 def _extract_centered_box(arr, shape, centre):
     """utils/arrays.py:237-268: periodic extraction of a centred box."""
     N = arr.shape
@@ -112,9 +159,7 @@ class MSWPT:
         self.windowing = windowing
         d = tiling.ndim
         # transforms.py:511-555: sum of squares over every box, box order.
-        sg = np.zeros(tiling.N)
-        for b in range(tiling.total_num_boxes):
-            sg = sg + np.square(filter_idx(b, tiling, redundancy, windowing))
+        sg = sum_gsquare_windowed(tiling, redundancy, windowing)
         if not np.all(sg > 0):
             raise ValueError("uncovered Fourier bins")
         self.sum_gsquare = sg

@@ -333,20 +333,33 @@ def test_culled_forward_matches_dense(gpu, d, kind):
     ref = cpu.forward_sum(dup(x0), dup(p0), dup(alpha), dup(a0), dup(om),
                           np.concatenate([np.ones(B), -np.ones(B)]), 1500.0, ts, pts)
     ts_d = _t(ts, gpu)
-    for prec, E, tol in (("fp64", 0.0, 1e-12), ("fp64", 40.0, 1e-10), ("fp32_split", 40.0, 1e-5)):
+    # every dropped term is below 2^-E of its beam's 2|A(t)| <= 2|a0|
+    # (|1 + c t ap| >= 1 for the closed-form beams): the certified bound
+    amp_sum = 2.0 * 2.0 * np.sum(np.abs(a0))
+    # (precision, E, rel-L2 tolerance, beam split): the cases without a
+    # workspace have no beam split, so each block works through all
+    # ceil(2 B / 256) = 3 record chunks (the masks' double buffer switches)
+    for prec, E, tol, split in (("fp64", 0.0, 1e-12, False), ("fp64", 40.0, 1e-10, True),
+                                ("fp32_split", 40.0, 1e-5, True), ("fp64", 32.0, None, True),
+                                ("fp32_split", 32.0, 1e-5, True), ("fp64", 32.0, None, False),
+                                ("fp32_split", 32.0, 1e-5, False)):
         out = torch.full((ts.size, pts.shape[0]), np.nan,
                          dtype=torch.float64 if prec == "fp64" else torch.float32, device=gpu)
         cnt = torch.zeros(1, dtype=torch.uint64, device=gpu)
         ss = seg.struct()
-        # the first case runs without workspace (no beam split), the others split
-        ws = (None if E == 0.0 else
-              _lib.workspace(_lib.load().bh_forward_sum_culled_workspace_size(ts.size, seg.S), gpu))
+        ws = (_lib.workspace(_lib.load().bh_forward_sum_culled_workspace_size(ts.size, seg.S), gpu)
+              if split else None)
         _lib.call("bh_forward_sum_culled", C.byref(fb.struct()), d, 1, 1500.0, _lib.ptr(ts_d),
                   ts.size, C.byref(ss), E, _lib.PRECISION[prec], _lib.ptr(out), _lib.ptr(cnt),
                   _lib.ptr(ws), 0 if ws is None else ws.numel(), _lib.stream_handle())
         got = _n(out).astype(np.float64)
         assert np.all(np.isfinite(got))
-        assert rel(ref, got) < tol
+        if E > 0:
+            bound = 2.0 ** -E * amp_sum
+            slack = 0.0 if prec == "fp64" else 1e-5 * np.max(np.abs(ref))
+            assert np.max(np.abs(got - ref)) <= bound + 1e-12 * np.max(np.abs(ref)) + slack
+        if tol is not None:
+            assert rel(ref, got) < tol
         pairs = int(cnt.item())
         dense_pairs = 2 * B * ts.size * seg.ngroups
         assert 0 < pairs <= dense_pairs
@@ -365,7 +378,7 @@ def test_culled_static_matches_dense(gpu, d):
     n = 128 if d == 2 else 32
     dx = 1e-4
     L = n * dx
-    B = 400
+    B = 800
     x0, p0, om, alpha, a0 = _random_beams(rng, B, d, L, 0.5 / dx)
     xt, pt, Mt, At = obeams.solve_hom_diag(x0, p0, alpha, a0, np.ones(B), np.array([2e-6]), 1500.0)
     sb = StaticBeams(B, d, gpu)
@@ -374,15 +387,30 @@ def test_culled_static_matches_dense(gpu, d):
     pts = _grid_points("grid", n, d, dx)
     ref = cpu.static_sum(xt[:, 0], pt[:, 0], Mt[:, 0], At[:, 0], om, pts) * 2.0
     seg = Segments(pts, [dx] * d, gpu)
-    for prec, E, tol in (("fp64", 0.0, 1e-12), ("fp64", 40.0, 1e-10), ("fp32_split", 40.0, 1e-5)):
+    bound_sum = 2.0 * 2.0 * np.sum(np.abs(At[:, 0]))
+    full = _lib.load().bh_static_sum_culled_workspace_size(B, d, seg.S)
+    # the record table alone (rounded up to 256 bytes): no beam split, every
+    # block works through ceil(800 / 256) = 4 record chunks
+    recs_only = (_lib.load().bh_static_sum_workspace_size(B, d) + 255) // 256 * 256
+    for prec, E, tol, nbytes in (("fp64", 0.0, 1e-12, full), ("fp64", 40.0, 1e-10, full),
+                                 ("fp32_split", 40.0, 1e-5, full), ("fp64", 32.0, None, full),
+                                 ("fp32_split", 32.0, 1e-5, full),
+                                 ("fp64", 32.0, None, recs_only),
+                                 ("fp32_split", 32.0, 1e-5, recs_only)):
         out = torch.empty(pts.shape[0], dtype=torch.float64 if prec == "fp64" else torch.float32,
                           device=gpu)
-        ws = _lib.workspace(_lib.load().bh_static_sum_culled_workspace_size(B, d, seg.S), gpu)
+        ws = _lib.workspace(nbytes, gpu)
         ss = seg.struct()
         _lib.call("bh_static_sum_culled", C.byref(sb.struct()), d, C.byref(ss), 2.0, E,
-                  _lib.PRECISION[prec], _lib.ptr(out), None, _lib.ptr(ws), ws.numel(),
+                  _lib.PRECISION[prec], _lib.ptr(out), None, _lib.ptr(ws), nbytes,
                   _lib.stream_handle())
-        assert rel(ref, _n(out).astype(np.float64)) < tol
+        got = _n(out).astype(np.float64)
+        if E > 0:
+            slack = 0.0 if prec == "fp64" else 1e-5 * np.max(np.abs(ref))
+            assert (np.max(np.abs(got - ref))
+                    <= 2.0 ** -E * bound_sum + 1e-12 * np.max(np.abs(ref)) + slack)
+        if tol is not None:
+            assert rel(ref, got) < tol
 
 
 # ---------------------------------------------------------------------------

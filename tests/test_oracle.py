@@ -67,6 +67,20 @@ def test_filters_bounded_and_dual_partition_of_unity(cfg):
     assert np.max(np.abs(tot - 1)) < 1e-13
 
 
+@pytest.mark.parametrize("cfg", CFGS + [(2, (256, 64), (4, 8), (4, 1))])
+@pytest.mark.parametrize("redundancy", [1, 2])
+@pytest.mark.parametrize("windowing", ["rectangular", "rectangular_mirror"])
+def test_windowed_sum_gsquare_equals_full_grid_sum(cfg, redundancy, windowing):
+    """The oracle's support-restricted sum of squares equals the literal
+    full-grid sum over every box (transforms.py:511-555) bit for bit."""
+    from oracle.transforms import sum_gsquare_windowed
+    t = Tiling(*cfg)
+    ref = np.zeros(t.N)
+    for b in range(t.total_num_boxes):
+        ref = ref + np.square(filter_idx(b, t, redundancy, windowing))
+    assert np.array_equal(ref, sum_gsquare_windowed(t, redundancy, windowing))
+
+
 @pytest.mark.parametrize("cfg", CFGS)
 @pytest.mark.parametrize("windowing", ["rectangular", "rectangular_mirror"])
 def test_inverse_of_forward_is_identity(cfg, windowing):
