@@ -7,6 +7,7 @@ when it is missing: there is no CPU fallback for any operator.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from pathlib import Path
 
 import numpy as np
@@ -72,6 +73,7 @@ _SZ = C.c_size_t
 SIGNATURES = {
     "bh_last_error": (C.c_char_p, []),
     "bh_version": (C.c_int, []),
+    "bh_device_status": (C.c_int, [C.c_int, C.POINTER(C.c_uint32), _P]),
     "bh_fft_workspace_size": (_SZ, [_I64, _I64]),
     "bh_fft": (C.c_int, [_P, _I64, _I64, _I64, _I64, _I64, _I64, C.c_int, C.c_double, _P, _SZ, _P]),
     "bh_mswpt_setup": (C.c_int, [C.POINTER(Tiling), _P, _P, _P]),
@@ -181,10 +183,24 @@ def load():
     return lib
 
 
+# Tensors whose raw addresses were taken by ``ptr`` since the last ``call``
+# returned, per thread.  A pointer argument written as ptr(make_tensor(...))
+# would otherwise let the temporary die before the call is made, and the
+# caching allocator could hand its memory to the next argument's temporary.
+_held = threading.local()
+
+
 def call(name, *args):
-    """Call C-ABI function `name`; raise BeamaxHIPError on a non-zero status."""
+    """Call C-ABI function `name`; raise BeamaxHIPError on a non-zero status.
+
+    Every tensor passed through ``ptr`` for this call stays referenced until
+    the C function has returned (its kernels are then enqueued, and the
+    stream orders any later reuse of the memory after them)."""
     lib = load()
-    rc = getattr(lib, name)(*args)
+    try:
+        rc = getattr(lib, name)(*args)
+    finally:
+        _held.tensors = []
     if rc != 0:
         msg = lib.bh_last_error().decode(errors="replace")
         if rc == -1:
@@ -194,8 +210,35 @@ def call(name, *args):
 
 
 def ptr(t):
-    """Raw device pointer of a torch tensor (None -> NULL)."""
-    return None if t is None else C.c_void_p(t.data_ptr())
+    """Raw device pointer of a torch tensor (None -> NULL); the tensor is held
+    until the next ``call`` on this thread returns."""
+    if t is None:
+        return None
+    held = getattr(_held, "tensors", None)
+    if held is None:
+        held = _held.tensors = []
+    held.append(t)
+    return C.c_void_p(t.data_ptr())
+
+
+STATUS_BITS = {1: "forward parameter kernel met a beam index outside [-1, total)",
+               2: "adjoint / time-reversal parameter kernel met a beam index outside [-1, total)",
+               4: "coefficient scatter/gather met an index outside [-1, total)"}
+
+
+def device_status(clear: bool = True) -> int:
+    """The index-guard status bits of the current device (synchronises the
+    current stream); 0 when no kernel met an out-of-range index."""
+    bits = C.c_uint32(0)
+    call("bh_device_status", int(clear), C.byref(bits), stream_handle())
+    return int(bits.value)
+
+
+def check_device_status() -> None:
+    """Raise BeamaxHIPError if any index guard fired since the last check."""
+    bits = device_status(clear=True)
+    if bits:
+        raise BeamaxHIPError("; ".join(m for b, m in STATUS_BITS.items() if bits & b))
 
 
 def stream_handle():

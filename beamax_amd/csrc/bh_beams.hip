@@ -19,6 +19,15 @@
 
 namespace {
 
+// Per-device status word of this translation unit's kernels (bit
+// BH_STATUS_FWD_INDEX / BH_STATUS_ADJ_INDEX: a beam index outside [-1, total)
+// reached a parameter kernel).  Read and cleared by bh_device_status.
+__device__ unsigned int g_status = 0u;
+
+__device__ inline void flag_status(unsigned bit) { atomicOr(&g_status, bit); }
+
+constexpr double kQuietNaN = __builtin_nan("");
+
 using bh::TilingDev;
 
 // ---------------------------------------------------------------------------
@@ -75,15 +84,20 @@ __global__ __launch_bounds__(128) void k_forward_params(TilingDev T, const int32
   const double L[3] = {L0, L1, L2};
   const long long flat = idx[i];
   if (flat < 0 || flat >= T.coeffs_cumsum[T.num_levels]) {
-    // index outside the coefficient vector: an inert beam (zero amplitude)
+    // -1 is the reference's padding index (zero coefficient): an inert beam
+    // of zero amplitude.  Any other index outside the coefficient vector is
+    // a caller error: the beam's amplitude is NaN, so every output it
+    // touches is NaN, and the status word records it.
+    const double amp = flat == -1 ? 0.0 : kQuietNaN;
+    if (flat != -1) flag_status(BH_STATUS_FWD_INDEX);
     for (int a = 0; a < d; ++a) {
       out.p0[i * d + a] = a == 0 ? bh::kTwoPi : 0.0;
       out.x0[i * d + a] = 0.0;
       out.alpha[2 * (i * d + a)] = 0.0;
       out.alpha[2 * (i * d + a) + 1] = 1.0;
     }
-    out.a0[2 * i] = 0.0;
-    out.a0[2 * i + 1] = 0.0;
+    out.a0[2 * i] = amp;
+    out.a0[2 * i + 1] = amp;
     out.omega[i] = 1.0;
     return;
   }
@@ -306,7 +320,10 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
   const TilingDev& T = A.T;
   const long long flat = A.idx[i];
   if (flat < 0 || flat >= T.coeffs_cumsum[T.num_levels]) {
-    // index outside the coefficient vector: an inert beam (zero amplitude)
+    // -1: an inert padding beam; any other index outside the coefficient
+    // vector: NaN amplitude and the status word (as in k_forward_params)
+    const double amp = flat == -1 ? 0.0 : kQuietNaN;
+    if (flat != -1) flag_status(BH_STATUS_ADJ_INDEX);
     for (int a = 0; a < D; ++a) {
       A.out.x[i * D + a] = 0.0;
       A.out.p[i * D + a] = 1.0;
@@ -315,8 +332,8 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
         A.out.M[2 * ((i * D + a) * D + b) + 1] = a == b ? 1.0 : 0.0;
       }
     }
-    A.out.A[2 * i] = 0.0;
-    A.out.A[2 * i + 1] = 0.0;
+    A.out.A[2 * i] = amp;
+    A.out.A[2 * i + 1] = amp;
     A.out.omega[i] = 1.0;
     if (A.t_start) {
       A.t_start[i] = 0.0;
@@ -833,5 +850,34 @@ extern "C" int bh_static_sum(const bh_static_beams* b, int ndim, const double* p
   else { if (f64) BH_STATIC(3, true); else BH_STATIC(3, false); }
 #undef BH_STATIC
   BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+namespace bh {
+int beams_status(int clear, unsigned* bits, hipStream_t s) {
+  unsigned v = 0u;
+  if (hipMemcpyFromSymbolAsync(&v, HIP_SYMBOL(g_status), sizeof(v), 0, hipMemcpyDeviceToHost,
+                               s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    BH_FAIL(BH_ERR_HIP, "bh_device_status: reading the status word failed");
+  if (clear && v) {
+    const unsigned zero = 0u;
+    if (hipMemcpyToSymbolAsync(HIP_SYMBOL(g_status), &zero, sizeof(zero), 0,
+                               hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      BH_FAIL(BH_ERR_HIP, "bh_device_status: clearing the status word failed");
+  }
+  *bits |= v;
+  return BH_OK;
+}
+}  // namespace bh
+
+extern "C" int bh_device_status(int clear, uint32_t* status, bh_stream s) {
+  bh::clear_error();
+  if (!status) BH_FAIL(BH_ERR_ARG, "bh_device_status: null status");
+  unsigned bits = 0u;
+  if (int rc = bh::beams_status(clear, &bits, (hipStream_t)s)) return rc;
+  if (int rc = bh::transpose_status(clear, &bits, (hipStream_t)s)) return rc;
+  *status = bits;
   return BH_OK;
 }

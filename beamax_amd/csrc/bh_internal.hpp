@@ -57,4 +57,10 @@ size_t kth_key_ws_bytes();
 void kth_largest_key(const unsigned long long* keys, long long n, long long k,
                      unsigned long long* out, void* ws, hipStream_t s);
 
+// Status words of the kernels that guard beam / coefficient indices (one per
+// translation unit): OR the bits into *bits, clear them if asked.  Both
+// synchronise the stream.
+int beams_status(int clear, unsigned* bits, hipStream_t s);
+int transpose_status(int clear, unsigned* bits, hipStream_t s);
+
 }  // namespace bh

@@ -156,14 +156,22 @@ __global__ void k_fill_zero(double* __restrict__ p, long long n) {
   if (i < n) p[i] = 0.0;
 }
 
-// out[idx[i]] = scale * vals[i]; an index outside [0, total) is skipped (the
-// same guard as the parameter kernels, which make such beams inert)
+// status word of this translation unit (BH_STATUS_COEFF_INDEX), read and
+// cleared by bh_device_status
+__device__ unsigned int g_status = 0u;
+
+// out[idx[i]] = scale * vals[i]; the padding index -1 is skipped (the
+// parameter kernels make such beams inert), any other index outside
+// [0, total) is skipped and recorded in the status word
 __global__ void k_scatter(const int64_t* __restrict__ idx, const double* __restrict__ vals,
                           long long k, double scale, long long total, double* __restrict__ out) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= k) return;
   const int64_t j = idx[i];
-  if (j < 0 || j >= total) return;
+  if (j < 0 || j >= total) {
+    if (j != -1) atomicOr(&g_status, (unsigned)BH_STATUS_COEFF_INDEX);
+    return;
+  }
   out[2 * j] = scale * vals[2 * i];
   out[2 * j + 1] = scale * vals[2 * i + 1];
 }
@@ -175,6 +183,7 @@ __global__ void k_gather(const double* __restrict__ coeffs, long long total,
   if (i >= k) return;
   const int64_t j = idx[i];
   const bool ok = j >= 0 && j < total;
+  if (!ok && j != -1) atomicOr(&g_status, (unsigned)BH_STATUS_COEFF_INDEX);
   vals[2 * i] = ok ? coeffs[2 * j] : 0.0;
   vals[2 * i + 1] = ok ? coeffs[2 * j + 1] : 0.0;
 }
@@ -290,3 +299,22 @@ extern "C" int bh_coeff_gather(const double* coeffs, int64_t total, const int64_
   BH_LAUNCH_CHECK();
   return BH_OK;
 }
+
+namespace bh {
+int transpose_status(int clear, unsigned* bits, hipStream_t s) {
+  unsigned v = 0u;
+  if (hipMemcpyFromSymbolAsync(&v, HIP_SYMBOL(g_status), sizeof(v), 0, hipMemcpyDeviceToHost,
+                               s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    BH_FAIL(BH_ERR_HIP, "bh_device_status: reading the status word failed");
+  if (clear && v) {
+    const unsigned zero = 0u;
+    if (hipMemcpyToSymbolAsync(HIP_SYMBOL(g_status), &zero, sizeof(zero), 0,
+                               hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      BH_FAIL(BH_ERR_HIP, "bh_device_status: clearing the status word failed");
+  }
+  *bits |= v;
+  return BH_OK;
+}
+}  // namespace bh

@@ -78,6 +78,15 @@ typedef struct ihipStream_t* bh_stream;
 #define BH_ERR_UNSUPPORTED -3
 #define BH_ERR_WORKSPACE -4
 
+/* Bits of bh_device_status: a kernel met a beam or coefficient index outside
+ * [-1, total) (-1 is the reference's padding index of an empty threshold slot,
+ * whose value is 0: forward_solver_utils.py:58-61, 174-177; it stays an inert
+ * zero-amplitude beam).
+ * Such a beam gets a NaN amplitude, so the outputs it reaches are NaN. */
+#define BH_STATUS_FWD_INDEX 1u   /* bh_forward_params */
+#define BH_STATUS_ADJ_INDEX 2u   /* bh_adjoint_params, bh_tr_params, bh_boundary_terminal */
+#define BH_STATUS_COEFF_INDEX 4u /* bh_coeff_scatter, bh_coeff_gather */
+
 #define BH_MAX_DIM 3
 #define BH_MAX_LEVELS 12
 
@@ -131,6 +140,13 @@ typedef struct {
 
 const char* bh_last_error(void);
 int bh_version(void);
+
+/* The status bits the index guards of the current device's kernels have set
+ * since the last clear (BH_STATUS_*), OR-ed into *status; clear != 0 resets
+ * them.  Synchronises stream s (not capturable in a graph).  No reference
+ * counterpart: it reports caller errors (an index the reference's thresholds
+ * never produce) that the asynchronous kernels cannot return as a status. */
+int bh_device_status(int clear, uint32_t* status, bh_stream s);
 
 /* Batched 1-D complex FFT (FP64) along one strided axis, in place.
  * Transform t in [0,batch) starts at (t/inner)*outer_stride +

@@ -20,3 +20,17 @@ def gpu():
     from beamax_amd import _lib
     _lib.load()
     return torch.device("cuda", 0)
+
+
+@pytest.fixture(autouse=True)
+def _index_guards_stay_quiet(request):
+    """After every GPU test: no kernel met an out-of-range beam or
+    coefficient index (the guards' status word is still clear)."""
+    yield
+    if request.node.get_closest_marker("gpu") is None:
+        return
+    import torch
+    if not torch.cuda.is_available() or request.node.name.startswith("test_index_guard"):
+        return
+    from beamax_amd import _lib
+    _lib.check_device_status()

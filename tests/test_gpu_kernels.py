@@ -180,6 +180,32 @@ def test_forward_params_match_oracle(gpu, cfg):
     assert rel(a0 * vals, _n(fb.a0)) < 1e-13
 
 
+def test_index_guard_flags_out_of_range_beams(gpu):
+    """-1 (the reference's padding index, value 0) gives an inert beam; an
+    index past the coefficient vector gives a NaN beam and sets the status
+    word, which bh_device_status reports and clears.  The guard is tested
+    before the index is used, so no kernel reads outside its tables."""
+    from beamax_amd.solvers.msgb_solver import ForwardBeams
+    from beamax_amd import _lib
+    w, o = _pair((2, (64, 32), (4, 8), (2, 1)))
+    L = np.asarray((64, 32)) * 1e-4
+    assert _lib.device_status(clear=True) == 0
+    idx = np.array([3, -1, w.total_coeffs + 5, 7], dtype=np.int64)
+    vals = np.ones(4, dtype=complex)
+    fb = ForwardBeams(4, 2, gpu)
+    _lib.call("bh_forward_params", C.byref(w.tiling), _lib.dbl_array(L), _lib.ptr(_t(idx, gpu)),
+              _lib.ptr(_t(vals, gpu)), 4, fb.struct(), _lib.stream_handle())
+    a0 = _n(fb.a0)
+    assert np.all(np.isfinite(a0[[0, 3]])) and a0[1] == 0 and np.isnan(a0[2])
+    assert _lib.device_status(clear=True) == 1
+    assert _lib.device_status(clear=True) == 0
+    with pytest.raises(_lib.BeamaxHIPError, match="forward parameter kernel"):
+        _lib.call("bh_forward_params", C.byref(w.tiling), _lib.dbl_array(L),
+                  _lib.ptr(_t(idx[2:3], gpu)), _lib.ptr(_t(vals[:1], gpu)), 1,
+                  fb.struct(0, 1), _lib.stream_handle())
+        _lib.check_device_status()
+
+
 @pytest.mark.parametrize("d", [1, 2, 3])
 def test_solve_hom_diag_matches_oracle(gpu, d):
     from beamax_amd.gb import gb_solvers

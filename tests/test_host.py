@@ -121,3 +121,22 @@ def test_biased_phase_fraction_extraction():
         d = abs((float(x) - 1.0) - frac)
         worst = max(worst, min(d, 1.0 - d))
     assert worst <= 2.0 ** -24 + 2.0 ** -30
+
+
+def test_ptr_holds_temporaries_until_the_call_returns():
+    """A tensor made inside the argument list of a C-ABI call lives until the
+    call has returned (no GPU needed: the call fails its argument check)."""
+    import gc
+    import weakref
+    import torch
+    from beamax_amd import _lib
+    t = torch.empty(8)
+    ref = weakref.ref(t)
+    p = _lib.ptr(t)
+    del t
+    gc.collect()
+    assert ref() is not None and p.value == ref().data_ptr()
+    with pytest.raises(ValueError, match="null status"):
+        _lib.call("bh_device_status", 0, None, None)
+    gc.collect()
+    assert ref() is None
