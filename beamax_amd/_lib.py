@@ -86,6 +86,7 @@ SIGNATURES = {
     "bh_threshold": (C.c_int, [_P, _I64, C.c_int, C.c_double, C.POINTER(Tiling), _P, _P, _P, _P, _SZ,
                                _P]),
     "bh_topn": (C.c_int, [_P, _I64, _I64, _P, _P, _P, _SZ, _P]),
+    "bh_topn_noalloc": (C.c_int, [_P, _I64, _I64, _P, _P, _P]),
     "bh_forward_params": (C.c_int, [C.POINTER(Tiling), C.POINTER(C.c_double), _P, _P, _I64,
                                     FwdBeams, _P]),
     "bh_forward_sum": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, C.c_double, _P, _I64, _P,

@@ -13,6 +13,8 @@
 //    tiles, then stable merge levels (one thread per element, its output slot
 //    from a binary search in the partner run).
 // 5. Gather of indices and values.
+#include <mutex>
+
 #include "bh_internal.hpp"
 
 namespace {
@@ -367,15 +369,11 @@ extern "C" size_t bh_topn_workspace_size(int64_t n, int64_t k) {
   return w.bytes;
 }
 
-extern "C" int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out,
-                       double* val_out, void* ws, size_t ws_bytes, bh_stream s) {
-  bh::clear_error();
-  if (n <= 0) BH_FAIL(BH_ERR_ARG, "bh_topn: empty input (n=%lld)", (long long)n);
-  if (k <= 0) BH_FAIL(BH_ERR_ARG, "top_n threshold must be a positive integer.");
-  if (k > n) k = n;
-  if (!c || !idx_out || !val_out || !ws) BH_FAIL(BH_ERR_ARG, "bh_topn: null pointer");
-  const size_t need = bh_topn_workspace_size(n, k);
-  if (ws_bytes < need) BH_FAIL(BH_ERR_WORKSPACE, "bh_topn: workspace %zu < %zu", ws_bytes, need);
+namespace {
+// The selection on a workspace the caller has checked (k <= n, sized by
+// bh_topn_workspace_size).
+int topn_run(const double* c, int64_t n, int64_t k, int64_t* idx_out, double* val_out, void* ws,
+             bh_stream s) {
   hipStream_t st = (hipStream_t)s;
   TopnWs w = carve(ws, n, k);
   const long long nblk = (n + kChunk - 1) / kChunk;
@@ -415,4 +413,80 @@ extern "C" int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out,
                      (double2*)val_out);
   BH_LAUNCH_CHECK();
   return BH_OK;
+}
+}  // namespace
+
+extern "C" int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out,
+                       double* val_out, void* ws, size_t ws_bytes, bh_stream s) {
+  bh::clear_error();
+  if (n <= 0) BH_FAIL(BH_ERR_ARG, "bh_topn: empty input (n=%lld)", (long long)n);
+  if (k <= 0) BH_FAIL(BH_ERR_ARG, "top_n threshold must be a positive integer.");
+  if (k > n) k = n;
+  if (!c || !idx_out || !val_out || !ws) BH_FAIL(BH_ERR_ARG, "bh_topn: null pointer");
+  const size_t need = bh_topn_workspace_size(n, k);
+  if (ws_bytes < need) BH_FAIL(BH_ERR_WORKSPACE, "bh_topn: workspace %zu < %zu", ws_bytes, need);
+  return topn_run(c, n, k, idx_out, val_out, ws, s);
+}
+
+// ---------------------------------------------------------------------------
+// bh_topn_noalloc: the same selection with its workspace in a fixed device
+// array of this library (no allocation by the library or the caller).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long long kNoallocMaxN = 1ll << 22;   // coefficients
+constexpr long long kNoallocMaxK = 1ll << 18;   // selected entries
+
+constexpr size_t round256(size_t x) { return (x + 255) / 256 * 256; }
+// carve()'s layout for n keys and a sort of P (a power of two) entries
+constexpr size_t topn_layout_bytes(long long n, long long P) {
+  return round256((size_t)n * 8) + 4 * round256((size_t)P * 8) +
+         round256((size_t)2 * ((n + kChunk - 1) / kChunk) * 8) + round256(256 * 4) +
+         round256(sizeof(SelState));
+}
+constexpr size_t kScratchBytes = topn_layout_bytes(kNoallocMaxN, kNoallocMaxK);
+
+// One copy per device, held while the library stays loaded.  Host code never
+// uses this name as an address: the device address comes from
+// hipGetSymbolAddress, once per device (scratch_address).
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+int scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0)
+    BH_FAIL(BH_ERR_HIP, "bh_topn_noalloc: no current device");
+  if (dev >= kMaxDevices)
+    BH_FAIL(BH_ERR_UNSUPPORTED, "bh_topn_noalloc: device %d >= %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      BH_FAIL(BH_ERR_HIP, "bh_topn_noalloc: hipGetSymbolAddress failed on device %d: %s", dev,
+              hipGetErrorString(e));
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return BH_OK;
+}
+}  // namespace
+
+extern "C" int bh_topn_noalloc(const double* c, int64_t n, int64_t k, int64_t* idx_out,
+                               double* val_out, bh_stream s) {
+  bh::clear_error();
+  if (n <= 0) BH_FAIL(BH_ERR_ARG, "bh_topn_noalloc: empty input (n=%lld)", (long long)n);
+  if (k <= 0) BH_FAIL(BH_ERR_ARG, "top_n threshold must be a positive integer.");
+  if (k > n) k = n;
+  if (!c || !idx_out || !val_out) BH_FAIL(BH_ERR_ARG, "bh_topn_noalloc: null pointer");
+  if (n > kNoallocMaxN || k > kNoallocMaxK || bh_topn_workspace_size(n, k) > kScratchBytes)
+    BH_FAIL(BH_ERR_UNSUPPORTED,
+            "bh_topn_noalloc: n=%lld, k=%lld exceed its fixed scratch (n <= %lld, k <= %lld); "
+            "use bh_topn with a workspace",
+            (long long)n, (long long)k, kNoallocMaxN, kNoallocMaxK);
+  void* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  return topn_run(c, n, k, idx_out, val_out, ws, s);
 }

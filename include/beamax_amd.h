@@ -186,6 +186,17 @@ int bh_mswpt_inverse(const bh_tiling* t, const double* gpacked, const double* co
 size_t bh_topn_workspace_size(int64_t n, int64_t k);
 int bh_topn(const double* c, int64_t n, int64_t k, int64_t* idx_out, double* val_out,
             void* ws, size_t ws_bytes, bh_stream s);
+/* bh_topn without a workspace argument and without any allocation: its
+ * workspace is a fixed array of this library in each device's memory
+ * (about 40 MiB, held while the library is loaded), for callers that cannot
+ * allocate device memory (code under HIP graph capture, allocation-free
+ * serving loops).  No counterpart in the reference.  Limits: n <= 4194304,
+ * k <= 262144 (BH_ERR_UNSUPPORTED beyond).  Every call on a device shares
+ * that device's array: calls that overlap on one device (other streams,
+ * other threads) race and must be serialised by the caller.  The first call
+ * on a device looks its array up (hipGetSymbolAddress). */
+int bh_topn_noalloc(const double* c, int64_t n, int64_t k, int64_t* idx_out, double* val_out,
+                    bh_stream s);
 
 /* The other thresholding strategies (forward_solver_utils.py:14-178):
  * strategy 0 hard (|c| > val), 1 percentile (|c| > the val-th percentile of
