@@ -253,6 +253,78 @@ def test_cpp_port_matches_numpy():
     assert np.max(np.abs(st - ref[2])) < 1e-13 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("E", [8.0, 20.0, 32.0, 1e6])
+def test_cut_sum_drops_only_terms_below_the_bound(E):
+    """The CPU forward sum with the envelope cut over sensor runs: against a
+    NumPy sum masked term by term (exponent <= E), and against the dense
+    sum within the certified bound 2^-E sum_b 2|A_b(t)| (|A(t)| <= |a0| for
+    the closed form); a large E keeps every term."""
+    rng = np.random.default_rng(6)
+    d, B, n = 2, 60, 24
+    dx = 2e-4
+    xi = rng.standard_normal((B, d)) * 1500
+    om = np.linalg.norm(xi, axis=1)
+    p = 2 * np.pi * xi / om[:, None]
+    alpha = 1j * (rng.random((B, d)) + 0.3) * 2 * (np.pi * 400) ** 2 / om[:, None]
+    a0 = rng.standard_normal(B) + 1j * rng.standard_normal(B)
+    x0 = rng.random((B, d)) * n * dx
+    mode = np.where(rng.random(B) < 0.5, 1.0, -1.0)
+    ts = np.linspace(0, 3e-6, 9)
+    m = np.zeros((n, n))
+    m[0] = m[-1] = m[:, 0] = m[:, -1] = 1
+    sens = np.argwhere(m) * dx
+    runs = ocpu.point_runs(sens, (dx, dx))
+    assert len(runs[2]) - 1 == 4
+    got, terms = ocpu.forward_sum_runs(x0, p, alpha, a0, om, mode, 1500.0, ts, runs,
+                                       sens.shape[0], E, nthreads=3)
+    dense = ocpu.forward_sum(x0, p, alpha, a0, om, mode, 1500.0, ts, sens)
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p, alpha, a0, mode, ts, 1500.0)
+    ref = np.zeros_like(dense)
+    kept = 0
+    for b in range(B):
+        dl = sens[None] - xt[b][:, None, :]
+        expo = 0.5 * om[b] * np.log2(np.e) * np.einsum("tsi,tij,tsj->ts", dl, Mt[b].imag, dl)
+        keep = expo <= E
+        kept += int(keep.sum())
+        term = obeams.real_beam_sum(xt[b:b + 1], pt[b:b + 1], Mt[b:b + 1], At[b:b + 1],
+                                    om[b:b + 1], sens)
+        ref += np.where(keep, term, 0.0)
+    scale = np.max(np.abs(dense))
+    # terms within rounding of the threshold may fall either side: each is
+    # at most 2^-E of its beam's 2|A|
+    bound = 2.0 ** -E * 2 * np.sum(np.abs(a0))
+    assert np.max(np.abs(got - ref)) <= 1e-13 * scale + 2 * bound * 1e-6
+    assert abs(terms - kept) <= max(2, kept // 10 ** 6)
+    assert np.max(np.abs(got - dense)) <= bound + 1e-13 * scale
+    if E == 1e6:
+        assert terms == B * ts.size * sens.shape[0]
+
+
+@pytest.mark.parametrize("E", [10.0, 32.0, 1e6])
+def test_static_cut_sum_within_the_bound(E):
+    """The static (image) sum with the cut over image rows, against the dense
+    C++ static sum: within 2^-E sum_b 2|A_b|, every term kept at a huge E."""
+    rng = np.random.default_rng(7)
+    B, d, n, dx = 50, 2, 30, 1e-4
+    xi = rng.standard_normal((B, d)) * 2000
+    om = np.linalg.norm(xi, axis=1)
+    p = 2 * np.pi * xi / om[:, None]
+    al = 1j * (rng.random((B, d)) + 0.3) * 2 * (np.pi * 500) ** 2 / om[:, None]
+    a0 = rng.standard_normal(B) + 1j * rng.standard_normal(B)
+    x0 = rng.random((B, d)) * n * dx
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p, al, a0, np.ones(B), np.array([1e-6]), 1500.0)
+    pts = np.stack(np.nonzero(np.ones((n, n))), 1) * dx
+    runs = ocpu.point_runs(pts, (dx, dx))
+    got, terms = ocpu.static_sum_runs(xt[:, 0], pt[:, 0], Mt[:, 0], At[:, 0], om, runs,
+                                      pts.shape[0], E, nthreads=3)
+    ref = ocpu.static_sum(xt[:, 0], pt[:, 0], Mt[:, 0], At[:, 0], om, pts)
+    bound = 2.0 ** -E * 2 * np.sum(np.abs(At[:, 0]))
+    assert np.max(np.abs(got - ref)) <= bound + 1e-13 * np.max(np.abs(ref))
+    assert terms <= B * pts.shape[0]
+    if E == 1e6:
+        assert terms == B * pts.shape[0]
+
+
 # ---------------------------------------------------------------------------
 # top_n (forward_solver_utils.py:65-89; test_msgb_solver_misc.py:352-389, 515-558)
 # ---------------------------------------------------------------------------
