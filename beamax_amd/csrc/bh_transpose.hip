@@ -137,6 +137,186 @@ __global__ __launch_bounds__(256) void k_fwd_transpose(TArgs A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32-split form by per-pair quadratics (the forward's evaluation turned
+// around).  Work unit: 64 time rows (one per lane) x up to 32 units of 128
+// points (two adjacent groups of a run, the forward's culling unit).  A wave
+// takes one beam at a time: lane = row builds the row's record and a 32-bit
+// mask of the units inside the certified envelope (the forward's intervals,
+// so both sums hold the same terms); the active (row, unit) pairs are
+// compacted into an LDS list in (row, unit) order; then in passes of 64 each
+// lane forms one pair's phase/envelope quadratic in the point index
+// (pair_quad) and the wave walks the pass, lane l taking the unit's points l
+// and l + 64, weighting each term by the residual (gathered into group order
+// once per call, zero on padding slots) and accumulating r 2|A| 2^-e (cos,
+// -sin)(2 pi cycles) -- r times the conjugate of the forward's complex term.
+// Per lane the pass partials are FP32, the beam's totals FP64; the wave
+// reduces them once per (beam, tile) into its slot.
+// ---------------------------------------------------------------------------
+constexpr int kTileU = 32;   // 128-point units per tile (bits of a row's mask)
+
+struct TQArgs {
+  bh_fwd_beams b;
+  long long count;
+  int modes;
+  double c;
+  const double* ts;
+  long long nt;
+  SegDev G;
+  double E;
+  const float* rg;       // residual in group order: [nt][ngroups * 64]
+  double* slots;         // [ntt * ntu][count] complex
+  long long ntu;         // unit tiles
+  long long nunits;      // ngroups / 2
+};
+
+// r at (t, slot) for every 64-point group slot, zero where the slot is padding
+__global__ void k_gather_resid(const void* __restrict__ resid, int f64, long long nt, long long S,
+                               const int32_t* __restrict__ col, long long nslot,
+                               float* __restrict__ rg) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nt * nslot) return;
+  const long long t = i / nslot, q = i - t * nslot;
+  const int c = col[q];
+  float v = 0.f;
+  if (c >= 0)
+    v = f64 ? (float)((const double*)resid)[t * S + c] : ((const float*)resid)[t * S + c];
+  rg[i] = v;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_fwd_transpose_q(TQArgs A) {
+  __shared__ RecS<D> rec[4][64];
+  __shared__ unsigned short plist[4][64 * kTileU];   // row << 5 | unit
+  __shared__ PairQ coef[4][64];
+  __shared__ double upt[kTileU][D], ustep[kTileU][D];
+  __shared__ int useg[kTileU], uj0[kTileU];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const long long t0 = (long long)blockIdx.y * 64;
+  const long long ub = (long long)blockIdx.z * kTileU;
+  const long long nul = A.nunits - ub;
+  const int nu = (int)(nul < kTileU ? nul : kTileU);
+  if (threadIdx.x < nu) {
+    const long long g = 2 * (ub + threadIdx.x);
+    const int k = A.G.grp_seg[g];
+    for (int a = 0; a < D; ++a) {
+      upt[threadIdx.x][a] = A.G.pts[g * 64 * D + a];
+      ustep[threadIdx.x][a] = A.G.v[(long long)k * D + a];
+    }
+    useg[threadIdx.x] = k;
+    uj0[threadIdx.x] = (int)((g - A.G.g0[k]) * 64);
+  }
+  __syncthreads();
+  const long long tl = t0 + lane;
+  const bool tvalid = tl < A.nt;
+  const double tv = tvalid ? A.ts[tl] : 0.0;
+  const long long tile = (long long)blockIdx.y * A.ntu + blockIdx.z;
+  const long long nw = (long long)gridDim.x * 4;
+  const long long S64 = A.G.ngroups * 64;
+  const unsigned all_units = nu == 32 ? 0xffffffffu : ((1u << nu) - 1u);
+  const double ld = (double)lane;
+  const float lf = (float)lane;
+  for (long long k = (long long)blockIdx.x * 4 + wv; k < A.count; k += nw) {
+    double x0[D], p0[D];
+    cd al[D];
+    for (int a = 0; a < D; ++a) {
+      x0[a] = A.b.x0[k * D + a];
+      p0[a] = A.b.p0[k * D + a];
+      al[a] = mkc(A.b.alpha[2 * (k * D + a)], A.b.alpha[2 * (k * D + a) + 1]);
+    }
+    const cd a0 = mkc(A.b.a0[2 * k], A.b.a0[2 * k + 1]);
+    const double w = A.b.omega[k];
+    double acc_re = 0.0, acc_im = 0.0;
+    for (int md = 0; md < A.modes; ++md) {
+      RecS<D> R;
+      bool live = tvalid;
+      if (live) {
+        State<D> St;
+        hom_diag_state_fast<D, true>(x0, p0, al, a0, md ? -A.c : A.c, tv, St);
+        make_rec<D>(St, w, R);
+        live = !rec_zero<D>(R);
+      }
+      if (!live) zero_rec<D>(R);
+      unsigned msk = 0u;
+      if (live) {
+        if (A.E > 0.0) {
+          double Q[D][D];
+          env_matrix<D>(R, Q);
+          int cur = -1, lo = 1, hi = 0;
+          for (int u = 0; u < nu; ++u) {
+            const int ks = useg[u];
+            if (ks != cur) {
+              seg_interval<D>(A.G, ks, R.x, Q, A.E, lo, hi);
+              cur = ks;
+            }
+            const int j0 = uj0[u];
+            if (hi >= lo && hi >= j0 && lo <= j0 + 127) msk |= 1u << u;
+          }
+        } else {
+          msk = all_units;
+        }
+      }
+      wave_lds_fence();   // the previous mode's list, records and pass are consumed
+      rec[wv][lane] = R;
+      // compaction of the active (row, unit) pairs, row-major
+      const int cnt = __popc(msk);
+      int incl = cnt;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+      }
+      const int total = __shfl(incl, 63);
+      int pos = incl - cnt;
+      for (unsigned m = msk; m; m &= m - 1)
+        plist[wv][pos++] = (unsigned short)((lane << 5) | __builtin_ctz(m));
+      wave_lds_fence();
+      for (int b = 0; b < total; b += 64) {
+        const int e = b + lane;
+        if (e < total) {
+          const int pr = plist[wv][e];
+          coef[wv][lane] = pair_quad<D>(rec[wv][pr >> 5], upt[pr & 31], ustep[pr & 31]);
+        }
+        wave_lds_fence();
+        const int n = __builtin_amdgcn_readfirstlane(total - b < 64 ? total - b : 64);
+        float sr = 0.f, si = 0.f;
+        for (int sl = 0; sl < n; ++sl) {
+          const int pr = __builtin_amdgcn_readfirstlane((int)plist[wv][b + sl]);
+          const PairQ c = coef[wv][sl];
+          const float* rr = A.rg + (t0 + (pr >> 5)) * S64 + (2 * (ub + (pr & 31))) * 64;
+          const float r0 = rr[lane], r1 = rr[64 + lane];
+          const double cyc0 = fma(ld, fma(ld, c.C, c.B), c.A);
+          const double ld1 = ld + 64.0;
+          const double cyc1 = fma(ld1, fma(ld1, c.C, c.B), c.A);
+          const float f0 = __uint_as_float(
+              __builtin_amdgcn_alignbit(0x7Fu, (unsigned)__double_as_longlong(cyc0), 9u));
+          const float f1 = __uint_as_float(
+              __builtin_amdgcn_alignbit(0x7Fu, (unsigned)__double_as_longlong(cyc1), 9u));
+          const float u0 = fmaf(lf, c.s, c.nl0s);
+          const float u1 = fmaf(64.f, c.s, u0);
+          const float m0 = r0 * __builtin_amdgcn_exp2f(fmaf(-u0, u0, c.nlog));
+          const float m1 = r1 * __builtin_amdgcn_exp2f(fmaf(-u1, u1, c.nlog));
+          sr = fmaf(m0, __builtin_amdgcn_cosf(f0), sr);
+          sr = fmaf(m1, __builtin_amdgcn_cosf(f1), sr);
+          si = fmaf(-m0, __builtin_amdgcn_sinf(f0), si);
+          si = fmaf(-m1, __builtin_amdgcn_sinf(f1), si);
+        }
+        acc_re += (double)sr;
+        acc_im += (double)si;
+        wave_lds_fence();   // the pass's coefficients are consumed
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      acc_re += __shfl_xor(acc_re, off);
+      acc_im += __shfl_xor(acc_im, off);
+    }
+    if (lane == 0) {
+      A.slots[2 * (tile * A.count + k)] = acc_re;
+      A.slots[2 * (tile * A.count + k) + 1] = acc_im;
+    }
+  }
+}
+
 // grad[k] = sum over tiles (fixed order) of slots[tile][k]
 __global__ void k_slot_reduce(const double* __restrict__ slots, long long tiles, long long count,
                               double* __restrict__ grad) {
@@ -205,13 +385,35 @@ inline TShape transpose_shape(long long count, long long nt, long long ngroups) 
   return s;
 }
 
+// launch shape of the FP32-split form: unit tiles instead of group tiles
+inline TShape transpose_q_shape(long long count, long long nt, long long ngroups) {
+  TShape s;
+  s.ntt = (nt + 63) / 64;
+  s.ntg = (ngroups / 2 + kTileU - 1) / kTileU;
+  const long long tiles = s.ntt * s.ntg;
+  long long gx = (4096 + tiles - 1) / tiles;
+  const long long maxgx = (count + 3) / 4;
+  if (gx > maxgx) gx = maxgx;
+  if (gx > 65535) gx = 65535;
+  s.gx = gx < 1 ? 1 : gx;
+  return s;
+}
+
+size_t round256(size_t x) { return (x + 255) / 256 * 256; }
+
 }  // namespace
 
+// The FP64 form's slots, or the FP32-split form's slots followed by the
+// residual in group order; the larger of the two.
 extern "C" size_t bh_forward_transpose_workspace_size(int64_t count, int64_t nt,
                                                       int64_t ngroups) {
   if (count <= 0 || nt <= 0 || ngroups <= 0) return 0;
   const TShape sh = transpose_shape(count, nt, ngroups);
-  return (size_t)sh.ntt * sh.ntg * count * 2 * sizeof(double);
+  const size_t f64 = (size_t)sh.ntt * sh.ntg * count * 2 * sizeof(double);
+  const TShape sq = transpose_q_shape(count, nt, ngroups);
+  const size_t f32 = round256((size_t)sq.ntt * sq.ntg * count * 2 * sizeof(double)) +
+                     (size_t)nt * ngroups * 64 * sizeof(float);
+  return f64 > f32 ? f64 : f32;
 }
 
 extern "C" int bh_forward_transpose(const bh_fwd_beams* b, int ndim, int mirror, double c,
@@ -236,6 +438,39 @@ extern "C" int bh_forward_transpose(const bh_fwd_beams* b, int ndim, int mirror,
   const size_t need = bh_forward_transpose_workspace_size(b->count, nt, seg->ngroups);
   if (!ws || ws_bytes < need)
     BH_FAIL(BH_ERR_WORKSPACE, "bh_forward_transpose: workspace %zu < %zu", ws_bytes, need);
+  if (precision == BH_PRECISION_FP32_SPLIT) {
+    if (seg->ngroups % 2)
+      BH_FAIL(BH_ERR_ARG, "bh_forward_transpose: %lld groups (runs hold an even number)",
+              (long long)seg->ngroups);
+    const TShape sq = transpose_q_shape(b->count, nt, seg->ngroups);
+    TQArgs q{};
+    q.b = *b;
+    q.count = b->count;
+    q.modes = mirror ? 2 : 1;
+    q.c = c;
+    q.ts = ts;
+    q.nt = nt;
+    q.G = to_segdev(seg);
+    q.E = cull_log2;
+    q.slots = (double*)ws;
+    float* rg = (float*)((char*)ws + round256((size_t)sq.ntt * sq.ntg * b->count * 2 *
+                                              sizeof(double)));
+    q.rg = rg;
+    q.ntu = sq.ntg;
+    q.nunits = seg->ngroups / 2;
+    const long long nslot = seg->ngroups * 64;
+    hipLaunchKernelGGL(k_gather_resid, dim3(bh::grid1d(nt * nslot, 256)), dim3(256), 0, st, resid,
+                       resid_f64, (long long)nt, (long long)seg->S, seg->col, nslot, rg);
+    dim3 grid((unsigned)sq.gx, (unsigned)sq.ntt, (unsigned)sq.ntg);
+    if (ndim == 1) hipLaunchKernelGGL(k_fwd_transpose_q<1>, grid, dim3(256), 0, st, q);
+    else if (ndim == 2) hipLaunchKernelGGL(k_fwd_transpose_q<2>, grid, dim3(256), 0, st, q);
+    else hipLaunchKernelGGL(k_fwd_transpose_q<3>, grid, dim3(256), 0, st, q);
+    BH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_slot_reduce, dim3(bh::grid1d(b->count, 256)), dim3(256), 0, st,
+                       (const double*)ws, sq.ntt * sq.ntg, (long long)b->count, grad);
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+  }
   TArgs a{};
   a.b = *b;
   a.count = b->count;
