@@ -6,7 +6,8 @@ with shapes (b, Nt, d), (b, Nt, d), (b, Nt, d, d), (b, Nt, 1)
 (gb_solvers.py:24-93, 356-422) and runs the HIP kernel ``bh_solve_hom_diag``.
 Inside ``MSGBSolver`` the same closed form is fused into the summation
 kernel; these functions also serve as markers selecting that path.
-``solve_hom_TR`` (gb_solvers.py:484-594) is fused into ``bh_adjoint_params``.
+``solve_hom_TR`` (gb_solvers.py:484-594) runs ``bh_solve_hom_TR``; inside
+``MSGBSolver`` it is fused into ``bh_adjoint_params``.
 ``solve_ODE_base`` (gb_solvers.py:941-1016) integrates the beam ODEs in a
 heterogeneous (grid) medium on the GPU (``bh_ode_forward``); inside
 ``MSGBSolver`` it selects the trajectory-fed summation.  ``solve_ODE_batch_t``
@@ -55,14 +56,54 @@ def solve_hom_diag(x0, p0, M0, a0, mode, ts, c, lam=None, config=None):
     return xt, pt, Mt, At
 
 
-def solve_hom_TR(xT, pT, mT, aT, mode, ts, c, lam=None, config=None):
-    """Terminal-value closed form (gb_solvers.py:484-594).
+def _tr_time_grid(ts, b):
+    """Per-beam time rows (b, nt) from ts as the reference reads it
+    (gb_solvers.py:547-564): a scalar, a vector of b per-beam times, a common
+    timeline, (b, nt), (nt, b) or anything else flattened into one timeline."""
+    import torch
+    t = ts if isinstance(ts, torch.Tensor) else torch.as_tensor(np.asarray(ts, dtype=np.float64))
+    t = t.to(torch.float64)
+    if t.ndim == 0:
+        return t.reshape(1, 1).expand(b, 1)
+    if t.ndim == 1:
+        if t.shape[0] == b:
+            return t.reshape(b, 1)
+        return t.reshape(1, -1).expand(b, t.shape[0])
+    if t.shape[0] == b:
+        return t.reshape(b, -1)
+    if t.ndim == 2 and t.shape[1] == b:
+        return t.transpose(0, 1)
+    flat = t.reshape(-1)
+    return flat.reshape(1, -1).expand(b, flat.shape[0])
 
-    Inside ``MSGBSolver.adjoint`` this propagation is fused into the adjoint
-    parameter kernel; called directly it is not provided on the GPU.
-    """
-    raise NotImplementedError(
-        "solve_hom_TR runs fused inside MSGBSolver.adjoint (bh_adjoint_params)")
+
+def solve_hom_TR(xT, pT, mT, aT, mode, ts, c, lam=None, config=None):
+    """Terminal-value closed form (gb_solvers.py:484-594) on the GPU
+    (``bh_solve_hom_TR``): states at each beam's times measured from its
+    first time, x = xT + c0 p^ dt, M = mT (I + c0 dt P_perp mT / |p|)^-1,
+    A = aT / sqrt(det(...)).  Returns (xt, pt, Mt, At) with shapes (b, Nt, d),
+    (b, Nt, d), (b, Nt, d, d), (b, Nt, 1).  Inside ``MSGBSolver.adjoint`` the
+    same propagation to t = 0 is fused into the adjoint parameter kernel."""
+    import torch
+    dev = xT.device if isinstance(xT, torch.Tensor) else torch.device("cuda")
+    f64 = lambda v: torch.as_tensor(v, device=dev).to(torch.float64).contiguous()
+    c128 = lambda v: torch.as_tensor(v, device=dev).to(torch.complex128).contiguous()
+    xT, pT = f64(xT), f64(pT)
+    b, d = pT.shape
+    mT = c128(mT).reshape(b, d, d)
+    aT = c128(aT).reshape(b, -1)[:, 0].contiguous()
+    mode = f64(mode).reshape(b)
+    tsb = _tr_time_grid(ts if not isinstance(ts, torch.Tensor) else ts.to(dev), b)
+    tsb = tsb.to(device=dev, dtype=torch.float64).contiguous()
+    nt = tsb.shape[1]
+    xt = torch.empty((b, nt, d), dtype=torch.float64, device=dev)
+    pt = torch.empty_like(xt)
+    Mt = torch.empty((b, nt, d, d), dtype=torch.complex128, device=dev)
+    At = torch.empty((b, nt, 1), dtype=torch.complex128, device=dev)
+    _lib.call("bh_solve_hom_TR", b, d, _lib.ptr(xT), _lib.ptr(pT), _lib.ptr(mT), _lib.ptr(aT),
+              _lib.ptr(mode), _c0(c), _lib.ptr(tsb), nt, _lib.ptr(xt), _lib.ptr(pt),
+              _lib.ptr(Mt), _lib.ptr(At), _lib.stream_handle())
+    return xt, pt, Mt, At
 
 
 def speed_of(c):

@@ -382,3 +382,36 @@ def test_forward_with_other_thresholds_matches_oracle(gpu, strategy, val):
                       cull_log2=0.0)
     got = _n(s.forward(p0, dom, sensors, ts, wpt))
     assert rel(ref, got) < 1e-12
+
+
+def test_hybrid_hf_seam_runs_the_gpu_solver(gpu):
+    """The hybrid solver's HF seam (hybrid_solver.py:795-853) with this
+    package's MSGBSolver: the forward and the adjoint through the seam are
+    the solver's own results, bit for bit."""
+    import torch
+    from beamax_amd.solvers.hybrid import solve_hf
+    rng = np.random.default_rng(31)
+    n = 64
+    dom = bx.Domain(N=(n, n), dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    wpt = bx.MSWPT(bx.DyadicDecomposition(2, (n, n), (4, 8), (1, 1)), 2, "rectangular_mirror")
+    mask = np.zeros((n, n))
+    mask[0, :] = 1
+    sensors = bx.Sensor(dom, binary_mask=mask)
+    ts = dom.generate_time_domain()[:200]
+    p0 = rng.standard_normal((n, n))
+    s = bx.MSGBSolver(thr=500, thr_strat="top_n", batch_size=64, input_type="spatial",
+                      ode_solver=gb_solvers.solve_hom_diag, sum_method="scan_real")
+    a = _n(solve_hf(s, p0, dom, sensors, mask, ts, wpt, "forward"))
+    b = _n(s.forward(p0, dom, sensors, ts, wpt))
+    assert np.array_equal(a, b)
+    nt = 2 * n
+    tsd = np.linspace(ts[0], ts[-1], nt)
+    ddom = bx.Domain(N=(nt, n), dx=(tsd[1] - tsd[0], 1e-4), c=1500.0, periodic=(False, False))
+    dw = bx.MSWPT(bx.DyadicDecomposition(2, (nt, n), (4, 8), (2, 1)), 2, "rectangular_mirror")
+    data = _line_data(rng, nt, n, tsd[1] - tsd[0])
+    img = bx.Sensor(dom, binary_mask=np.ones((n, n)))
+    kw = dict(sources=sensors, data_domain=ddom, data_wpt=dw)
+    qa = _n(solve_hf(s, torch.as_tensor(data, device=gpu), dom, img, None, tsd, None, "adjoint",
+                     **kw))
+    qb = _n(s.adjoint(torch.as_tensor(data, device=gpu), dom, img, sensors, tsd, ddom, dw))
+    assert np.array_equal(qa, qb)

@@ -140,3 +140,32 @@ def test_ptr_holds_temporaries_until_the_call_returns():
         _lib.call("bh_device_status", 0, None, None)
     gc.collect()
     assert ref() is None
+
+
+def test_hybrid_hf_seam_dispatch():
+    """hybrid_solver.py:795-853: an MSGBSolver gets its own forward /
+    time_reversal / adjoint signature, any other HF solver the mask-based
+    one; an unknown method is a ValueError naming it."""
+    from beamax_amd.gb import gb_solvers
+    from beamax_amd.solvers.hybrid import solve_hf
+
+    class Duck:
+        def __init__(self):
+            self.calls = []
+
+        def forward(self, data, domain, mask, ts, **kw):
+            self.calls.append(("forward", data, domain, mask, ts, kw))
+            return "fwd"
+
+        def adjoint(self, data, domain, mask, sources, ts):
+            self.calls.append(("adjoint", data, domain, mask, sources, ts))
+            return "adj"
+
+    duck = Duck()
+    assert solve_hf(duck, 1, 2, "sensors", "mask", 5, "wpt", "forward", extra=7) == "fwd"
+    assert duck.calls[-1] == ("forward", 1, 2, "mask", 5, {"extra": 7})
+    assert solve_hf(duck, 1, 2, "sensors", "mask", 5, "wpt", "adjoint", sources="src") == "adj"
+    assert duck.calls[-1] == ("adjoint", 1, 2, "mask", "src", 5)
+    s = bx.MSGBSolver(10, "top_n", 8, "spatial", gb_solvers.solve_hom_diag, "scan_real")
+    with pytest.raises(ValueError, match="Unknown HF method: 'nope'"):
+        solve_hf(s, None, None, None, None, None, None, "nope")
