@@ -508,10 +508,11 @@ def warm_start(net: UNet, op, data, image):
     return alpha * xh
 
 
-def evaluate(net: Optional[UNet], problem, seeds, device, tol=1e-3, log=None):
+def evaluate(net: Optional[UNet], problem, seeds, device, tol=1e-3, log=None, graph=True):
     """CG on the normal equations from zero and from the learned start, for
     each held-out seed: iterations, seconds (CG alone, and with the
-    network's inference and the least-squares scaling), error against p0."""
+    network's inference and the least-squares scaling), error against p0.
+    Both runs replay a captured CG iteration (``graph``)."""
     import torch
     from .reconstruct import cgnr
     from .solvers.linear import FrozenForward
@@ -524,8 +525,8 @@ def evaluate(net: Optional[UNet], problem, seeds, device, tol=1e-3, log=None):
         d = measured_data(problem, p0, sd)
         img = fa(d).reshape(N)
         F = FrozenForward.from_field(s, img, dom, sensors, ts, wpt)
-        cgnr(F, d, tol=tol, maxiter=3)                  # warm-up (workspaces)
-        x, info = cgnr(F, d, tol=tol, maxiter=1000)
+        cgnr(F, d, tol=tol, maxiter=3, graph=graph)      # warm-up (workspaces)
+        x, info = cgnr(F, d, tol=tol, maxiter=1000, graph=graph)
         row = {"seed": int(sd), "plain_iters": info["iterations"],
                "plain_s": info["seconds"],
                "plain_err": float(((x - p0).norm() / p0.norm()).item())}
@@ -535,7 +536,7 @@ def evaluate(net: Optional[UNet], problem, seeds, device, tol=1e-3, log=None):
             x0 = warm_start(net, F, d, img)
             torch.cuda.synchronize()
             t_init = time.perf_counter() - t0
-            x1, info1 = cgnr(F, d, x0=x0, tol=tol, maxiter=1000)
+            x1, info1 = cgnr(F, d, x0=x0, tol=tol, maxiter=1000, graph=graph)
             row.update({"learned_iters": info1["iterations"], "learned_cg_s": info1["seconds"],
                         "learned_s": info1["seconds"] + t_init, "init_s": t_init,
                         "start_residual": info1["residuals"][0],
@@ -591,6 +592,13 @@ def _main(argv=None):
         return
     rows = evaluate(net, problem, list(range(a.eval)), dev, log=log)
     mean = lambda k: float(np.mean([r[k] for r in rows]))
+    ratios = np.array([r["learned_s"] / r["plain_s"] for r in rows])
+    result["time_ratio_per_seed"] = {
+        "median": float(np.median(ratios)), "min": float(ratios.min()),
+        "max": float(ratios.max()), "p10": float(np.percentile(ratios, 10)),
+        "p90": float(np.percentile(ratios, 90)),
+        "iterations_plain_hist": np.bincount([r["plain_iters"] for r in rows]).tolist(),
+        "iterations_learned_hist": np.bincount([r["learned_iters"] for r in rows]).tolist()}
     result.update({"eval_seeds": len(rows), "plain_iters": mean("plain_iters"),
                    "plain_s": mean("plain_s"), "learned_iters": mean("learned_iters"),
                    "learned_s": mean("learned_s"), "learned_cg_s": mean("learned_cg_s"),

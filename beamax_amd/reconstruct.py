@@ -4,10 +4,21 @@
 transpose (``solvers.linear.FrozenForward``) by conjugate gradients on
 F^T F x = F^T d (CGNR; Hestenes and Stiefel 1952, Saad "Iterative Methods for
 Sparse Linear Systems" alg. 8.4), with an optional symmetric positive
-definite preconditioner M ~ (F^T F)^-1 applied as z = M(g).  The stopping
-test is the plain normal-equation residual ||F^T (d - F x)|| / ||F^T d||,
-whatever the preconditioner.  The reference has no reconstruction solver;
-this is the operator use its adjoint/time-reversal examples lead to
+definite preconditioner M ~ (F^T F)^-1 applied as z = M(g).  ``fcgnr`` is the
+flexible variant (Notay 2000, "Flexible conjugate gradients"): each new
+direction is made F^T F-conjugate to the last ``window`` directions by
+explicit projection, so the preconditioner may change from step to step
+(a network, for instance).  The stopping test of both is the plain
+normal-equation residual ||F^T (d - F x)|| / ||F^T d||, whatever the
+preconditioner.
+
+Every scalar of an iteration (step length, conjugation weights, residual
+norm) stays a device tensor: an iteration issues no host synchronisation
+except the one read of the residual norm for the stopping test, and with
+``graph=True`` the whole iteration (operator, transpose, preconditioner,
+vector updates) is captured once as a HIP graph and replayed.  The
+reference has no reconstruction solver; this is the operator use its
+adjoint/time-reversal examples lead to
 (``examples/reconstruction/2d_time_reversal_and_adjoint.py``).
 """
 
@@ -19,7 +30,9 @@ import numpy as np
 
 
 def _dot(a, b):
-    return float((a.reshape(-1).double() * b.reshape(-1).double()).sum().item())
+    """<a, b> as a 0-d FP64 device tensor (no host read)."""
+    import torch
+    return torch.dot(a.reshape(-1), b.reshape(-1))
 
 
 def _sync(t):
@@ -28,62 +41,130 @@ def _sync(t):
         torch.cuda.synchronize(t.device)
 
 
+class _State:
+    """The iteration's device tensors (fixed addresses, so a captured graph
+    reads and writes the same memory on every replay)."""
+
+    def __init__(self, op, d, x0):
+        import torch
+        self.op = op
+        self.d = d
+        if x0 is None:
+            self.x = torch.zeros(op.shape[1], dtype=torch.float64, device=d.device)
+            self.r = d.clone()
+        else:
+            self.x = x0.to(torch.float64).reshape(op.shape[1]).clone()
+            self.r = d - op.apply(self.x).to(torch.float64)
+        self.g = op.transpose(self.r).to(torch.float64)
+        self.gg = _dot(self.g, self.g)
+
+
 def cgnr(op, data, *, x0=None, tol: float = 1e-3, maxiter: int = 500, precond=None,
-         callback=None):
+         callback=None, graph: bool = False):
     """Returns (x, info); info holds iterations, residual history, seconds and
     whether ``tol`` was reached.  ``op`` needs ``apply(x)``, ``transpose(r)``
-    and ``shape = ((Nt, S), N)``."""
+    and ``shape = ((Nt, S), N)``; ``precond(g)`` returns M g for the FP64
+    gradient image g."""
     import torch
     _sync(data)
     t0 = time.perf_counter()
     d = data.reshape(op.shape[0]).to(torch.float64)
+    st = _State(op, d, x0)
     if x0 is None:
-        x = torch.zeros(op.shape[1], dtype=torch.float64, device=d.device)
-        r = d.clone()
-    else:
-        x = x0.to(torch.float64).reshape(op.shape[1]).clone()
-        r = d - op.apply(x).to(torch.float64)
-    g = op.transpose(r)                    # F^T r
-    if x0 is None:
-        g0 = np.sqrt(_dot(g, g))
+        g0 = float(st.gg.sqrt().item())
     else:
         # residuals stay relative to ||F^T d||, as from a zero start
-        gd = op.transpose(d)
-        g0 = np.sqrt(_dot(gd, gd))
-    hist = [np.sqrt(_dot(g, g)) / g0 if g0 > 0 else 0.0]
-    if g0 == 0.0:
-        return x, {"iterations": 0, "residuals": hist, "seconds": 0.0, "converged": True}
-    if hist[0] <= tol:
-        _sync(x)
-        return x, {"iterations": 0, "residuals": hist, "seconds": time.perf_counter() - t0,
-                   "converged": True}
-    z = precond(g) if precond is not None else g
-    p = z.clone()
-    gz = _dot(g, z)
-    it = 0
-    converged = False
-    while it < maxiter:
-        w = op.apply(p).to(torch.float64)  # F p
-        ww = _dot(w, w)
-        if ww <= 0.0:
-            break
-        alpha = gz / ww
-        x.add_(p, alpha=alpha)
-        r.add_(w, alpha=-alpha)
-        g = op.transpose(r)
+        gd = op.transpose(d).to(torch.float64)
+        g0 = float(_dot(gd, gd).sqrt().item())
+    hist = [float(st.gg.sqrt().item()) / g0 if g0 > 0 else 0.0]
+    if g0 == 0.0 or hist[0] <= tol:
+        _sync(st.x)
+        return st.x, {"iterations": 0, "residuals": hist,
+                      "seconds": time.perf_counter() - t0, "converged": True}
+    z = precond(st.g) if precond is not None else st.g
+    p = z.to(torch.float64).clone()
+    gz = _dot(st.g, z)
+
+    def step():
+        w = op.apply(p).to(torch.float64)               # F p
+        alpha = gz / _dot(w, w)
+        st.x.addcmul_(p, alpha)
+        st.r.addcmul_(w, alpha, value=-1.0)
+        st.g.copy_(op.transpose(st.r))
+        st.gg.copy_(_dot(st.g, st.g))
+        zn = precond(st.g) if precond is not None else st.g
+        gz_new = _dot(st.g, zn)
+        p.mul_(gz_new / gz).add_(zn)
+        gz.copy_(gz_new)
+
+    run = step
+    if graph and d.is_cuda:
+        step()                                           # warm: workspaces, caches
+        hist.append(float(st.gg.sqrt().item()) / g0)
+        if hist[-1] > tol:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                step()
+            run = g.replay
+    it = len(hist) - 1
+    converged = hist[-1] <= tol
+    while not converged and it < maxiter:
+        run()
         it += 1
-        res = np.sqrt(_dot(g, g)) / g0
+        res = float(st.gg.sqrt().item()) / g0            # the one host read per iteration
         hist.append(res)
         if callback is not None:
-            callback(it, x, res)
-        if res <= tol:
-            converged = True
-            break
-        z = precond(g) if precond is not None else g
-        gz_new = _dot(g, z)
-        beta = gz_new / gz
-        gz = gz_new
-        p.mul_(beta).add_(z)
-    _sync(x)
-    return x, {"iterations": it, "residuals": hist, "seconds": time.perf_counter() - t0,
-               "converged": converged}
+            callback(it, st.x, res)
+        converged = res <= tol
+    _sync(st.x)
+    return st.x, {"iterations": it, "residuals": hist, "seconds": time.perf_counter() - t0,
+                  "converged": converged}
+
+
+def fcgnr(op, data, precond, *, x0=None, tol: float = 1e-3, maxiter: int = 200,
+          window: int = 4, callback=None):
+    """Flexible CG on the normal equations: z_k = precond(g_k) may be a
+    nonlinear or step-dependent map; p_k = z_k minus its F^T F-projections on
+    the last ``window`` directions (Notay's FCG(m)); the F^T F p products come
+    from gradient differences, so an iteration costs one ``apply``, one
+    ``transpose`` and one ``precond``.  Same return value as ``cgnr``."""
+    import torch
+    _sync(data)
+    t0 = time.perf_counter()
+    d = data.reshape(op.shape[0]).to(torch.float64)
+    st = _State(op, d, x0)
+    if x0 is None:
+        g0 = float(st.gg.sqrt().item())
+    else:
+        gd = op.transpose(d).to(torch.float64)
+        g0 = float(_dot(gd, gd).sqrt().item())
+    hist = [float(st.gg.sqrt().item()) / g0 if g0 > 0 else 0.0]
+    dirs = []                                            # (p, F^T F p, |F p|^2)
+    it = 0
+    converged = g0 == 0.0 or hist[0] <= tol
+    while not converged and it < maxiter:
+        z = precond(st.g).to(torch.float64).reshape(st.g.shape)
+        p = z.clone()
+        for pj, apj, nj in dirs:
+            p.addcmul_(pj, _dot(z, apj) / nj, value=-1.0)
+        w = op.apply(p).to(torch.float64)
+        ww = _dot(w, w)
+        alpha = _dot(p, st.g) / ww
+        st.x.addcmul_(p, alpha)
+        st.r.addcmul_(w, alpha, value=-1.0)
+        gn = op.transpose(st.r).to(torch.float64)
+        ap = (st.g - gn) / alpha                          # F^T F p
+        st.g.copy_(gn)
+        st.gg.copy_(_dot(gn, gn))
+        dirs.append((p, ap, ww))
+        if len(dirs) > window:
+            dirs.pop(0)
+        it += 1
+        res = float(st.gg.sqrt().item()) / g0
+        hist.append(res)
+        if callback is not None:
+            callback(it, st.x, res)
+        converged = res <= tol
+    _sync(st.x)
+    return st.x, {"iterations": it, "residuals": hist, "seconds": time.perf_counter() - t0,
+                  "converged": converged}

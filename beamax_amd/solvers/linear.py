@@ -92,7 +92,7 @@ class FrozenForward:
                   _lib.ptr(self._vals), _lib.stream_handle())
         _lib.call("bh_forward_params", C.byref(wpt.tiling), self._gs, _lib.ptr(self.idx),
                   _lib.ptr(self._vals), self.k, self._beams.struct(), _lib.stream_handle())
-        return self.solver.sum_forward(self._beams, self.domain, self.points, _np(self.ts), out=out)
+        return self.solver.sum_forward(self._beams, self.domain, self.points, self.ts, out=out)
 
     def transpose(self, r, out=None):
         """F^T r: real field of shape ``domain.N`` (float64)."""

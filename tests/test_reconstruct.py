@@ -50,3 +50,31 @@ def test_cgnr_preconditioned_matches_plain():
     assert i1["converged"] and i2["converged"]
     assert i2["iterations"] < i1["iterations"]
     assert np.linalg.norm(x1.numpy() - x2.numpy()) <= 1e-6 * np.linalg.norm(x1.numpy())
+
+
+def test_fcgnr_with_a_fixed_spd_preconditioner_matches_cgnr():
+    """With a fixed SPD preconditioner, flexible CG on the normal equations
+    takes the same steps as preconditioned CGNR (up to rounding) and reaches
+    the same least-squares solution; with a step-dependent preconditioner
+    it still converges."""
+    import torch
+    from beamax_amd.reconstruct import fcgnr
+    rng = np.random.default_rng(8)
+    m, n = 70, 25
+    A = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, 1.5, n))
+    d = rng.standard_normal(m)
+    op = _MatOp(A, (m,), (n,))
+    D = torch.as_tensor(1.0 / np.sum(A * A, axis=0))
+    x1, i1 = cgnr(op, torch.as_tensor(d), tol=1e-10, maxiter=500, precond=lambda g: D * g)
+    x2, i2 = fcgnr(op, torch.as_tensor(d), lambda g: D * g, tol=1e-10, maxiter=500, window=3)
+    assert i1["converged"] and i2["converged"]
+    assert abs(i1["iterations"] - i2["iterations"]) <= 2
+    assert np.linalg.norm(x1.numpy() - x2.numpy()) <= 1e-7 * np.linalg.norm(x1.numpy())
+    k = [0]
+
+    def varying(g):
+        k[0] += 1
+        return (1.0 + 0.5 * np.sin(k[0])) * D * g
+    x3, i3 = fcgnr(op, torch.as_tensor(d), varying, tol=1e-10, maxiter=500, window=3)
+    assert i3["converged"]
+    assert np.linalg.norm(x3.numpy() - x1.numpy()) <= 1e-7 * np.linalg.norm(x1.numpy())
