@@ -94,6 +94,8 @@ SIGNATURES = {
                                  _P]),
     "bh_solve_hom_diag": (C.c_int, [_I64, C.c_int, _P, _P, _P, _P, _P, C.c_double, _P, _I64, _P,
                                     _P, _P, _P, _P]),
+    "bh_solve_hom_TR": (C.c_int, [_I64, C.c_int, _P, _P, _P, _P, _P, C.c_double, _P, _I64, _P,
+                                  _P, _P, _P, _P]),
     "bh_adjoint_source_workspace_size": (_SZ, [_I64, _I64]),
     "bh_adjoint_source": (C.c_int, [_P, _I64, _I64, C.c_double, _P, C.c_int, _P, _P, _P, _SZ, _P]),
     "bh_fourier_crop_time_workspace_size": (_SZ, [_I64, _I64, _I64]),
@@ -110,6 +112,7 @@ SIGNATURES = {
                                        C.POINTER(Segs), C.c_double, C.c_int, C.c_int, _P, _P, _P,
                                        _SZ, _P]),
     "bh_forward_sum_culled_workspace_size": (_SZ, [_I64, _I64]),
+    "bh_forward_culled_workspace_size": (_SZ, [_I64, C.c_int, _I64, _I64]),
     "bh_forward_sum_culled": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, C.c_double, _P,
                                         _I64, C.POINTER(Segs), C.c_double, C.c_int, _P, _P, _P,
                                         _SZ, _P]),

@@ -290,6 +290,73 @@ __device__ inline void cinv_det(const cd (&Z)[D][D], cd (&Zi)[D][D], cd& det) {
   }
 }
 
+// The closed form of solve_hom_TR (gb_solvers.py:484-594) over a time
+// offset dt from the reference state (xT, pT, MT, aT): x = xT + c0 p^ dt,
+// Z = I + c0 dt P_perp MT / |p| (P_perp = I - p p^T / |p|^2), M = MT Z^-1,
+// A = aT / sqrt(det Z) with the principal root.
+template <int D>
+__device__ inline void hom_tr_step(const double* xT, const double* pT, const cd (&MT)[D][D], cd aT,
+                                   double c0, double dt, double* x, cd (&M)[D][D], cd& A) {
+  double np2 = 0.0;
+  for (int a = 0; a < D; ++a) np2 += pT[a] * pT[a];
+  const double np = sqrt(np2);
+  for (int a = 0; a < D; ++a) x[a] = xT[a] + (c0 * (pT[a] / np)) * dt;
+  cd Z[D][D], Zi[D][D], det;
+  for (int a = 0; a < D; ++a)
+    for (int b = 0; b < D; ++b) {
+      cd pm = mkc(0.0);
+      for (int e = 0; e < D; ++e) {
+        const double pp = ((a == e) ? 1.0 : 0.0) - pT[a] * pT[e] / np2;
+        pm = pm + pp * MT[e][b];
+      }
+      Z[a][b] = ((c0 * dt) * pm) / np;
+      if (a == b) Z[a][b] = mkc(1.0) + Z[a][b];
+    }
+  cinv_det<D>(Z, Zi, det);
+  for (int a = 0; a < D; ++a)
+    for (int b = 0; b < D; ++b) {
+      cd s = mkc(0.0);
+      for (int e = 0; e < D; ++e) s = s + MT[a][e] * Zi[e][b];
+      M[a][b] = s;
+    }
+  A = aT / csqrt_(det);
+}
+
+// solve_hom_TR as a SolverFn: thread = (beam, time), states at ts[beam][j]
+// measured from ts[beam][0]
+template <int D>
+__global__ __launch_bounds__(128) void k_hom_tr(long long nb, const double* xT, const double* pT,
+                                                const double* mT, const double* aT,
+                                                const double* mode, double c, const double* ts,
+                                                long long nt, double* xt, double* pt, double* Mt,
+                                                double* At) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nb * nt) return;
+  const long long k = i / nt, j = i - k * nt;
+  double x0[D], p0[D];
+  cd MT[D][D];
+  for (int a = 0; a < D; ++a) {
+    x0[a] = xT[k * D + a];
+    p0[a] = pT[k * D + a];
+    for (int b = 0; b < D; ++b)
+      MT[a][b] = mkc(mT[2 * ((k * D + a) * D + b)], mT[2 * ((k * D + a) * D + b) + 1]);
+  }
+  double x[D];
+  cd M[D][D], A;
+  hom_tr_step<D>(x0, p0, MT, mkc(aT[2 * k], aT[2 * k + 1]), c * mode[k],
+                 ts[k * nt + j] - ts[k * nt], x, M, A);
+  for (int a = 0; a < D; ++a) {
+    xt[i * D + a] = x[a];
+    pt[i * D + a] = p0[a];
+    for (int b = 0; b < D; ++b) {
+      Mt[2 * ((i * D + a) * D + b)] = M[a][b].re;
+      Mt[2 * ((i * D + a) * D + b) + 1] = M[a][b].im;
+    }
+  }
+  At[2 * i] = A.re;
+  At[2 * i + 1] = A.im;
+}
+
 struct AdjArgs {
   TilingDev T;
   const int32_t* centres;
@@ -483,33 +550,9 @@ __global__ __launch_bounds__(128) void k_adjoint_params(AdjArgs A) {
     return;
   }
   // propagate t_start -> 0
-  np2 = 0.0;
-  for (int a = 0; a < D; ++a) np2 += pT[a] * pT[a];
-  np = sqrt(np2);
-  const double c0 = c * signum;
-  const double dt = 0.0 - t_start;
   double x[D];
-  for (int a = 0; a < D; ++a) x[a] = xT[a] + (c0 * (pT[a] / np)) * dt;
-  cd Z[D][D], Zi[D][D], det;
-  for (int a = 0; a < D; ++a)
-    for (int b = 0; b < D; ++b) {
-      cd pm = mkc(0.0);
-      for (int e = 0; e < D; ++e) {
-        const double pp = ((a == e) ? 1.0 : 0.0) - pT[a] * pT[e] / np2;
-        pm = pm + pp * MT[e][b];
-      }
-      Z[a][b] = ((c0 * dt) * pm) / np;
-      if (a == b) Z[a][b] = mkc(1.0) + Z[a][b];
-    }
-  cinv_det<D>(Z, Zi, det);
-  cd M0[D][D];
-  for (int a = 0; a < D; ++a)
-    for (int b = 0; b < D; ++b) {
-      cd s = mkc(0.0);
-      for (int e = 0; e < D; ++e) s = s + MT[a][e] * Zi[e][b];
-      M0[a][b] = s;
-    }
-  const cd A0 = aT / csqrt_(det);
+  cd M0[D][D], A0;
+  hom_tr_step<D>(xT, pT, MT, aT, c * signum, 0.0 - t_start, x, M0, A0);
   for (int a = 0; a < D; ++a) {
     A.out.x[i * D + a] = x[perm[a]];
     A.out.p[i * D + a] = pT[perm[a]];
@@ -684,6 +727,26 @@ extern "C" int bh_forward_sum(const bh_fwd_beams* b, int ndim, int mirror, doubl
   if (ndim == 1) { if (f64) launch_fwd_spt<1, true>(a, per, st); else launch_fwd_spt<1, false>(a, per, st); }
   else if (ndim == 2) { if (f64) launch_fwd_spt<2, true>(a, per, st); else launch_fwd_spt<2, false>(a, per, st); }
   else { if (f64) launch_fwd_spt<3, true>(a, per, st); else launch_fwd_spt<3, false>(a, per, st); }
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+extern "C" int bh_solve_hom_TR(int64_t b, int ndim, const double* xT, const double* pT,
+                               const double* mT, const double* aT, const double* mode, double c,
+                               const double* ts, int64_t nt, double* xt, double* pt, double* Mt,
+                               double* At, bh_stream s) {
+  bh::clear_error();
+  if (ndim < 1 || ndim > 3) BH_FAIL(BH_ERR_ARG, "bh_solve_hom_TR: ndim=%d", ndim);
+  if (b < 0 || nt < 0) BH_FAIL(BH_ERR_ARG, "bh_solve_hom_TR: b=%lld nt=%lld", (long long)b,
+                               (long long)nt);
+  if (b == 0 || nt == 0) return BH_OK;
+  if (!xT || !pT || !mT || !aT || !mode || !ts || !xt || !pt || !Mt || !At)
+    BH_FAIL(BH_ERR_ARG, "bh_solve_hom_TR: null pointer");
+  const unsigned g = bh::grid1d(b * nt, 128);
+  hipStream_t st = (hipStream_t)s;
+  if (ndim == 1) hipLaunchKernelGGL(k_hom_tr<1>, dim3(g), dim3(128), 0, st, (long long)b, xT, pT, mT, aT, mode, c, ts, (long long)nt, xt, pt, Mt, At);
+  else if (ndim == 2) hipLaunchKernelGGL(k_hom_tr<2>, dim3(g), dim3(128), 0, st, (long long)b, xT, pT, mT, aT, mode, c, ts, (long long)nt, xt, pt, Mt, At);
+  else hipLaunchKernelGGL(k_hom_tr<3>, dim3(g), dim3(128), 0, st, (long long)b, xT, pT, mT, aT, mode, c, ts, (long long)nt, xt, pt, Mt, At);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }

@@ -46,6 +46,11 @@ struct CulledArgs {
   unsigned long long* pairs;
   double* part;        // split > 1: partial sums [split][rows][S] (FP64)
   int split;
+  // forward entries in window order (null: entries in table order): position
+  // bi holds entry perm[bi] (its beam and mode as above), which can only be
+  // active on time rows [row_on[bi], row_off[bi]]
+  const int32_t* perm;
+  const int2* rows_act;
   long long rows;      // output rows (nt for the forward, 1 for the static form)
   int ostride;         // output element o goes to out[o * ostride + ooff]
   int ooff;
@@ -150,12 +155,21 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
     const long long bi = cb + threadIdx.x;
     Rec R;
     bool live = bi < total;
+    if constexpr (!STATIC) {
+      // outside its window the entry's envelope reaches no run: a wave whose
+      // entries are all outside skips the records (the branch is uniform)
+      if (live && A.rows_act) {
+        const int2 w = A.rows_act[bi];
+        live = t >= w.x && t <= w.y;
+      }
+    }
     if (live) {
       if constexpr (STATIC) {
         R = ((const Rec*)A.recs)[bi];
       } else {
-        const bool neg = A.mirror ? bi >= A.count : bi >= A.npos;
-        const long long j = (A.mirror && neg) ? bi - A.count : bi;
+        const long long e = A.perm ? (long long)A.perm[bi] : bi;
+        const bool neg = A.mirror ? e >= A.count : e >= A.npos;
+        const long long j = (A.mirror && neg) ? e - A.count : e;
         double x0[D], p0[D];
         cd al[D];
         for (int a = 0; a < D; ++a) {
@@ -166,7 +180,7 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
         State<D> St;
         if (A.traj) {
           // x, p, M (complex), A (complex) of beam bi at this time row
-          const double* y = A.traj + (t * A.total + bi) * (2 * D + 2 * D * D + 2);
+          const double* y = A.traj + (t * A.total + e) * (2 * D + 2 * D * D + 2);
           for (int a = 0; a < D; ++a) {
             St.x[a] = y[a];
             St.p[a] = y[D + a];
@@ -190,7 +204,8 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
     double Q[D][D];
     const bool cull = A.E > 0.0;
     if (live && cull) env_matrix<D>(R, Q);
-    const int nr = nruns;
+    const int nr = __ballot(live) ? nruns : 0;
+    if (nr == 0 && lane < nu) mask[par][lane][wv] = 0ull;   // no live record in the wave
     for (int r = 0; r < nr; ++r) {
       const int4 ru = runs[r];
       const int rk = __builtin_amdgcn_readfirstlane(ru.x);
@@ -420,6 +435,305 @@ __global__ void k_static_records_c(bh_static_beams b, void* recs) {
   make_rec<D>(S, b.omega[i], ((Rec*)recs)[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Activity windows of the forward entries.  For a closed-form beam with
+// alpha = i b (b > 0, the forward beams' form), 1/M(t) = 1/M0 + k t P_perp
+// (k = c0 / |p|), so Im M(t) = (B^-1 + k^2 t^2 P_perp B P_perp)^-1 decreases
+// (in the positive semidefinite order) with |t|: the envelope metric
+// Q(t) = w log2(e) Im M(t) / 2 dominates Q(tmax) for every |t| <= tmax.  An
+// entry can therefore be active on a run only while its centre x(t) = x0 +
+// c0 p^ t lies within sqrt(E) of the run in the metric of Q(tmax): a capsule
+// around the run, whose intersection with the line x(t) is one time interval
+// (a convex condition).  The runs are lengthened by 2.01 points at each end
+// and E by 1e-6 relative, covering the one-point widening of the culled
+// sums' intervals and the FP32 rounding of their envelope coefficients, so
+// every (entry, row) whose masks the kernel could set lies inside its
+// window.  Any other beam (Re alpha != 0, a metric that is not positive
+// definite, a time grid that is not increasing) gets every row.
+// ---------------------------------------------------------------------------
+struct WinArgs {
+  bh_fwd_beams b;
+  long long count, total, npos;
+  int mirror;
+  double c;
+  const double* ts;
+  long long nt;
+  SegDev G;
+  double E;
+  const double* trange;      // [0] max |t|, [1] 1 if ts is not increasing
+  unsigned long long* key;   // (row_on << 32) | entry
+  long long* idx;            // entry
+  int2* win;                 // per entry: first and last row
+};
+
+__global__ void k_time_range(const double* __restrict__ ts, long long nt, double* out) {
+  __shared__ double smax[256];
+  __shared__ int sbad[256];
+  double m = 0.0;
+  int bad = 0;
+  for (long long i = threadIdx.x; i < nt; i += blockDim.x) {
+    m = fmax(m, fabs(ts[i]));
+    if (i > 0 && !(ts[i] > ts[i - 1])) bad = 1;
+  }
+  smax[threadIdx.x] = m;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + o]);
+      sbad[threadIdx.x] |= sbad[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = smax[0];
+    out[1] = sbad[0] ? 1.0 : 0.0;
+  }
+}
+
+// times t where |y0 + vy t - s| <= R for a point s of the segment [ya, yb]
+// (Euclidean, D dimensions): the union of the two end balls and the slab
+// part, one interval (lo > hi when empty)
+template <int D>
+__device__ inline void capsule_times(const double* y0, const double* vy, const double* ya,
+                                     const double* yb, double R, double& lo, double& hi) {
+  lo = 1e300;
+  hi = -1e300;
+  double vv = 0.0;
+  for (int a = 0; a < D; ++a) vv += vy[a] * vy[a];
+  const double R2 = R * R;
+  auto quad = [&](double a2, double a1, double a0, double& l, double& h) {
+    // a2 t^2 + a1 t + a0 <= 0 with a2 >= 0
+    if (a2 > 0.0) {
+      const double disc = a1 * a1 - 4.0 * a2 * a0;
+      if (disc < 0.0) { l = 1e300; h = -1e300; return; }
+      const double sq = sqrt(disc);
+      l = (-a1 - sq) / (2.0 * a2);
+      h = (-a1 + sq) / (2.0 * a2);
+    } else if (a1 != 0.0) {
+      const double tt = -a0 / a1;
+      if (a1 > 0.0) { l = -1e300; h = tt; } else { l = tt; h = 1e300; }
+    } else if (a0 <= 0.0) {
+      l = -1e300; h = 1e300;
+    } else {
+      l = 1e300; h = -1e300;
+    }
+  };
+  for (int end = 0; end < 2; ++end) {
+    const double* cc = end ? yb : ya;
+    double B = 0.0, C = -R2;
+    for (int a = 0; a < D; ++a) {
+      const double d = y0[a] - cc[a];
+      B += d * vy[a];
+      C += d * d;
+    }
+    double l, h;
+    quad(vv, 2.0 * B, C, l, h);
+    if (l <= h) { lo = fmin(lo, l); hi = fmax(hi, h); }
+  }
+  double u[D], uu = 0.0;
+  for (int a = 0; a < D; ++a) { u[a] = yb[a] - ya[a]; uu += u[a] * u[a]; }
+  if (uu > 0.0) {
+    double w0u = 0.0, vyu = 0.0, w0w0 = 0.0, w0vy = 0.0;
+    for (int a = 0; a < D; ++a) {
+      const double w0 = y0[a] - ya[a];
+      w0u += w0 * u[a];
+      vyu += vy[a] * u[a];
+      w0w0 += w0 * w0;
+      w0vy += w0 * vy[a];
+    }
+    // squared distance to the line through the segment, times |u|^2
+    double l, h;
+    quad(fmax(vv * uu - vyu * vyu, 0.0), 2.0 * (w0vy * uu - w0u * vyu), w0w0 * uu - w0u * w0u - R2 * uu,
+         l, h);
+    // the foot of the perpendicular inside the segment: 0 <= w.u <= |u|^2
+    double sl, sh;
+    if (vyu != 0.0) {
+      const double t1 = -w0u / vyu, t2 = (uu - w0u) / vyu;
+      sl = fmin(t1, t2);
+      sh = fmax(t1, t2);
+    } else if (w0u >= 0.0 && w0u <= uu) {
+      sl = -1e300; sh = 1e300;
+    } else {
+      sl = 1e300; sh = -1e300;
+    }
+    l = fmax(l, sl);
+    h = fmin(h, sh);
+    if (l <= h) { lo = fmin(lo, l); hi = fmax(hi, h); }
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(128) void k_beam_windows(WinArgs W) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= W.total) return;
+  const bool neg = W.mirror ? e >= W.count : e >= W.npos;
+  const long long j = (W.mirror && neg) ? e - W.count : e;
+  int on = 0, off = (int)W.nt - 1;
+  double x0[D], p0[D];
+  cd al[D];
+  bool pure = true;
+  for (int a = 0; a < D; ++a) {
+    x0[a] = W.b.x0[j * D + a];
+    p0[a] = W.b.p0[j * D + a];
+    al[a] = mkc(W.b.alpha[2 * (j * D + a)], W.b.alpha[2 * (j * D + a) + 1]);
+    pure = pure && al[a].re == 0.0 && al[a].im > 0.0;
+  }
+  const cd a0 = mkc(W.b.a0[2 * j], W.b.a0[2 * j + 1]);
+  const double w = W.b.omega[j];
+  const double tmax = W.trange[0];
+  if (W.E > 0.0 && W.trange[1] == 0.0 && pure) {
+    if (a0.re == 0.0 && a0.im == 0.0) {
+      on = (int)W.nt;   // zero amplitude: never evaluated
+      off = -1;
+    } else {
+      const double c0 = neg ? -W.c : W.c;
+      State<D> St;
+      hom_diag_state_fast<D>(x0, p0, al, a0, c0, tmax, St);
+      // metric Q(tmax) and its Cholesky factor (lower): Q = L L^T
+      double Q[D][D], L[D][D] = {};
+      const double sc = 0.5 * w * bh::kLog2e;
+      for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b) Q[a][b] = sc * 0.5 * (St.M[a][b].im + St.M[b][a].im);
+      bool pd = true;
+      for (int a = 0; a < D && pd; ++a) {
+        for (int b = 0; b <= a; ++b) {
+          double v = Q[a][b];
+          for (int k = 0; k < b; ++k) v -= L[a][k] * L[b][k];
+          if (a == b) {
+            if (!(v > 0.0)) { pd = false; break; }
+            L[a][a] = sqrt(v);
+          } else {
+            L[a][b] = v / L[b][b];
+          }
+        }
+      }
+      if (pd) {
+        double np2 = 0.0;
+        for (int a = 0; a < D; ++a) np2 += p0[a] * p0[a];
+        const double inp = 1.0 / sqrt(np2);
+        // y = L^T x
+        auto ty = [&](const double* x, double* y) {
+          for (int a = 0; a < D; ++a) {
+            double v = 0.0;
+            for (int b = a; b < D; ++b) v += L[b][a] * x[b];
+            y[a] = v;
+          }
+        };
+        double vel[D], y0[D], vy[D];
+        for (int a = 0; a < D; ++a) vel[a] = c0 * p0[a] * inp;
+        ty(x0, y0);
+        ty(vel, vy);
+        const double R = sqrt(W.E * (1.0 + 1e-6));
+        double tlo = 1e300, thi = -1e300;
+        for (long long k = 0; k < W.G.nseg; ++k) {
+          const double n = (double)W.G.n[k];
+          double pa[D], pb[D], ya[D], yb[D];
+          for (int a = 0; a < D; ++a) {
+            const double v = W.G.v[k * D + a];
+            pa[a] = W.G.p0[k * D + a] - 2.01 * v;
+            pb[a] = W.G.p0[k * D + a] + (n - 1.0 + 2.01) * v;
+          }
+          ty(pa, ya);
+          ty(pb, yb);
+          double l, h;
+          capsule_times<D>(y0, vy, ya, yb, R, l, h);
+          // only the part inside the time grid counts (a run the beam's line
+          // crossed before the first time must not stretch the window)
+          l = fmax(l, W.ts[0]);
+          h = fmin(h, W.ts[W.nt - 1]);
+          if (l <= h) { tlo = fmin(tlo, l); thi = fmax(thi, h); }
+        }
+        if (!(tlo <= thi)) {
+          on = (int)W.nt;
+          off = -1;
+        } else {
+          // first row with ts >= tlo, last row with ts <= thi (ts increasing)
+          long long a = 0, b = W.nt;
+          while (a < b) { const long long m = (a + b) >> 1; if (W.ts[m] < tlo) a = m + 1; else b = m; }
+          on = (int)a;
+          a = 0; b = W.nt;
+          while (a < b) { const long long m = (a + b) >> 1; if (W.ts[m] <= thi) a = m + 1; else b = m; }
+          off = (int)a - 1;
+        }
+      }
+    }
+  }
+  W.win[e] = make_int2(on, off);
+  // order: eight ranges of the table (so that every beam range of the launch
+  // draws from all of them alike), then window start and end, then entry
+  const unsigned long long rg = (unsigned long long)((e * 8) / W.total);
+  W.key[e] = (rg << 61) | ((unsigned long long)(unsigned)on << 45) |
+             ((unsigned long long)(unsigned)(off + 1) << 29) | (unsigned long long)e;
+  W.idx[e] = e;
+}
+
+// entries in window order: perm[bi] = sorted entry, rows_act[bi] its window
+__global__ void k_window_order(const long long* __restrict__ sidx, const int2* __restrict__ win,
+                               long long total, int32_t* __restrict__ perm,
+                               int2* __restrict__ rows_act) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long e = sidx[i];
+  perm[i] = (int32_t)e;
+  rows_act[i] = win[e];
+}
+
+size_t round256(size_t x) { return (x + 255) / 256 * 256; }
+
+// bytes of the window tables of `total` entries
+size_t window_bytes(long long total) {
+  const long long P = bh::sort_pairs_capacity(total);
+  return round256(16) + 4 * round256((size_t)P * 8) + round256((size_t)total * 8) +
+         round256((size_t)total * 4) + round256((size_t)total * 8);
+}
+
+// the windows and the entry order of a forward launch, carved from ws
+int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt, void* ws,
+                  const int32_t** perm, const int2** rows_act, hipStream_t st) {
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* r = p; p += round256(bytes); return (void*)r; };
+  const long long total = a.total;
+  const long long P = bh::sort_pairs_capacity(total);
+  double* trange = (double*)take(16);
+  unsigned long long* key = (unsigned long long*)take((size_t)P * 8);
+  long long* idx = (long long*)take((size_t)P * 8);
+  unsigned long long* key2 = (unsigned long long*)take((size_t)P * 8);
+  long long* idx2 = (long long*)take((size_t)P * 8);
+  int2* win = (int2*)take((size_t)total * 8);
+  int32_t* pm = (int32_t*)take((size_t)total * 4);
+  int2* ra = (int2*)take((size_t)total * 8);
+  hipLaunchKernelGGL(k_time_range, dim3(1), dim3(256), 0, st, ts, nt, trange);
+  WinArgs W{};
+  W.b = a.b;
+  W.count = a.count;
+  W.total = total;
+  W.npos = a.npos;
+  W.mirror = a.mirror;
+  W.c = a.c;
+  W.ts = ts;
+  W.nt = nt;
+  W.G = a.G;
+  W.E = a.E;
+  W.trange = trange;
+  W.key = key;
+  W.idx = idx;
+  W.win = win;
+  const unsigned g = bh::grid1d(total, 128);
+  if (ndim == 1) hipLaunchKernelGGL(k_beam_windows<1>, dim3(g), dim3(128), 0, st, W);
+  else if (ndim == 2) hipLaunchKernelGGL(k_beam_windows<2>, dim3(g), dim3(128), 0, st, W);
+  else hipLaunchKernelGGL(k_beam_windows<3>, dim3(g), dim3(128), 0, st, W);
+  unsigned long long* ko;
+  long long* io;
+  if (int rc = bh::sort_pairs(key, idx, key2, idx2, total, &ko, &io, st)) return rc;
+  hipLaunchKernelGGL(k_window_order, dim3(bh::grid1d(total, 256)), dim3(256), 0, st, io, win,
+                     total, pm, ra);
+  BH_LAUNCH_CHECK();
+  *perm = pm;
+  *rows_act = ra;
+  return BH_OK;
+}
+
 // Launch shape: groups per block GPB in {32, 16, 8, 4} (fewer groups per
 // block = more blocks but more record work per group), then a beam split so
 // that the grid covers the chip several times over (blocks resident per CU
@@ -489,6 +803,15 @@ extern "C" size_t bh_forward_sum_culled_workspace_size(int64_t nt, int64_t S) {
   return (size_t)8 * nt * S * sizeof(double);
 }
 
+// The partial sums of the beam ranges, then the activity windows and the
+// window order of the forward entries (mirror: 2 count of them).
+extern "C" size_t bh_forward_culled_workspace_size(int64_t count, int mirror, int64_t nt,
+                                                   int64_t S) {
+  if (nt <= 0 || S <= 0 || count <= 0) return bh_forward_sum_culled_workspace_size(nt, S);
+  return round256(bh_forward_sum_culled_workspace_size(nt, S)) +
+         window_bytes(count * (mirror ? 2 : 1));
+}
+
 namespace {
 int forward_culled(const char* name, const bh_fwd_beams* b, int ndim, long long npos, int mirror,
                    double c, const double* ts, int64_t nt, const bh_segments* seg,
@@ -530,15 +853,27 @@ int forward_culled(const char* name, const bh_fwd_beams* b, int ndim, long long 
   a.pairs = pairs;
   a.ostride = ostride;
   const long long nchunk = (a.total + kRecChunk - 1) / kRecChunk;
+  // activity windows (closed-form beams, a cut, room for the tables after
+  // the partial sums): the entries in window order
+  size_t part_bytes = ws_bytes;
+  a.perm = nullptr;
+  a.rows_act = nullptr;
+  const size_t part_std = round256(bh_forward_sum_culled_workspace_size(nt, seg->S));
+  if (!traj && cull_log2 > 0.0 && ws && ws_bytes >= part_std + window_bytes(a.total) &&
+      a.total < (1ll << 29) && nt < 65535) {
+    if (int rc = build_windows(ndim, a, ts, nt, (char*)ws + part_std, &a.perm, &a.rows_act, st))
+      return rc;
+    part_bytes = part_std;
+  }
   // real sum: 2 Re sum A G.  Complex sum: Re = half of it, Im = half of the
   // real sum with every arg(A) lowered by a quarter cycle (cos(x - pi/2) = sin x)
   for (int part = 0; part < (complex_out ? 2 : 1); ++part) {
     a.scale = complex_out ? 0.5 : 1.0;
     a.phase = part ? -0.25 : 0.0;
     a.ooff = part;
-    if (ndim == 1) { if (f64) launch_culled<1, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<1, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
-    else if (ndim == 2) { if (f64) launch_culled<2, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<2, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
-    else { if (f64) launch_culled<3, true, false>(a, nt, nchunk, ws, ws_bytes, st); else launch_culled<3, false, false>(a, nt, nchunk, ws, ws_bytes, st); }
+    if (ndim == 1) { if (f64) launch_culled<1, true, false>(a, nt, nchunk, ws, part_bytes, st); else launch_culled<1, false, false>(a, nt, nchunk, ws, part_bytes, st); }
+    else if (ndim == 2) { if (f64) launch_culled<2, true, false>(a, nt, nchunk, ws, part_bytes, st); else launch_culled<2, false, false>(a, nt, nchunk, ws, part_bytes, st); }
+    else { if (f64) launch_culled<3, true, false>(a, nt, nchunk, ws, part_bytes, st); else launch_culled<3, false, false>(a, nt, nchunk, ws, part_bytes, st); }
     BH_LAUNCH_CHECK();
   }
   return BH_OK;

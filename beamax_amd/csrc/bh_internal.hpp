@@ -57,6 +57,14 @@ size_t kth_key_ws_bytes();
 void kth_largest_key(const unsigned long long* keys, long long n, long long k,
                      unsigned long long* out, void* ws, hipStream_t s);
 
+// Ascending sort of n (key, index) pairs by (key, index): bitonic 512-pair
+// tiles, then stable merge levels.  key/idx and key2/idx2 each hold
+// sort_pairs_capacity(n) entries (a power of two, padded here); the sorted
+// pairs end in one of the two buffers, returned in *key_out / *idx_out.
+long long sort_pairs_capacity(long long n);
+int sort_pairs(unsigned long long* key, long long* idx, unsigned long long* key2, long long* idx2,
+               long long n, unsigned long long** key_out, long long** idx_out, hipStream_t s);
+
 // Status words of the kernels that guard beam / coefficient indices (one per
 // translation unit): OR the bits into *bits, clear them if asked.  Both
 // synchronise the stream.

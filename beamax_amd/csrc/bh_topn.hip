@@ -345,6 +345,32 @@ __global__ void k_emit_key(const SelState* st, unsigned long long* out) { *out =
 }  // namespace
 
 namespace bh {
+long long sort_pairs_capacity(long long n) {
+  return pow2_at_least(n < kSortTile ? kSortTile : n);
+}
+
+int sort_pairs(unsigned long long* key, long long* idx, unsigned long long* key2, long long* idx2,
+               long long n, unsigned long long** key_out, long long** idx_out, hipStream_t s) {
+  const long long P = sort_pairs_capacity(n);
+  if (P > n)
+    hipLaunchKernelGGL(k_pad, dim3(bh::grid1d(P - n, 256)), dim3(256), 0, s, key, idx, n, P);
+  hipLaunchKernelGGL(k_bitonic_tile, dim3((unsigned)(P / kSortTile)), dim3(256), 0, s, key, idx, P);
+  unsigned long long* ka = key;
+  long long* ia = idx;
+  unsigned long long* kb = key2;
+  long long* ib = idx2;
+  for (long long L = kSortTile; L < P; L <<= 1) {
+    hipLaunchKernelGGL(k_merge_runs, dim3(bh::grid1d(P, 256)), dim3(256), 0, s, ka, ia, kb, ib, P,
+                       L);
+    unsigned long long* tk = ka; ka = kb; kb = tk;
+    long long* ti = ia; ia = ib; ib = ti;
+  }
+  *key_out = ka;
+  *idx_out = ia;
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
 size_t kth_key_ws_bytes() { return 256 * 4 + 256; }
 
 void kth_largest_key(const unsigned long long* keys, long long n, long long k,

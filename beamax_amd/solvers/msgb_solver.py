@@ -342,6 +342,22 @@ class MSGBSolver:
             cache[key] = hit
         return hit
 
+    def _points_device(self, points, d, device):
+        """Evaluation points on the device, cached by their contents (a
+        repeated call neither copies them nor stalls a captured stream)."""
+        import torch
+        if isinstance(points, torch.Tensor) and points.device == torch.device(device) \
+                and points.dtype == torch.float64:
+            return points.reshape(-1, d).contiguous()
+        pn = np.ascontiguousarray(_np(points), dtype=np.float64).reshape(-1, d)
+        key = (pn.shape, hash(pn.tobytes()), str(device))
+        cache = self.__dict__.setdefault("_pts_cache", {})
+        hit = cache.get(key)
+        if hit is None:
+            hit = torch.as_tensor(pn, device=device)
+            cache[key] = hit
+        return hit
+
     def _segments(self, points, dx, device):
         from ..segments import Segments
         if hasattr(points, "data_ptr"):
@@ -389,8 +405,8 @@ class MSGBSolver:
                 out = torch.empty((tsd.numel(), seg.S), dtype=dt, device=dev)
             b = beams.struct()
             ss = seg.struct()
-            ws = _lib.workspace(
-                _lib.load().bh_forward_sum_culled_workspace_size(tsd.numel(), seg.S), dev)
+            ws = _lib.workspace(_lib.load().bh_forward_culled_workspace_size(
+                beams.count, int(npos is None), tsd.numel(), seg.S), dev)
             _lib.call("bh_forward_sum_modes", C.byref(b), d, 0 if npos is None else int(npos),
                       int(npos is None), self._c0(domain), _lib.ptr(tsd), tsd.numel(), C.byref(ss),
                       self.cull_log2, _lib.PRECISION[self.precision], int(not self.use_real),
@@ -407,15 +423,14 @@ class MSGBSolver:
                     out = torch.empty((tsd.numel(), seg.S), dtype=dt, device=dev)
                 b = beams.struct()
                 ss = seg.struct()
-                ws = _lib.workspace(
-                    _lib.load().bh_forward_sum_culled_workspace_size(tsd.numel(), seg.S), dev)
+                ws = _lib.workspace(_lib.load().bh_forward_culled_workspace_size(
+                    beams.count, 1, tsd.numel(), seg.S), dev)
                 _lib.call("bh_forward_sum_culled", C.byref(b), d, 1, self._c0(domain), _lib.ptr(tsd),
                           tsd.numel(), C.byref(ss), self.cull_log2, _lib.PRECISION[self.precision],
                           _lib.ptr(out), _lib.ptr(self.pair_counter), _lib.ptr(ws), ws.numel(),
                           _lib.stream_handle())
                 return out
-        pos = torch.as_tensor(np.ascontiguousarray(sensor_positions, dtype=np.float64).reshape(-1, d),
-                              device=dev)
+        pos = self._points_device(sensor_positions, d, dev)
         tsd = self._ts_device(ts, dev)
         nt, S = tsd.numel(), pos.shape[0]
         dt = torch.float64 if self.precision == "fp64" else torch.float32
@@ -735,8 +750,7 @@ class MSGBSolver:
                           _lib.ptr(self.pair_counter), _lib.ptr(ws), ws.numel(),
                           _lib.stream_handle())
                 return out
-        pts = torch.as_tensor(np.ascontiguousarray(points, dtype=np.float64).reshape(-1, d),
-                              device=dev) if not isinstance(points, torch.Tensor) else points
+        pts = self._points_device(points, d, dev)
         P = pts.shape[0]
         dt = torch.float64 if self.precision == "fp64" else torch.float32
         if out is None:

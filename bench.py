@@ -288,8 +288,8 @@ def recon_leg(device, tol=1e-3, seed=0):
     F = FrozenForward.from_field(s, img.reshape(N), dom, sensors, ts, wpt)
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
-    cgnr(F, d, tol=tol, maxiter=5)                      # warm-up (kernels, workspaces)
-    x, info = cgnr(F, d, tol=tol, maxiter=1000)
+    cgnr(F, d, tol=tol, maxiter=5, graph=True)          # warm-up (kernels, workspaces)
+    x, info = cgnr(F, d, tol=tol, maxiter=1000, graph=True)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     ev[0].record()
     w = F.apply(x)
@@ -308,7 +308,7 @@ def recon_leg(device, tol=1e-3, seed=0):
         x0 = warm_start(net, F, d, img)
         torch.cuda.synchronize()
         t_init = time.perf_counter() - t1
-        x1, info1 = cgnr(F, d, x0=x0, tol=tol, maxiter=1000)
+        x1, info1 = cgnr(F, d, x0=x0, tol=tol, maxiter=1000, graph=True)
         learned = {
             "method": "CGNR from the U-Net estimate of p0 (adjoint image in, least-squares "
                       "scaled), trained on phantom seeds 1000-5095",
@@ -326,7 +326,7 @@ def recon_leg(device, tol=1e-3, seed=0):
         "converged": info["converged"],
         "final_residual": info["residuals"][-1],
         "residual": "||F^T(d - F x)|| / ||F^T d|| (normal equations, plain operator)",
-        "method": "CGNR, no preconditioner",
+        "method": "CGNR, no preconditioner, each iteration one captured HIP graph replay",
         "rel_err_vs_phantom": float(((x - p0).norm() / p0.norm()).item()),
         "apply_ms": round(ev[0].elapsed_time(ev[1]), 3),
         "transpose_ms": round(ev[1].elapsed_time(ev[2]), 3),

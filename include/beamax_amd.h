@@ -28,6 +28,7 @@
  *                       compute_gaussian_beam_real (forward_solver_utils.py:677-713,
  *                       beamax/gb/gb_solvers.py:356-422, beamax/gb/core.py:194-329)
  *   bh_solve_hom_diag   gb_solvers.solve_hom_diag (beamax/gb/gb_solvers.py:356-422)
+ *   bh_solve_hom_TR     gb_solvers.solve_hom_TR (beamax/gb/gb_solvers.py:484-594)
  *   bh_adjoint_source   _form_adjoint_source (msgb_solver.py:68-120)
  *   bh_adjoint_params   compute_adj_parameters + compute_TR_parameters +
  *                       solve_hom_TR at t = 0 (adjoint_solver_utils.py:75-220,
@@ -231,6 +232,13 @@ int bh_solve_hom_diag(int64_t b, int ndim, const double* x0, const double* p0,
                       const double* alpha, const double* a0, const double* mode, double c,
                       const double* ts, int64_t nt, double* xt, double* pt, double* Mt,
                       double* At, bh_stream s);
+/* solve_hom_TR as a SolverFn (gb_solvers.py:484-594): terminal states xT, pT
+ * (b x d), mT (b x d x d complex), aT (b complex), mode (b); per-beam times
+ * ts (b x nt, row-major), each measured from ts[beam][0]; outputs xt, pt
+ * (b,nt,d), Mt (b,nt,d,d complex), At (b,nt complex). */
+int bh_solve_hom_TR(int64_t b, int ndim, const double* xT, const double* pT, const double* mT,
+                    const double* aT, const double* mode, double c, const double* ts, int64_t nt,
+                    double* xt, double* pt, double* Mt, double* At, bh_stream s);
 
 /* Adjoint source -c^2 Re IFFT_t(2 pi i f FFT_t(w r)) along axis 0 of an
  * (nt, ns) FP64 array; window (nullable) has nt entries (window_2d = 0) or
@@ -302,8 +310,17 @@ typedef struct {
  * bh_forward_sum_culled_workspace_size the beam range is split over more
  * workgroups and the partial sums are added in a fixed order.  The static
  * form needs at least bh_static_sum_workspace_size bytes (records) and uses
- * bh_static_sum_culled_workspace_size for the split. */
+ * bh_static_sum_culled_workspace_size for the split.
+ * With the larger size of bh_forward_culled_workspace_size (count = beams of
+ * the table, mirror as passed to the sum) the forward forms of closed-form
+ * beams also compute each (beam, mode)'s time window -- the rows on which
+ * its envelope can reach a run, from the metric of its envelope at the
+ * latest time, which bounds every earlier one for alpha = i b, b > 0 -- and
+ * visit the beams in window order, so that workgroups skip the records of
+ * beams outside their window; the evaluated pairs are the same, the order in
+ * which they are added differs. */
 size_t bh_forward_sum_culled_workspace_size(int64_t nt, int64_t S);
+size_t bh_forward_culled_workspace_size(int64_t count, int mirror, int64_t nt, int64_t S);
 int bh_forward_sum_culled(const bh_fwd_beams* b, int ndim, int mirror, double c,
                           const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
                           int precision, void* out, unsigned long long* pairs, void* ws,

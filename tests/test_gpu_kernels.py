@@ -228,6 +228,59 @@ def test_solve_hom_diag_matches_oracle(gpu, d):
     assert rel(rA, _n(At)[..., 0]) < 1e-13
 
 
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_solve_hom_TR_matches_oracle_and_reverses_hom_diag(gpu, d):
+    """gb_solvers.solve_hom_TR as a SolverFn (gb_solvers.py:484-594) against
+    the oracle's closed form, for per-beam (t_start, 0) intervals and for a
+    shared timeline (column j is the oracle at t_start = ts[0] - ts[j]); it
+    undoes solve_hom_diag (the reference's test_gb_reversible hom part,
+    tests/test_gb.py:236-266) and matches solve_ODE_batch_t in a constant
+    medium (tests/test_gb.py:269-299)."""
+    import torch
+    from beamax_amd.gb import gb_solvers
+    rng = np.random.default_rng(40 + d)
+    b = 29
+    x0 = rng.random((b, d)) * 0.01
+    xi = rng.standard_normal((b, d)) * 3000
+    p0 = 2 * np.pi * xi / np.linalg.norm(xi, axis=1, keepdims=True)
+    om = np.linalg.norm(xi, axis=1)
+    alpha = 1j * (rng.random((b, d)) + 0.2) * 2 * (np.pi * 1e3) ** 2 / om[:, None]
+    a0 = rng.standard_normal(b) + 1j * rng.standard_normal(b)
+    mode = np.where(rng.random(b) < 0.5, 1.0, -1.0)
+    T = 2e-5
+    xt, pt, Mt, At = obeams.solve_hom_diag(x0, p0, alpha, a0, mode, np.array([0.0, T]), 1500.0)
+    xT, pT, mT, aT = xt[:, 1], pt[:, 1], Mt[:, 1], At[:, 1]
+    # per-beam intervals (t_start, 0), t_start varying per beam
+    t_start = rng.uniform(0.2, 1.0, b) * T
+    tsb = np.stack([t_start, np.zeros(b)], axis=1)
+    gx, gp, gM, gA = gb_solvers.solve_hom_TR(_t(xT, gpu), _t(pT, gpu), _t(mT, gpu),
+                                             _t(aT, gpu), _t(mode, gpu), _t(tsb, gpu), 1500.0)
+    rx, rp, rM, rA = obeams.solve_hom_TR(xT, pT, mT, aT, mode, t_start, 1500.0)
+    assert gx.shape == (b, 2, d) and gA.shape == (b, 2, 1)
+    assert rel(rx, _n(gx)[:, 1]) < 1e-14 and rel(rp, _n(gp)[:, 1]) < 1e-15
+    assert rel(rM, _n(gM)[:, 1]) < 1e-13 and rel(rA, _n(gA)[:, 1, 0]) < 1e-13
+    assert rel(xT, _n(gx)[:, 0]) < 1e-15 and rel(aT, _n(gA)[:, 0, 0]) < 1e-15
+    # a shared timeline of 9 times
+    ts = np.linspace(T, 0.0, 9)
+    gx, gp, gM, gA = gb_solvers.solve_hom_TR(_t(xT, gpu), _t(pT, gpu), _t(mT, gpu),
+                                             _t(aT, gpu), _t(mode, gpu), _t(ts, gpu), 1500.0)
+    for j in range(9):
+        rx, rp, rM, rA = obeams.solve_hom_TR(xT, pT, mT, aT, mode, np.full(b, ts[0] - ts[j]),
+                                             1500.0)
+        assert rel(rM, _n(gM)[:, j]) < 1e-13 and rel(rA, _n(gA)[:, j, 0]) < 1e-13
+    # back to t = 0: the forward's initial state
+    assert rel(x0, _n(gx)[:, -1]) < 1e-13
+    assert rel(np.stack([np.diag(a) for a in alpha]), _n(gM)[:, -1]) < 1e-11
+    assert rel(a0, _n(gA)[:, -1, 0]) < 1e-11
+    # the batch ODE solver in a constant medium
+    ox, op_, oM, oA = gb_solvers.solve_ODE_batch_t(_t(xT, gpu), _t(pT, gpu), _t(mT, gpu),
+                                                   _t(aT, gpu), _t(mode, gpu), _t(tsb, gpu), 1500.0)
+    hx, hp, hM, hA = gb_solvers.solve_hom_TR(_t(xT, gpu), _t(pT, gpu), _t(mT, gpu), _t(aT, gpu),
+                                             _t(mode, gpu), _t(tsb, gpu), 1500.0)
+    assert rel(_n(hx), _n(ox)) < 1e-8 and rel(_n(hM), _n(oM)) < 1e-6
+    assert rel(_n(hA), _n(oA)) < 1e-6
+
+
 def _random_beams(rng, B, d, L, kmax):
     x0 = rng.random((B, d)) * L
     xi = rng.standard_normal((B, d))
@@ -391,6 +444,56 @@ def test_culled_forward_matches_dense(gpu, d, kind):
         assert 0 < pairs <= dense_pairs
         if E == 0.0:
             assert pairs == dense_pairs
+
+
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("kind", ["edges", "grid"])
+def test_culled_forward_windows_keep_every_pair(gpu, d, kind):
+    """With the activity windows (the larger workspace) the culled forward
+    visits the beams in window order and skips records outside their
+    windows: the evaluated (beam, unit) pairs are exactly the same -- every
+    active pair lies inside its beam's window -- and the sums agree with the
+    run without windows to the rounding of the changed order.  Beams whose
+    alpha has a real part get every row."""
+    import torch
+    from beamax_amd import _lib
+    from beamax_amd.segments import Segments
+    from beamax_amd.solvers.msgb_solver import ForwardBeams
+    rng = np.random.default_rng(50 + d)
+    n = 96 if d == 2 else 24
+    dx = 1e-4
+    L = n * dx
+    B = 600
+    x0, p0, om, alpha, a0 = _random_beams(rng, B, d, L, 0.5 / dx)
+    alpha[:20] = alpha[:20] + 1e3     # a few beams outside the window rule
+    fb = ForwardBeams(B, d, gpu)
+    for name, v in (("x0", x0), ("p0", p0), ("omega", om), ("alpha", alpha), ("a0", a0)):
+        getattr(fb, name).copy_(_t(v, gpu))
+    pts = _grid_points(kind, n, d, dx)
+    ts = np.arange(0, 1.5 * L / 1500.0, 2e-8 * 5)
+    seg = Segments(pts, [dx] * d, gpu)
+    ts_d = _t(ts, gpu)
+    lib = _lib.load()
+    for prec in ("fp32_split", "fp64"):
+        for E in (32.0, 40.0):
+            res = {}
+            for wins in (False, True):
+                nb = (lib.bh_forward_culled_workspace_size(B, 1, ts.size, seg.S) if wins
+                      else lib.bh_forward_sum_culled_workspace_size(ts.size, seg.S))
+                ws = _lib.workspace(nb, gpu)
+                out = torch.empty((ts.size, seg.S),
+                                  dtype=torch.float64 if prec == "fp64" else torch.float32,
+                                  device=gpu)
+                cnt = torch.zeros(1, dtype=torch.uint64, device=gpu)
+                ss = seg.struct()
+                _lib.call("bh_forward_sum_culled", C.byref(fb.struct()), d, 1, 1500.0,
+                          _lib.ptr(ts_d), ts.size, C.byref(ss), E, _lib.PRECISION[prec],
+                          _lib.ptr(out), _lib.ptr(cnt), _lib.ptr(ws), nb, _lib.stream_handle())
+                res[wins] = (_n(out).astype(np.float64), int(cnt.item()))
+            (o0, c0), (o1, c1) = res[False], res[True]
+            assert c0 == c1 and c0 > 0, (prec, E, c0, c1)
+            tol = 1e-12 if prec == "fp64" else 2e-6
+            assert np.max(np.abs(o0 - o1)) <= tol * np.max(np.abs(o0)), (prec, E)
 
 
 @pytest.mark.parametrize("d", [2, 3])
