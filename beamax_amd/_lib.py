@@ -143,8 +143,12 @@ SIGNATURES = {
     "bh_nn_gemm_workspace_size": (_SZ, [_I64, _I64, _I64, C.c_int]),
     "bh_nn_gemm": (C.c_int, [_P, C.c_int, _I64, _P, C.c_int, _I64, _I64, _I64, _I64, C.c_float,
                              _P, C.c_int, _P, C.c_int, _I64, C.c_int, _P, _SZ, _P]),
-    "bh_nn_im2col3": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, _P]),
-    "bh_nn_col2im3": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P]),
+    "bh_nn_conv3": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int,
+                              C.c_float, _P, C.c_int, _P]),
+    "bh_nn_conv3_wflip": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    "bh_nn_conv3_dw_workspace_size": (_SZ, [_I64, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bh_nn_conv3_dw": (C.c_int, [_P, _P, _I64, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _SZ,
+                                 _P]),
     "bh_nn_pool2": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P]),
     "bh_nn_up2": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P]),
     "bh_nn_add": (C.c_int, [_P, _P, _I64, _P, _P]),

@@ -1,7 +1,9 @@
 // Kernels of the learned reconstruction network (a U-Net on 2-D images,
-// NHWC bf16 activations): a bf16 MFMA GEMM with FP32 accumulation (convolutions
-// run as im2col + GEMM, their gradients as GEMMs + col2im), and the
-// elementwise pieces of forward and backward passes and of the Adam update.
+// NHWC bf16 activations): bf16 MFMA matrix products with FP32 accumulation
+// (a general GEMM, and 3x3 convolutions with their weight and input
+// gradients as implicit GEMMs that gather their operands straight from the
+// activations), and the elementwise pieces of forward and backward passes
+// and of the Adam update.
 // No reference counterpart: the reference has no reconstruction network.
 //
 // GEMM: C[M][N] = alpha * sum_k A(m, k) B(k, n) (+ bias[n], ReLU), with
@@ -119,48 +121,6 @@ __global__ void k_sum_partials(const float* __restrict__ part, int nsplit, long 
   float s = 0.f;
   for (int z = 0; z < nsplit; ++z) s += part[z * n + i];
   out[i] = s;
-}
-
-// col[p][tap * C + c] = X[n][y + dy][x + dx][c] (zero outside), tap = 3 (dy + 1) + dx + 1
-__global__ void k_im2col3(const bfbits* __restrict__ X, long long B, int H, int W, int C,
-                          bfbits* __restrict__ col) {
-  const int cv = C / 8;
-  const long long total = B * H * W * 9LL * cv;
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const int c8 = (int)(i % cv);
-  const int tap = (int)((i / cv) % 9);
-  const long long p = i / (9LL * cv);
-  const int x = (int)(p % W), y = (int)((p / W) % H);
-  const long long n = p / ((long long)W * H);
-  const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-  uint4 v = make_uint4(0, 0, 0, 0);
-  if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-    v = *reinterpret_cast<const uint4*>(X + ((n * H + yy) * W + xx) * C + c8 * 8);
-  *reinterpret_cast<uint4*>(col + p * 9LL * C + tap * (long long)C + c8 * 8) = v;
-}
-
-// dX[n][y][x][c] = sum_tap dcol[(n, y - dy, x - dx)][tap * C + c] (the adjoint
-// of k_im2col3), times (mask > 0) when mask is given; FP32 accumulation in tap
-// order, bf16 or FP32 output
-__global__ void k_col2im3(const float* __restrict__ dcol, long long B, int H, int W, int C,
-                          const bfbits* __restrict__ mask, int out_bf16, void* __restrict__ dX) {
-  const long long total = B * H * W * (long long)C;
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const int c = (int)(i % C);
-  const long long p = i / C;
-  const int x = (int)(p % W), y = (int)((p / W) % H);
-  const long long n = p / ((long long)W * H);
-  float s = 0.f;
-  for (int tap = 0; tap < 9; ++tap) {
-    const int ys = y - (tap / 3 - 1), xs = x - (tap % 3 - 1);
-    if (ys < 0 || ys >= H || xs < 0 || xs >= W) continue;
-    s += dcol[((n * H + ys) * W + xs) * 9LL * C + tap * (long long)C + c];
-  }
-  if (mask && !(bf2f(mask[i]) > 0.f)) s = 0.f;
-  if (out_bf16) ((bfbits*)dX)[i] = f2bf(s);
-  else ((float*)dX)[i] = s;
 }
 
 // 2x2 average pooling and its adjoint (NHWC)
@@ -339,7 +299,407 @@ __global__ void k_bf16_to_f32(const bfbits* __restrict__ x, long long n, float s
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
+// ---------------------------------------------------------------------------
+// Implicit-GEMM 3x3 convolution (NHWC, same padding):
+//   Y[p][o] = act(alpha sum_k A(p, k) Wt[o][k] + bias[o]),
+//   A(p, k) = X[n][y + dy][x + dx][c] (zero outside), k = tap C + c,
+//   tap = 3 (dy + 1) + dx + 1
+// without a column buffer: each k step the workgroup gathers its 128 pixels'
+// 32 k values (four 8-channel segments, each inside one tap since C is a
+// multiple of 8) straight from X into LDS.  Four waves of 32 pixels x 16 NB
+// output channels, v_mfma_f32_16x16x32_bf16 (the GEMM's fragment layout);
+// LDS double-buffered with the next step's global loads held in registers
+// while the current step multiplies, one barrier per step.
+// ---------------------------------------------------------------------------
+constexpr int CM = 128, CK = 32, CLD = CK + 8;
+
+struct ConvArgs {
+  const bfbits* X;
+  const bfbits* Wt;
+  const float* bias;
+  void* Y;
+  long long P;
+  int H, Wd, C, Cout;
+  float alpha;
+  int relu, out_bf16;
+};
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv3_igemm(ConvArgs a) {
+  constexpr int BNc = 16 * NB;
+  constexpr int BLD = (BNc * 4 + 255) / 256;   // weight segments per thread
+  __shared__ __attribute__((aligned(16))) bfbits As[2][CM][CLD];
+  __shared__ __attribute__((aligned(16))) bfbits Bs[2][BNc][CLD];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const long long p0 = blockIdx.x * (long long)CM;
+  const int n0 = blockIdx.y * BNc;
+  const int K = 9 * a.C;
+  const int seg = (t & 3) * 8;
+  int py[2], px[2];
+  long long pn[2];
+  bool pv[2];
+  for (int h = 0; h < 2; ++h) {
+    const long long p = p0 + (t >> 2) + 64 * h;
+    pv[h] = p < a.P;
+    const long long q = pv[h] ? p : 0;
+    px[h] = (int)(q % a.Wd);
+    py[h] = (int)((q / a.Wd) % a.H);
+    pn[h] = q / ((long long)a.Wd * a.H);
+  }
+  uint4 ra[2], rb[BLD];
+  auto gload = [&](int k0) {
+    const int k = k0 + seg;
+    const bool kin = k < K;
+    const int tap = kin ? k / a.C : 0;
+    const int ci = kin ? k - tap * a.C : 0;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    for (int h = 0; h < 2; ++h) {
+      const int yy = py[h] + dy, xx = px[h] + dx;
+      const bool ok = pv[h] && kin && yy >= 0 && yy < a.H && xx >= 0 && xx < a.Wd;
+      ra[h] = ok ? *reinterpret_cast<const uint4*>(
+                       a.X + ((pn[h] * a.H + yy) * a.Wd + xx) * (long long)a.C + ci)
+                 : make_uint4(0, 0, 0, 0);
+    }
+    for (int j = 0; j < BLD; ++j) {
+      const int i = t + 256 * j;
+      const int row = i >> 2, kk = k0 + (i & 3) * 8;
+      const bool ok = i < BNc * 4 && n0 + row < a.Cout && kk < K;
+      rb[j] = ok ? *reinterpret_cast<const uint4*>(a.Wt + (long long)(n0 + row) * K + kk)
+                 : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto sstore = [&](int buf) {
+    for (int h = 0; h < 2; ++h) *reinterpret_cast<uint4*>(&As[buf][(t >> 2) + 64 * h][seg]) = ra[h];
+    for (int j = 0; j < BLD; ++j) {
+      const int i = t + 256 * j;
+      if (i < BNc * 4) *reinterpret_cast<uint4*>(&Bs[buf][i >> 2][(i & 3) * 8]) = rb[j];
+    }
+  };
+  floatx4 acc[2][NB];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < NB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int nsteps = (K + CK - 1) / CK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int st = 0; st < nsteps; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nsteps;
+    if (more) gload((st + 1) * CK);
+    bf16x8 av[2], bv[NB];
+    for (int i = 0; i < 2; ++i)
+      av[i] = __builtin_bit_cast(bf16x8,
+                                 *reinterpret_cast<const uint4*>(&As[buf][w * 32 + i * 16 + fr][fk]));
+    for (int j = 0; j < NB; ++j)
+      bv[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Bs[buf][j * 16 + fr][fk]));
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < NB; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    if (more) sstore(buf ^ 1);
+    __syncthreads();
+  }
+  const int cc = lane & 15, cr = (lane >> 4) * 4;
+  for (int j = 0; j < NB; ++j) {
+    const int n = n0 + j * 16 + cc;
+    if (n >= a.Cout) continue;
+    const float bv = a.bias ? a.bias[n] : 0.f;
+    for (int i = 0; i < 2; ++i)
+      for (int r = 0; r < 4; ++r) {
+        const long long p = p0 + w * 32 + i * 16 + cr + r;
+        if (p >= a.P) continue;
+        float v = a.alpha * acc[i][j][r] + bv;
+        if (a.relu) v = fmaxf(v, 0.f);
+        if (a.out_bf16) ((bfbits*)a.Y)[p * a.Cout + n] = f2bf(v);
+        else ((float*)a.Y)[p * a.Cout + n] = v;
+      }
+  }
+}
+
+// Weight gradient of the 3x3 convolution without a column buffer:
+//   dW[o][k] = sum_p dZ[p][o] A(p, k)   (A as in k_conv3_igemm)
+// with the pixels as the MFMA reduction dimension.  Both operands are stored
+// pixel-major, so each 32-pixel step stages them transposed: a staging thread
+// loads an 8 pixel x 8 channel block (eight 16-byte loads, X gathered at the
+// block's tap, zero outside the image), transposes it in registers and writes
+// eight 16-byte LDS rows of 8 pixels.  Workgroup tile: 16 MO output channels x
+// 128 k, four waves of 16 MO x 32; grid z splits the pixels into chunks whose
+// FP32 partial tiles are added in order afterwards (deterministic).
+constexpr int DWK = 128, DWP = 32, DWLD = DWP + 8;
+
+struct ConvDwArgs {
+  const bfbits* X;
+  const bfbits* dZ;
+  float* out;
+  long long P;
+  int H, Wd, C, Cout;
+  long long chunk;          // pixels per split, a multiple of DWP
+  long long split_stride;   // floats between split partials
+};
+
+// o[j] = column j of the 8 x 8 bf16 block whose row i is r[i]
+__device__ inline void transpose8x8(const uint4 (&r)[8], uint4 (&o)[8]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    unsigned lo[4], hi[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const unsigned x = (&r[2 * b].x)[j], y = (&r[2 * b + 1].x)[j];
+      lo[b] = (x & 0xffffu) | (y << 16);
+      hi[b] = (x >> 16) | (y & 0xffff0000u);
+    }
+    o[2 * j] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+    o[2 * j + 1] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+  }
+}
+
+template <int MO>
+__global__ __launch_bounds__(256) void k_conv3_dw(ConvDwArgs a) {
+  constexpr int BO = 16 * MO;
+  __shared__ __attribute__((aligned(16))) bfbits Zs[2][BO][DWLD];
+  __shared__ __attribute__((aligned(16))) bfbits Xs[2][DWK][DWLD];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int k0 = blockIdx.x * DWK, o0 = blockIdx.y * BO;
+  const int K = 9 * a.C;
+  const long long pbeg = blockIdx.z * a.chunk;
+  const long long pend = min(a.P, pbeg + a.chunk);
+  // staging roles: threads [0, 8 MO) stage dZ blocks (pixel block t % 4,
+  // channel group t / 4), threads [128, 192) stage X blocks (pixel block,
+  // k group); the other threads only multiply
+  const bool zst = t < 8 * MO;
+  const bool xst = t >= 128 && t < 192;
+  const int sl = zst ? t : t - 128;
+  const int sb = sl & 3, sg = sl >> 2;
+  bool gin = false;
+  int ci = 0, dy = 0, dx = 0;
+  if (zst) gin = o0 + 8 * sg < a.Cout;
+  if (xst) {
+    const int k = k0 + 8 * sg;
+    gin = k < K;
+    const int tap = gin ? k / a.C : 0;
+    ci = gin ? k - tap * a.C : 0;
+    dy = tap / 3 - 1;
+    dx = tap % 3 - 1;
+  }
+  uint4 r[8];
+  auto gload = [&](long long p0) {
+    const long long pb = p0 + 8 * sb;
+    const uint4 z4 = make_uint4(0, 0, 0, 0);
+    if (zst) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        r[i] = (gin && pb + i < pend)
+                   ? *reinterpret_cast<const uint4*>(a.dZ + (pb + i) * a.Cout + o0 + 8 * sg)
+                   : z4;
+    } else if (xst) {
+      const unsigned q = (unsigned)(pb < pend ? pb : 0);
+      int x = (int)(q % (unsigned)a.Wd);
+      const unsigned rr = q / (unsigned)a.Wd;
+      int y = (int)(rr % (unsigned)a.H);
+      long long n = rr / (unsigned)a.H;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int yy = y + dy, xx = x + dx;
+        const bool ok = gin && pb + i < pend && yy >= 0 && yy < a.H && xx >= 0 && xx < a.Wd;
+        r[i] = ok ? *reinterpret_cast<const uint4*>(
+                        a.X + ((n * a.H + yy) * a.Wd + xx) * (long long)a.C + ci)
+                  : z4;
+        if (++x == a.Wd) {
+          x = 0;
+          if (++y == a.H) {
+            y = 0;
+            ++n;
+          }
+        }
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+    if (!zst && !xst) return;
+    uint4 o[8];
+    transpose8x8(r, o);
+    bfbits(*dst)[DWLD] = zst ? Zs[buf] : Xs[buf];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) *reinterpret_cast<uint4*>(&dst[8 * sg + j][8 * sb]) = o[j];
+  };
+  floatx4 acc[MO][2];
+  for (int i = 0; i < MO; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int nsteps = (int)((pend - pbeg + DWP - 1) / DWP);
+  if (nsteps > 0) {
+    gload(pbeg);
+    sstore(0);
+  }
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int st = 0; st < nsteps; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nsteps;
+    if (more) gload(pbeg + (long long)(st + 1) * DWP);
+    bf16x8 za[MO], xb[2];
+    for (int i = 0; i < MO; ++i)
+      za[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Zs[buf][16 * i + fr][fk]));
+    for (int j = 0; j < 2; ++j)
+      xb[j] = __builtin_bit_cast(
+          bf16x8, *reinterpret_cast<const uint4*>(&Xs[buf][32 * w + 16 * j + fr][fk]));
+    for (int i = 0; i < MO; ++i)
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(za[i], xb[j], acc[i][j], 0, 0, 0);
+    if (more) sstore(buf ^ 1);
+    __syncthreads();
+  }
+  float* out = a.out + blockIdx.z * a.split_stride;
+  const int cc = lane & 15, cr = (lane >> 4) * 4;
+  for (int j = 0; j < 2; ++j) {
+    const int k = k0 + 32 * w + 16 * j + cc;
+    if (k >= K) continue;
+    for (int i = 0; i < MO; ++i)
+      for (int q = 0; q < 4; ++q) {
+        const int o = o0 + 16 * i + cr + q;
+        if (o < a.Cout) out[(long long)o * K + k] = acc[i][j][q];
+      }
+  }
+}
+
+// Weights of the input gradient as a convolution of dY:
+// Wf[c][tap' Cout + o] = Wt[o][(8 - tap') C + c]
+__global__ void k_conv3_wflip(const bfbits* __restrict__ Wt, int Cout, int C,
+                              bfbits* __restrict__ Wf) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long n = (long long)Cout * 9 * C;
+  if (i >= n) return;
+  const int o = (int)(i % Cout);
+  const int tp = (int)((i / Cout) % 9);
+  const int c = (int)(i / (9LL * Cout));
+  Wf[i] = Wt[(long long)o * 9 * C + (8 - tp) * (long long)C + c];
+}
+
 }  // namespace
+
+// output-channel tile of the convolution kernels: 16, 32, 48, 64 or 96
+// channels (wider layers take several tiles of 96)
+static int conv_tile(int Cout) {
+  return Cout <= 16 ? 1 : Cout <= 32 ? 2 : Cout <= 48 ? 3 : Cout <= 64 ? 4 : 6;
+}
+
+extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const void* Wt,
+                           int Cout, const float* bias, int relu, float alpha, void* Y,
+                           int out_bf16, bh_stream s) {
+  bh::clear_error();
+  if (B <= 0 || H <= 0 || W <= 0 || C <= 0 || Cout <= 0)
+    BH_FAIL(BH_ERR_ARG, "bh_nn_conv3: shape (%lld, %d, %d, %d) -> %d", (long long)B, H, W, C,
+            Cout);
+  if (!X || !Wt || !Y) BH_FAIL(BH_ERR_ARG, "bh_nn_conv3: null pointer");
+  if (C % 8 || Cout % 8 || !aligned16(X) || !aligned16(Wt))
+    BH_FAIL(BH_ERR_ARG, "bh_nn_conv3: channels (%d, %d) must be multiples of 8 and the "
+                        "operands 16-byte aligned", C, Cout);
+  ConvArgs a;
+  a.X = (const bfbits*)X;
+  a.Wt = (const bfbits*)Wt;
+  a.bias = bias;
+  a.Y = Y;
+  a.P = B * (long long)H * W;
+  a.H = H;
+  a.Wd = W;
+  a.C = C;
+  a.Cout = Cout;
+  a.alpha = alpha;
+  a.relu = relu ? 1 : 0;
+  a.out_bf16 = out_bf16 ? 1 : 0;
+  hipStream_t st = (hipStream_t)s;
+  const unsigned gx = (unsigned)((a.P + CM - 1) / CM);
+  const int nb = conv_tile(Cout);
+  const unsigned gy = (unsigned)((Cout + 16 * nb - 1) / (16 * nb));
+  const dim3 grid(gx, gy);
+  if (nb == 1) hipLaunchKernelGGL(k_conv3_igemm<1>, grid, dim3(256), 0, st, a);
+  else if (nb == 2) hipLaunchKernelGGL(k_conv3_igemm<2>, grid, dim3(256), 0, st, a);
+  else if (nb == 3) hipLaunchKernelGGL(k_conv3_igemm<3>, grid, dim3(256), 0, st, a);
+  else if (nb == 4) hipLaunchKernelGGL(k_conv3_igemm<4>, grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_conv3_igemm<6>, grid, dim3(256), 0, st, a);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+// pixel split of the weight gradient: about 2048 workgroups, at most 256
+// chunks, each a multiple of 32 pixels
+static void conv_dw_split(long long P, int C, int Cout, long long* chunk, int* nsplit) {
+  const long long tiles = (long long)((9 * C + DWK - 1) / DWK) *
+                          ((Cout + 16 * conv_tile(Cout) - 1) / (16 * conv_tile(Cout)));
+  long long ns = (2048 + tiles - 1) / tiles;
+  ns = std::max(1LL, std::min(ns, std::min(256LL, (P + 255) / 256)));
+  long long ch = (P + ns - 1) / ns;
+  ch = (ch + DWP - 1) / DWP * DWP;
+  *chunk = ch;
+  *nsplit = (int)((P + ch - 1) / ch);
+}
+
+extern "C" size_t bh_nn_conv3_dw_workspace_size(int64_t B, int H, int W, int C, int Cout) {
+  if (B <= 0 || H <= 0 || W <= 0 || C <= 0 || Cout <= 0) return 0;
+  long long chunk;
+  int ns;
+  conv_dw_split(B * (long long)H * W, C, Cout, &chunk, &ns);
+  return ns > 1 ? (size_t)ns * Cout * 9 * C * sizeof(float) : 0;
+}
+
+extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, int W, int C,
+                              int Cout, float* dW, void* ws, size_t ws_bytes, bh_stream s) {
+  bh::clear_error();
+  if (B <= 0 || H <= 0 || W <= 0 || C <= 0 || Cout <= 0)
+    BH_FAIL(BH_ERR_ARG, "bh_nn_conv3_dw: shape (%lld, %d, %d, %d) -> %d", (long long)B, H, W, C,
+            Cout);
+  if (!X || !dZ || !dW) BH_FAIL(BH_ERR_ARG, "bh_nn_conv3_dw: null pointer");
+  if (C % 8 || Cout % 8 || !aligned16(X) || !aligned16(dZ))
+    BH_FAIL(BH_ERR_ARG, "bh_nn_conv3_dw: channels (%d, %d) must be multiples of 8 and the "
+                        "operands 16-byte aligned", C, Cout);
+  const long long P = B * (long long)H * W;
+  if (P >= (1LL << 32)) BH_FAIL(BH_ERR_ARG, "bh_nn_conv3_dw: %lld pixels (limit 2^32)", P);
+  ConvDwArgs a;
+  a.X = (const bfbits*)X;
+  a.dZ = (const bfbits*)dZ;
+  a.P = P;
+  a.H = H;
+  a.Wd = W;
+  a.C = C;
+  a.Cout = Cout;
+  int ns;
+  conv_dw_split(P, C, Cout, &a.chunk, &ns);
+  const long long n = (long long)Cout * 9 * C;
+  a.split_stride = n;
+  if (ns > 1) {
+    if (!ws || ws_bytes < (size_t)ns * n * sizeof(float))
+      BH_FAIL(BH_ERR_ARG, "bh_nn_conv3_dw: workspace of %zu bytes, %zu needed", ws_bytes,
+              (size_t)ns * n * sizeof(float));
+    a.out = (float*)ws;
+  } else {
+    a.out = dW;
+  }
+  hipStream_t st = (hipStream_t)s;
+  const int mo = conv_tile(Cout);
+  const dim3 grid((unsigned)((9 * C + DWK - 1) / DWK), (unsigned)((Cout + 16 * mo - 1) / (16 * mo)),
+                  (unsigned)ns);
+  if (mo == 1) hipLaunchKernelGGL(k_conv3_dw<1>, grid, dim3(256), 0, st, a);
+  else if (mo == 2) hipLaunchKernelGGL(k_conv3_dw<2>, grid, dim3(256), 0, st, a);
+  else if (mo == 3) hipLaunchKernelGGL(k_conv3_dw<3>, grid, dim3(256), 0, st, a);
+  else if (mo == 4) hipLaunchKernelGGL(k_conv3_dw<4>, grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_conv3_dw<6>, grid, dim3(256), 0, st, a);
+  BH_LAUNCH_CHECK();
+  if (ns > 1) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(bh::grid1d(n, 256)), dim3(256), 0, st,
+                       (const float*)ws, ns, n, dW);
+    BH_LAUNCH_CHECK();
+  }
+  return BH_OK;
+}
+
+extern "C" int bh_nn_conv3_wflip(const void* Wt, int Cout, int C, void* Wf, bh_stream s) {
+  bh::clear_error();
+  if (Cout <= 0 || C <= 0 || !Wt || !Wf) BH_FAIL(BH_ERR_ARG, "bh_nn_conv3_wflip: arguments");
+  const long long n = (long long)Cout * 9 * C;
+  hipLaunchKernelGGL(k_conv3_wflip, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
+                     (const bfbits*)Wt, Cout, C, (bfbits*)Wf);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
 
 extern "C" size_t bh_nn_gemm_workspace_size(int64_t M, int64_t N, int64_t K, int nsplit) {
   if (nsplit <= 1 || M <= 0 || N <= 0 || K <= 0) return 0;
@@ -407,32 +767,6 @@ extern "C" int bh_nn_gemm(const void* A, int ta, int64_t lda, const void* B, int
 static int check_img(const char* name, int64_t B, int H, int W, int C) {
   if (B <= 0 || H <= 0 || W <= 0 || C <= 0)
     BH_FAIL(BH_ERR_ARG, "%s: shape (%lld, %d, %d, %d)", name, (long long)B, H, W, C);
-  return BH_OK;
-}
-
-extern "C" int bh_nn_im2col3(const void* X, int64_t B, int H, int W, int C, void* col,
-                             bh_stream s) {
-  bh::clear_error();
-  if (int rc = check_img("bh_nn_im2col3", B, H, W, C)) return rc;
-  if (!X || !col) BH_FAIL(BH_ERR_ARG, "bh_nn_im2col3: null pointer");
-  if (C % 8 || !aligned16(X) || !aligned16(col))
-    BH_FAIL(BH_ERR_ARG, "bh_nn_im2col3: C=%d must be a multiple of 8, buffers 16-byte aligned", C);
-  const long long n = B * (long long)H * W * 9LL * (C / 8);
-  hipLaunchKernelGGL(k_im2col3, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
-                     (const bfbits*)X, (long long)B, H, W, C, (bfbits*)col);
-  BH_LAUNCH_CHECK();
-  return BH_OK;
-}
-
-extern "C" int bh_nn_col2im3(const float* dcol, int64_t B, int H, int W, int C, const void* mask,
-                             void* dX, int out_bf16, bh_stream s) {
-  bh::clear_error();
-  if (int rc = check_img("bh_nn_col2im3", B, H, W, C)) return rc;
-  if (!dcol || !dX) BH_FAIL(BH_ERR_ARG, "bh_nn_col2im3: null pointer");
-  const long long n = B * (long long)H * W * C;
-  hipLaunchKernelGGL(k_col2im3, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s, dcol,
-                     (long long)B, H, W, C, (const bfbits*)mask, out_bf16 ? 1 : 0, dX);
-  BH_LAUNCH_CHECK();
   return BH_OK;
 }
 

@@ -6,9 +6,12 @@ from that estimate (scaled by the least-squares factor on the data) instead
 of from zero.  The network is trained on (adjoint image, p0) pairs that the
 MSGB forward and adjoint operators of this package generate, and every
 kernel of its forward and backward passes is the package's own HIP code
-(``csrc/bh_nn.hip``): convolutions as im2col + bf16 MFMA GEMM with FP32
-accumulation, their gradients as two more GEMMs (split-K for the weights) and
-col2im, pooling, upsampling, skip additions, ReLU masks, the loss and Adam.
+(``csrc/bh_nn.hip``): 3x3 convolutions as implicit bf16 MFMA GEMMs with FP32
+accumulation (operands gathered straight from the activations), their
+weight gradients as implicit GEMMs over the pixels (split into chunks added
+in order) and their input gradients as the same convolution of the output
+gradient with flipped weights; pooling, upsampling, skip additions, ReLU
+masks, the loss and Adam.
 The reference has no reconstruction solver, so there is nothing to mirror.
 
 Architecture (``UNet``): NHWC bf16 activations; 4 levels with channels
@@ -92,8 +95,8 @@ class Conv3:
 
 
 class _Work:
-    """Scratch buffers grown on demand (im2col columns, column gradients,
-    split-K partials, loss partials)."""
+    """Scratch buffers grown on demand (ReLU-masked gradients, flipped
+    weights, weight-gradient partials, loss partials)."""
 
     def __init__(self, device):
         self.device = device
@@ -203,16 +206,12 @@ class UNet:
     # ------------------------------------------------------------ primitives
     def _conv_fwd(self, conv: Conv3, x, B, H, W):
         import torch
-        P = B * H * W
-        col = self.work.get("col", P * 9 * conv.cin, torch.bfloat16)
-        _call("bh_nn_im2col3", _lib.ptr(x), B, H, W, conv.cin, _lib.ptr(col), _st())
         Wb = self.params.view(conv.wi, (conv.cout, 9 * conv.cin), bf16=True)
         bias = self.params.view(conv.bi, (conv.cout,))
         dt = torch.float32 if conv.out_f32 else torch.bfloat16
         y = torch.empty((B, H, W, conv.cout), dtype=dt, device=self.device)
-        _call("bh_nn_gemm", _lib.ptr(col), 0, 9 * conv.cin, _lib.ptr(Wb), 1, 9 * conv.cin, P,
-              conv.cout, 9 * conv.cin, 1.0, _lib.ptr(bias), int(conv.relu), _lib.ptr(y),
-              int(not conv.out_f32), conv.cout, 1, None, 0, _st())
+        _call("bh_nn_conv3", _lib.ptr(x), B, H, W, conv.cin, _lib.ptr(Wb), conv.cout,
+              _lib.ptr(bias), int(conv.relu), 1.0, _lib.ptr(y), int(not conv.out_f32), _st())
         return y
 
     def _conv_bwd(self, conv: Conv3, x, y, dy, B, H, W, need_dx=True):
@@ -227,17 +226,12 @@ class UNet:
             _call("bh_nn_relu_bwd", _lib.ptr(dy), _lib.ptr(y), n, _lib.ptr(dz), _st())
         else:
             dz = dy.reshape(-1)
-        col = self.work.get("col", P * 9 * conv.cin, torch.bfloat16)
-        _call("bh_nn_im2col3", _lib.ptr(x), B, H, W, conv.cin, _lib.ptr(col), _st())
-        # dW [cout][9 cin] = dz^T col, split over pixels
         K = 9 * conv.cin
-        tiles = ((conv.cout + 63) // 64) * ((K + 63) // 64)
-        nsplit = int(max(1, min(P // 256, 2048 // max(tiles, 1))))
-        ws_n = _lib.load().bh_nn_gemm_workspace_size(conv.cout, K, P, nsplit) // 4
-        ws = self.work.get("split", max(ws_n, 1), torch.float32)
         dW = self.params.view(conv.wi, (conv.cout, K), grad=True)
-        _call("bh_nn_gemm", _lib.ptr(dz), 1, conv.cout, _lib.ptr(col), 0, K, conv.cout, K, P, 1.0,
-              None, 0, _lib.ptr(dW), 0, K, nsplit, _lib.ptr(ws), ws.numel() * 4, _st())
+        ws_n = _lib.load().bh_nn_conv3_dw_workspace_size(B, H, W, conv.cin, conv.cout) // 4
+        ws = self.work.get("split", max(ws_n, 1), torch.float32)
+        _call("bh_nn_conv3_dw", _lib.ptr(x), _lib.ptr(dz), B, H, W, conv.cin, conv.cout,
+              _lib.ptr(dW), _lib.ptr(ws), ws.numel() * 4, _st())
         db = self.params.view(conv.bi, (conv.cout,), grad=True)
         cs_n = _lib.load().bh_nn_colsum_workspace_size(P, conv.cout) // 4
         cs = self.work.get("colsum", max(cs_n, 1), torch.float32)
@@ -245,12 +239,12 @@ class UNet:
               cs.numel() * 4, _st())
         if not need_dx:
             return None
-        dcol = self.work.get("dcol", P * K, torch.float32)
         Wb = self.params.view(conv.wi, (conv.cout, K), bf16=True)
-        _call("bh_nn_gemm", _lib.ptr(dz), 0, conv.cout, _lib.ptr(Wb), 0, K, P, K, conv.cout, 1.0,
-              None, 0, _lib.ptr(dcol), 0, K, 1, None, 0, _st())
+        Wf = self.work.get("wflip", conv.cin * 9 * conv.cout, torch.bfloat16)
+        _call("bh_nn_conv3_wflip", _lib.ptr(Wb), conv.cout, conv.cin, _lib.ptr(Wf), _st())
         dx = torch.empty((B, H, W, conv.cin), dtype=torch.bfloat16, device=self.device)
-        _call("bh_nn_col2im3", _lib.ptr(dcol), B, H, W, conv.cin, None, _lib.ptr(dx), 1, _st())
+        _call("bh_nn_conv3", _lib.ptr(dz), B, H, W, conv.cout, _lib.ptr(Wf), conv.cin, None, 0,
+              1.0, _lib.ptr(dx), 1, _st())
         return dx
 
     def _pool(self, x, B, H, W, Cc):

@@ -472,13 +472,23 @@ size_t bh_nn_gemm_workspace_size(int64_t M, int64_t N, int64_t K, int nsplit);
 int bh_nn_gemm(const void* A, int ta, int64_t lda, const void* B, int tb, int64_t ldb, int64_t M,
                int64_t N, int64_t K, float alpha, const float* bias, int relu, void* C,
                int out_bf16, int64_t ldc, int nsplit, void* ws, size_t ws_bytes, bh_stream s);
-/* im2col of a 3x3 same-padded convolution: col[p][tap*C + c], tap = 3(dy+1) + dx+1,
- * zero outside the image; C a multiple of 8. */
-int bh_nn_im2col3(const void* X, int64_t B, int H, int W, int C, void* col, bh_stream s);
-/* Its adjoint: dX = sum over taps of the FP32 column gradient dcol, zeroed
- * where mask (bf16, optional) is not positive; bf16 or FP32 output. */
-int bh_nn_col2im3(const float* dcol, int64_t B, int H, int W, int C, const void* mask, void* dX,
-                  int out_bf16, bh_stream s);
+/* 3x3 same-padded convolution as an implicit GEMM (no column buffer):
+ * Y[p][o] = act(alpha sum_{tap, c} X[n][y+dy][x+dx][c] Wt[o][tap*C + c] + bias[o]),
+ * tap = 3(dy+1) + dx+1, X zero outside the image; X (B, H, W, C) and Wt
+ * [Cout][9C] bf16, C and Cout multiples of 8, 16-byte aligned; bias FP32
+ * (optional); ReLU if relu; Y (B, H, W, Cout) bf16 (out_bf16) or FP32. */
+int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const void* Wt, int Cout,
+                const float* bias, int relu, float alpha, void* Y, int out_bf16, bh_stream s);
+/* Wf[c][tap*Cout + o] = Wt[o][(8 - tap)*C + c]: with these weights
+ * bh_nn_conv3 of dZ (B, H, W, Cout) gives the input gradient dX (B, H, W, C). */
+int bh_nn_conv3_wflip(const void* Wt, int Cout, int C, void* Wf, bh_stream s);
+/* Weight gradient dW[o][tap*C + c] = sum_p dZ[p][o] X(p shifted by tap)[c]
+ * (FP32), implicit as bh_nn_conv3; the pixels are split into chunks whose
+ * partial sums are added in order through a workspace of
+ * bh_nn_conv3_dw_workspace_size bytes (0: none needed). */
+size_t bh_nn_conv3_dw_workspace_size(int64_t B, int H, int W, int C, int Cout);
+int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, int W, int C, int Cout,
+                   float* dW, void* ws, size_t ws_bytes, bh_stream s);
 /* 2x2 average pooling of X (B, H, W, C) into Y (B, H/2, W/2, C); backward:
  * X = dY (B, H/2, W/2, C) into Y = dX (B, H, W, C). */
 int bh_nn_pool2(const void* X, int64_t B, int H, int W, int C, void* Y, int backward, bh_stream s);

@@ -64,9 +64,10 @@ def test_gemm_layouts_match_torch(gpu, ta, tb):
 
 
 def test_conv_forward_and_gradients_match_torch(gpu):
-    """One 3x3 conv layer through im2col + GEMM (forward), and its weight,
-    bias and input gradients (split-K GEMM, column sums, GEMM + col2im)
-    against torch.nn.functional.conv2d and autograd in FP32 on the CPU."""
+    """One 3x3 conv layer through the implicit-GEMM kernel (forward), and
+    its weight, bias and input gradients (implicit GEMM over the pixels,
+    column sums, the convolution with flipped weights) against
+    torch.nn.functional.conv2d and autograd in FP32 on the CPU."""
     import torch
     import torch.nn.functional as Fn
     from beamax_amd.learned import UNet
@@ -100,6 +101,61 @@ def test_conv_forward_and_gradients_match_torch(gpu):
     assert rel(dW.numpy(), dW_ref.numpy()) < 1e-5
     assert rel(db.numpy(), bt.grad.numpy()) < 1e-5
     assert rel(dx.float().cpu().numpy(), xt.grad.permute(0, 2, 3, 1).numpy()) < 5e-3
+
+
+@pytest.mark.parametrize("cin,cout,shape", [
+    (8, 24, (2, 10, 12)), (24, 48, (1, 17, 9)), (48, 96, (3, 8, 8)), (96, 192, (2, 6, 10)),
+    (192, 96, (1, 5, 7)), (16, 8, (2, 64, 40)), (64, 64, (1, 3, 3)), (24, 24, (4, 48, 64))])
+def test_implicit_conv_kernels_match_torch(gpu, cin, cout, shape):
+    """The three convolution entry points on their own, over every output
+    tile width (8 to 192 channels, several k and channel tiles), images
+    smaller than a tile, widths that are not multiples of 8 and pixel counts
+    that split the weight gradient into many chunks: forward (bias, ReLU,
+    alpha, bf16 and FP32 output), input gradient through the flipped weights,
+    weight gradient, all against FP32 torch on the CPU."""
+    import torch
+    import torch.nn.functional as Fn
+    from beamax_amd import _lib
+    B, H, W = shape
+    g = torch.Generator().manual_seed(cin * 1000 + cout)
+    x = _bf(torch.randn(B, H, W, cin, generator=g))
+    w = _bf(torch.randn(cout, 9, cin, generator=g) / (3.0 * cin ** 0.5))
+    bias = torch.randn(cout, generator=g)
+    dz = _bf(torch.randn(B, H, W, cout, generator=g))
+    wt = w.reshape(cout, 3, 3, cin).permute(0, 3, 1, 2).double().requires_grad_(True)
+    xt = x.permute(0, 3, 1, 2).double().requires_grad_(True)
+    z = Fn.conv2d(xt, wt, None, padding=1)
+    (z * dz.permute(0, 3, 1, 2).double()).sum().backward()
+    zref = z.detach().permute(0, 2, 3, 1)
+    st = _lib.stream_handle()
+    xd = x.to(torch.bfloat16).to(gpu)
+    wd = w.reshape(cout, 9 * cin).to(torch.bfloat16).to(gpu)
+    bd = bias.to(gpu)
+    y = torch.empty(B, H, W, cout, dtype=torch.float32, device=gpu)
+    _lib.call("bh_nn_conv3", _lib.ptr(xd), B, H, W, cin, _lib.ptr(wd), cout, None, 0, 1.0,
+              _lib.ptr(y), 0, st)
+    assert rel(y.cpu().numpy(), zref.numpy()) < 1e-5
+    yb = torch.empty(B, H, W, cout, dtype=torch.bfloat16, device=gpu)
+    _lib.call("bh_nn_conv3", _lib.ptr(xd), B, H, W, cin, _lib.ptr(wd), cout, _lib.ptr(bd), 1, 0.5,
+              _lib.ptr(yb), 1, st)
+    r2 = torch.relu(0.5 * zref + bias.double())
+    assert float((yb.float().cpu().double() - r2).abs().max()) <= \
+        2 ** -7 * float(r2.abs().max()) + 1e-6
+    dzd = dz.to(torch.bfloat16).to(gpu)
+    wf = torch.empty(cin * 9 * cout, dtype=torch.bfloat16, device=gpu)
+    _lib.call("bh_nn_conv3_wflip", _lib.ptr(wd), cout, cin, _lib.ptr(wf), st)
+    dx = torch.empty(B, H, W, cin, dtype=torch.float32, device=gpu)
+    _lib.call("bh_nn_conv3", _lib.ptr(dzd), B, H, W, cout, _lib.ptr(wf), cin, None, 0, 1.0,
+              _lib.ptr(dx), 0, st)
+    assert rel(dx.cpu().numpy(), xt.grad.permute(0, 2, 3, 1).numpy()) < 1e-5
+    lib = _lib.load()
+    nb = lib.bh_nn_conv3_dw_workspace_size(B, H, W, cin, cout)
+    ws = torch.empty(max(nb // 4, 1), dtype=torch.float32, device=gpu)
+    dW = torch.full((cout, 9 * cin), float("nan"), dtype=torch.float32, device=gpu)
+    _lib.call("bh_nn_conv3_dw", _lib.ptr(xd), _lib.ptr(dzd), B, H, W, cin, cout, _lib.ptr(dW),
+              _lib.ptr(ws), nb, st)
+    dW_ref = wt.grad.permute(0, 2, 3, 1).reshape(cout, 9 * cin)
+    assert rel(dW.cpu().numpy(), dW_ref.numpy()) < 1e-5
 
 
 class _Twin:
