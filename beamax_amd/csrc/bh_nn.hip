@@ -463,13 +463,15 @@ __global__ __launch_bounds__(256) void k_conv3_dw(ConvDwArgs a) {
   const int K = 9 * a.C;
   const long long pbeg = blockIdx.z * a.chunk;
   const long long pend = min(a.P, pbeg + a.chunk);
-  // staging roles: threads [0, 8 MO) stage dZ blocks (pixel block t % 4,
-  // channel group t / 4), threads [128, 192) stage X blocks (pixel block,
-  // k group); the other threads only multiply
-  const bool zst = t < 8 * MO;
-  const bool xst = t >= 128 && t < 192;
-  const int sl = zst ? t : t - 128;
-  const int sb = sl & 3, sg = sl >> 2;
+  // staging roles, by wave: waves 0 and 1 stage the dZ blocks (lane < 8 MO:
+  // pixel block lane % 4, channel group lane / 4) of even and odd steps,
+  // waves 2 and 3 the X blocks (pixel block, k group) of even and odd steps;
+  // a wave's loads are issued two steps before the step that multiplies
+  // them, with one register set per thread
+  const int par = w & 1;
+  const bool zst = w < 2 && lane < 8 * MO;
+  const bool xst = w >= 2;
+  const int sb = lane & 3, sg = lane >> 2;
   bool gin = false;
   int ci = 0, dy = 0, dx = 0;
   if (zst) gin = o0 + 8 * sg < a.Cout;
@@ -526,16 +528,15 @@ __global__ __launch_bounds__(256) void k_conv3_dw(ConvDwArgs a) {
   for (int i = 0; i < MO; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int nsteps = (int)((pend - pbeg + DWP - 1) / DWP);
-  if (nsteps > 0) {
-    gload(pbeg);
-    sstore(0);
-  }
+  if (par < nsteps) gload(pbeg + (long long)par * DWP);
+  if (par == 0 && nsteps > 0) sstore(0);
   __syncthreads();
   const int fr = lane & 15, fk = (lane >> 4) * 8;
   for (int st = 0; st < nsteps; ++st) {
     const int buf = st & 1;
-    const bool more = st + 1 < nsteps;
-    if (more) gload(pbeg + (long long)(st + 1) * DWP);
+    // this step's parity has stored step st: its registers take step st + 2;
+    // the other parity holds step st + 1 and stores it after the multiplies
+    if (par == buf && st + 2 < nsteps) gload(pbeg + (long long)(st + 2) * DWP);
     bf16x8 za[MO], xb[2];
     for (int i = 0; i < MO; ++i)
       za[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Zs[buf][16 * i + fr][fk]));
@@ -545,7 +546,7 @@ __global__ __launch_bounds__(256) void k_conv3_dw(ConvDwArgs a) {
     for (int i = 0; i < MO; ++i)
       for (int j = 0; j < 2; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(za[i], xb[j], acc[i][j], 0, 0, 0);
-    if (more) sstore(buf ^ 1);
+    if (par != buf && st + 1 < nsteps) sstore(buf ^ 1);
     __syncthreads();
   }
   float* out = a.out + blockIdx.z * a.split_stride;
@@ -558,6 +559,30 @@ __global__ __launch_bounds__(256) void k_conv3_dw(ConvDwArgs a) {
         const int o = o0 + 16 * i + cr + q;
         if (o < a.Cout) out[(long long)o * K + k] = acc[i][j][q];
       }
+  }
+}
+
+// out[i] = sum_z part[z][i] in a fixed order for many splits: a block owns
+// 16 outputs; its 16 phases each add a contiguous run of splits in order,
+// then the 16 phase sums are added in order
+__global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ part, int ns,
+                                                    long long n, float* __restrict__ out) {
+  __shared__ float red[16][17];
+  const int j = threadIdx.x & 15, q = threadIdx.x >> 4;
+  const long long i = blockIdx.x * 16LL + j;
+  const int per = (ns + 15) / 16;
+  const int z0 = min(ns, q * per), z1 = min(ns, z0 + per);
+  float s = 0.f;
+  if (i < n) {
+#pragma unroll 8
+    for (int z = z0; z < z1; ++z) s += part[z * n + i];
+  }
+  red[q][j] = s;
+  __syncthreads();
+  if (q == 0 && i < n) {
+    float t = 0.f;
+    for (int r = 0; r < 16; ++r) t += red[r][j];
+    out[i] = t;
   }
 }
 
@@ -620,13 +645,14 @@ extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const 
   return BH_OK;
 }
 
-// pixel split of the weight gradient: about 2048 workgroups, at most 256
-// chunks, each a multiple of 32 pixels
+// pixel split of the weight gradient: about 2048 workgroups (the kernel is
+// bound by the latency of its staging loads, so it needs several resident
+// workgroups per CU), chunks of at least 256 pixels, a multiple of 32
 static void conv_dw_split(long long P, int C, int Cout, long long* chunk, int* nsplit) {
   const long long tiles = (long long)((9 * C + DWK - 1) / DWK) *
                           ((Cout + 16 * conv_tile(Cout) - 1) / (16 * conv_tile(Cout)));
   long long ns = (2048 + tiles - 1) / tiles;
-  ns = std::max(1LL, std::min(ns, std::min(256LL, (P + 255) / 256)));
+  ns = std::max(1LL, std::min(ns, (P + 255) / 256));
   long long ch = (P + ns - 1) / ns;
   ch = (ch + DWP - 1) / DWP * DWP;
   *chunk = ch;
@@ -684,7 +710,7 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
   else hipLaunchKernelGGL(k_conv3_dw<6>, grid, dim3(256), 0, st, a);
   BH_LAUNCH_CHECK();
   if (ns > 1) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(bh::grid1d(n, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, st,
                        (const float*)ws, ns, n, dW);
     BH_LAUNCH_CHECK();
   }

@@ -543,6 +543,39 @@ def evaluate(net: Optional[UNet], problem, seeds, device, tol=1e-3, log=None, gr
     return rows
 
 
+def conv_layer_timing(H: int, cin: int, cout: int, batch: int = 16, reps: int = 3,
+                      device=None):
+    """Forward and weight gradient of one 3x3 layer (cin -> cout channels,
+    batch x H x H pixels, random bf16 data), each run ``reps`` times on the
+    current stream; returns microseconds per call of each (HIP events).  A
+    profiling target for the two convolution kernels on their own."""
+    import torch
+    dev = torch.device("cuda") if device is None else torch.device(device)
+    g = torch.Generator(device=dev).manual_seed(0)
+    P = batch * H * H
+    x = torch.randn(P * cin, device=dev, generator=g).to(torch.bfloat16)
+    w = torch.randn(cout * 9 * cin, device=dev, generator=g).to(torch.bfloat16)
+    dz = torch.randn(P * cout, device=dev, generator=g).to(torch.bfloat16)
+    y = torch.empty(P * cout, device=dev, dtype=torch.bfloat16)
+    dW = torch.empty(cout * 9 * cin, device=dev, dtype=torch.float32)
+    nb = _lib.load().bh_nn_conv3_dw_workspace_size(batch, H, H, cin, cout)
+    ws = torch.empty(max(nb // 4, 1), device=dev, dtype=torch.float32)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd = dw = 0.0
+    for _ in range(reps):
+        ev[0].record()
+        _call("bh_nn_conv3", _lib.ptr(x), batch, H, H, cin, _lib.ptr(w), cout, None, 1, 1.0,
+              _lib.ptr(y), 1, _st())
+        ev[1].record()
+        _call("bh_nn_conv3_dw", _lib.ptr(x), _lib.ptr(dz), batch, H, H, cin, cout, _lib.ptr(dW),
+              _lib.ptr(ws), nb, _st())
+        ev[2].record()
+        torch.cuda.synchronize(dev)
+        fwd += ev[0].elapsed_time(ev[1]) * 1e3 / reps
+        dw += ev[1].elapsed_time(ev[2]) * 1e3 / reps
+    return {"forward_us": fwd, "weight_grad_us": dw}
+
+
 def _main(argv=None):
     """Train the network on generated pairs, save it, and compare CG with and
     without it on held-out seeds (one JSON line on stdout)."""
@@ -557,8 +590,15 @@ def _main(argv=None):
     ap.add_argument("--eval", type=int, default=32, help="held-out seeds 0 .. n-1")
     ap.add_argument("--out", default=None, help="weights file (safetensors)")
     ap.add_argument("--weights", default=None, help="evaluate these weights, no training")
+    ap.add_argument("--conv", type=int, nargs=3, metavar=("H", "CIN", "COUT"), default=None,
+                    help="only time the forward and weight gradient of one 3x3 layer")
     a = ap.parse_args(argv)
     dev = torch.device("cuda")
+    if a.conv:
+        H, cin, cout = a.conv
+        print(json.dumps({"layer": [H, cin, cout], "batch": a.batch,
+                          **conv_layer_timing(H, cin, cout, a.batch, device=dev)}), flush=True)
+        return
     log = lambda m: print(m, flush=True)
     problem = recon_problem(dev)
     result = {}
