@@ -51,6 +51,9 @@ struct CulledArgs {
   // active on time rows [row_on[bi], row_off[bi]]
   const int32_t* perm;
   const int2* rows_act;
+  // with perm: per chunk of kRecChunk positions the hull [first, last] of
+  // its entries' windows; a block skips a chunk outside it (null: no skip)
+  const int2* chunk_rows;
   long long rows;      // output rows (nt for the forward, 1 for the static form)
   int ostride;         // output element o goes to out[o * ostride + ooff]
   int ooff;
@@ -150,8 +153,20 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
   // to LDS and, after a second barrier, the waves evaluate.  Two chunks
   // apart the mask buffer is rewritten only after every wave has passed the
   // second barrier of the chunk in between, so no wave still reads it.
+  // The mask buffers alternate over the chunks the block processes (skipped
+  // chunks pass no barrier, so the parity counts processed chunks only).
+  int npro = 0;
   for (long long cb = c_lo * kRecChunk; cb < c_hi * kRecChunk && cb < total; cb += kRecChunk) {
-    const int par = (int)((cb / kRecChunk) & 1);
+    if constexpr (!STATIC) {
+      if (A.chunk_rows) {
+        // no entry of the chunk can reach a run on this row: the whole
+        // block skips it (the test is block-uniform)
+        const int2 h = A.chunk_rows[cb / kRecChunk];
+        if (t < h.x || t > h.y) continue;
+      }
+    }
+    const int par = npro & 1;
+    ++npro;
     const long long bi = cb + threadIdx.x;
     Rec R;
     bool live = bi < total;
@@ -679,18 +694,47 @@ __global__ void k_window_order(const long long* __restrict__ sidx, const int2* _
   rows_act[i] = win[e];
 }
 
+// per chunk of kRecChunk positions, the hull of its entries' windows
+static_assert(kRecChunk == 256, "one block of four waves per chunk");
+__global__ __launch_bounds__(256) void k_chunk_rows(const int2* __restrict__ rows_act,
+                                                    long long total, int2* __restrict__ hull) {
+  __shared__ int smin[4], smax[4];
+  const long long i = blockIdx.x * (long long)kRecChunk + threadIdx.x;
+  int lo = 0x7fffffff, hi = -1;
+  if (i < total) {
+    const int2 w = rows_act[i];
+    lo = w.x;
+    hi = w.y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    smin[threadIdx.x >> 6] = lo;
+    smax[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    hull[blockIdx.x] = make_int2(min(min(smin[0], smin[1]), min(smin[2], smin[3])),
+                                 max(max(smax[0], smax[1]), max(smax[2], smax[3])));
+}
+
 size_t round256(size_t x) { return (x + 255) / 256 * 256; }
 
 // bytes of the window tables of `total` entries
 size_t window_bytes(long long total) {
   const long long P = bh::sort_pairs_capacity(total);
+  const long long nchunk = (total + kRecChunk - 1) / kRecChunk;
   return round256(16) + 4 * round256((size_t)P * 8) + round256((size_t)total * 8) +
-         round256((size_t)total * 4) + round256((size_t)total * 8);
+         round256((size_t)total * 4) + round256((size_t)total * 8) +
+         round256((size_t)nchunk * 8);
 }
 
 // the windows and the entry order of a forward launch, carved from ws
 int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt, void* ws,
-                  const int32_t** perm, const int2** rows_act, hipStream_t st) {
+                  const int32_t** perm, const int2** rows_act, const int2** chunk_rows,
+                  hipStream_t st) {
   char* p = (char*)ws;
   auto take = [&](size_t bytes) { char* r = p; p += round256(bytes); return (void*)r; };
   const long long total = a.total;
@@ -703,6 +747,8 @@ int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt,
   int2* win = (int2*)take((size_t)total * 8);
   int32_t* pm = (int32_t*)take((size_t)total * 4);
   int2* ra = (int2*)take((size_t)total * 8);
+  const long long nchunk = (total + kRecChunk - 1) / kRecChunk;
+  int2* hull = (int2*)take((size_t)nchunk * 8);
   hipLaunchKernelGGL(k_time_range, dim3(1), dim3(256), 0, st, ts, nt, trange);
   WinArgs W{};
   W.b = a.b;
@@ -729,8 +775,12 @@ int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt,
   hipLaunchKernelGGL(k_window_order, dim3(bh::grid1d(total, 256)), dim3(256), 0, st, io, win,
                      total, pm, ra);
   BH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_chunk_rows, dim3((unsigned)nchunk), dim3(kRecChunk), 0, st, ra, total,
+                     hull);
+  BH_LAUNCH_CHECK();
   *perm = pm;
   *rows_act = ra;
+  *chunk_rows = hull;
   return BH_OK;
 }
 
@@ -858,10 +908,12 @@ int forward_culled(const char* name, const bh_fwd_beams* b, int ndim, long long 
   size_t part_bytes = ws_bytes;
   a.perm = nullptr;
   a.rows_act = nullptr;
+  a.chunk_rows = nullptr;
   const size_t part_std = round256(bh_forward_sum_culled_workspace_size(nt, seg->S));
   if (!traj && cull_log2 > 0.0 && ws && ws_bytes >= part_std + window_bytes(a.total) &&
       a.total < (1ll << 29) && nt < 65535) {
-    if (int rc = build_windows(ndim, a, ts, nt, (char*)ws + part_std, &a.perm, &a.rows_act, st))
+    if (int rc = build_windows(ndim, a, ts, nt, (char*)ws + part_std, &a.perm, &a.rows_act,
+                               &a.chunk_rows, st))
       return rc;
     part_bytes = part_std;
   }
