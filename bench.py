@@ -43,9 +43,9 @@ TRANS_BOUND_EVALS = 256 * 64 * 2.4e9 / 4 / 2
 # workload (8 beam ranges): rocprofv3 FETCH_SIZE (KiB raw, x2.0) + WRITE_SIZE
 # (KiB raw, x1.0) of the steady-state dispatch, both scaled by what the
 # counters read for a 512 MiB copy (profiles/k_culled_sum_msgb2d_512.md), at
-# cull_log2 = 32 (46263.3 and 308388.5 KiB).  Counters cannot be read from
+# cull_log2 = 32 (52461.3 and 308558.8 KiB).  Counters cannot be read from
 # inside the process, so other settings report null.
-MEASURED_FWD_TRAFFIC = {32.0: 46263.3125 * 2.0 * 1024 + 308388.5 * 1.0 * 1024}
+MEASURED_FWD_TRAFFIC = {32.0: 52461.25 * 2.0 * 1024 + 308558.78125 * 1.0 * 1024}
 
 
 def build_problem(device, cull_log2=32.0, seed=0):
