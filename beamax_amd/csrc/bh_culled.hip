@@ -675,6 +675,7 @@ __global__ __launch_bounds__(128) void k_beam_windows(WinArgs W) {
     }
   }
   W.win[e] = make_int2(on, off);
+  if (!W.key) return;   // windows only, no sort keys
   // order: eight ranges of the table (so that every beam range of the launch
   // draws from all of them alike), then window start and end, then entry
   const unsigned long long rg = (unsigned long long)((e * 8) / W.total);
@@ -783,6 +784,49 @@ int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt,
   *chunk_rows = hull;
   return BH_OK;
 }
+
+}  // namespace
+
+namespace bh {
+
+size_t beam_windows_bytes(long long total) {
+  return round256(16) + round256((size_t)total * 8);
+}
+
+int beam_windows(int ndim, const bh_fwd_beams* b, int mirror, double c, const double* ts,
+                 long long nt, const bh_segments* seg, double E, void* ws, const int2** win,
+                 hipStream_t st) {
+  char* p = (char*)ws;
+  double* trange = (double*)p;
+  int2* w = (int2*)(p + round256(16));
+  WinArgs W{};
+  W.b = *b;
+  W.count = b->count;
+  W.total = b->count * (mirror ? 2 : 1);
+  W.npos = b->count;
+  W.mirror = mirror ? 1 : 0;
+  W.c = c;
+  W.ts = ts;
+  W.nt = nt;
+  W.G = to_segdev(seg);
+  W.E = E;
+  W.trange = trange;
+  W.key = nullptr;
+  W.idx = nullptr;
+  W.win = w;
+  hipLaunchKernelGGL(k_time_range, dim3(1), dim3(256), 0, st, ts, nt, trange);
+  const unsigned g = grid1d(W.total, 128);
+  if (ndim == 1) hipLaunchKernelGGL(k_beam_windows<1>, dim3(g), dim3(128), 0, st, W);
+  else if (ndim == 2) hipLaunchKernelGGL(k_beam_windows<2>, dim3(g), dim3(128), 0, st, W);
+  else hipLaunchKernelGGL(k_beam_windows<3>, dim3(g), dim3(128), 0, st, W);
+  BH_LAUNCH_CHECK();
+  *win = w;
+  return BH_OK;
+}
+
+}  // namespace bh
+
+namespace {
 
 // Launch shape: groups per block GPB in {32, 16, 8, 4} (fewer groups per
 // block = more blocks but more record work per group), then a beam split so

@@ -65,6 +65,17 @@ long long sort_pairs_capacity(long long n);
 int sort_pairs(unsigned long long* key, long long* idx, unsigned long long* key2, long long* idx2,
                long long n, unsigned long long** key_out, long long** idx_out, hipStream_t s);
 
+// Activity windows of a closed-form forward beam table (entries [0, count)
+// with mode +1 and, if mirror, [count, 2 count) with mode -1): per entry the
+// first and last time row on which its envelope can reach a point run at
+// cut E (rows outside hold no term above 2^-E of the beam).  Only for
+// imaginary alpha and increasing ts; other entries get all rows.  ws:
+// beam_windows_bytes(total) bytes; *win points into it.
+size_t beam_windows_bytes(long long total);
+int beam_windows(int ndim, const bh_fwd_beams* b, int mirror, double c, const double* ts,
+                 long long nt, const bh_segments* seg, double E, void* ws, const int2** win,
+                 hipStream_t st);
+
 // Status words of the kernels that guard beam / coefficient indices (one per
 // translation unit): OR the bits into *bits, clear them if asked.  Both
 // synchronise the stream.

@@ -153,7 +153,11 @@ __global__ __launch_bounds__(256) void k_fwd_transpose(TArgs A) {
 // Per lane the pass partials are FP32, the beam's totals FP64; the wave
 // reduces them once per (beam, tile) into its slot.
 // ---------------------------------------------------------------------------
-constexpr int kTileU = 32;   // 128-point units per tile (bits of a row's mask)
+// 128-point units per tile (bits of a row's mask): 8, 16 or 32, the fewest
+// that hold the points' units (fewer units, less LDS for the pair list, more
+// resident blocks)
+constexpr int kTileU = 32;
+inline int tile_units(long long nunits) { return nunits <= 8 ? 8 : nunits <= 16 ? 16 : kTileU; }
 
 struct TQArgs {
   bh_fwd_beams b;
@@ -168,6 +172,9 @@ struct TQArgs {
   double* slots;         // [ntt * ntu][count] complex
   long long ntu;         // unit tiles
   long long nunits;      // ngroups / 2
+  // activity windows per entry (mode +1: beam k, mode -1: count + k), or
+  // null: a wave skips a (beam, mode) whose window misses its 64 rows
+  const int2* win;
 };
 
 // r at (t, slot) for every 64-point group slot, zero where the slot is padding
@@ -184,19 +191,23 @@ __global__ void k_gather_resid(const void* __restrict__ resid, int f64, long lon
   rg[i] = v;
 }
 
-template <int D>
-__global__ __launch_bounds__(256) void k_fwd_transpose_q(TQArgs A) {
+// Up to 16 units per tile the 1-D and 2-D forms run at four waves per SIMD
+// (the LDS allows four blocks per CU; a few registers spill outside the
+// evaluation loop); otherwise the compiler's choice.
+template <int D, int TU>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((D <= 2 && TU <= 16) ? 4 : 1)))
+void k_fwd_transpose_q(TQArgs A) {
   __shared__ RecS<D> rec[4][64];
-  __shared__ unsigned short plist[4][64 * kTileU];   // row << 5 | unit
+  __shared__ unsigned short plist[4][64 * TU];   // row << 5 | unit
   __shared__ PairQ coef[4][64];
-  __shared__ double upt[kTileU][D], ustep[kTileU][D];
-  __shared__ int useg[kTileU], uj0[kTileU];
+  __shared__ double upt[TU][D], ustep[TU][D];
+  __shared__ int useg[TU], uj0[TU];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const long long t0 = (long long)blockIdx.y * 64;
-  const long long ub = (long long)blockIdx.z * kTileU;
+  const long long ub = (long long)blockIdx.z * TU;
   const long long nul = A.nunits - ub;
-  const int nu = (int)(nul < kTileU ? nul : kTileU);
+  const int nu = (int)(nul < TU ? nul : TU);
   if (threadIdx.x < nu) {
     const long long g = 2 * (ub + threadIdx.x);
     const int k = A.G.grp_seg[g];
@@ -229,6 +240,11 @@ __global__ __launch_bounds__(256) void k_fwd_transpose_q(TQArgs A) {
     const double w = A.b.omega[k];
     double acc_re = 0.0, acc_im = 0.0;
     for (int md = 0; md < A.modes; ++md) {
+      if (A.win) {
+        // no term of this (beam, mode) on rows [t0, t0 + 63] reaches the cut
+        const int2 w = A.win[md ? A.count + k : k];
+        if (w.y < t0 || w.x > t0 + 63) continue;
+      }
       RecS<D> R;
       bool live = tvalid;
       if (live) {
@@ -389,7 +405,8 @@ inline TShape transpose_shape(long long count, long long nt, long long ngroups) 
 inline TShape transpose_q_shape(long long count, long long nt, long long ngroups) {
   TShape s;
   s.ntt = (nt + 63) / 64;
-  s.ntg = (ngroups / 2 + kTileU - 1) / kTileU;
+  const int tu = tile_units(ngroups / 2);
+  s.ntg = (ngroups / 2 + tu - 1) / tu;
   const long long tiles = s.ntt * s.ntg;
   long long gx = (4096 + tiles - 1) / tiles;
   const long long maxgx = (count + 3) / 4;
@@ -403,17 +420,26 @@ size_t round256(size_t x) { return (x + 255) / 256 * 256; }
 
 }  // namespace
 
-// The FP64 form's slots, or the FP32-split form's slots followed by the
-// residual in group order; the larger of the two.
-extern "C" size_t bh_forward_transpose_workspace_size(int64_t count, int64_t nt,
-                                                      int64_t ngroups) {
-  if (count <= 0 || nt <= 0 || ngroups <= 0) return 0;
+namespace {
+// the FP64 form's slots, or the FP32-split form's slots followed by the
+// residual in group order; the larger of the two
+size_t transpose_base_bytes(int64_t count, int64_t nt, int64_t ngroups) {
   const TShape sh = transpose_shape(count, nt, ngroups);
   const size_t f64 = (size_t)sh.ntt * sh.ntg * count * 2 * sizeof(double);
   const TShape sq = transpose_q_shape(count, nt, ngroups);
   const size_t f32 = round256((size_t)sq.ntt * sq.ntg * count * 2 * sizeof(double)) +
                      (size_t)nt * ngroups * 64 * sizeof(float);
-  return f64 > f32 ? f64 : f32;
+  return round256(f64 > f32 ? f64 : f32);
+}
+}  // namespace
+
+// The slots (and the FP32-split form's residual in group order), then room
+// for the activity windows of both modes; with less than that (but at least
+// the slots) the FP32-split form runs without windows.
+extern "C" size_t bh_forward_transpose_workspace_size(int64_t count, int64_t nt,
+                                                      int64_t ngroups) {
+  if (count <= 0 || nt <= 0 || ngroups <= 0) return 0;
+  return transpose_base_bytes(count, nt, ngroups) + bh::beam_windows_bytes(2 * count);
 }
 
 extern "C" int bh_forward_transpose(const bh_fwd_beams* b, int ndim, int mirror, double c,
@@ -435,7 +461,7 @@ extern "C" int bh_forward_transpose(const bh_fwd_beams* b, int ndim, int mirror,
   const TShape sh = transpose_shape(b->count, nt, seg->ngroups);
   if (sh.ntg > 65535) BH_FAIL(BH_ERR_ARG, "bh_forward_transpose: %lld point groups",
                               (long long)seg->ngroups);
-  const size_t need = bh_forward_transpose_workspace_size(b->count, nt, seg->ngroups);
+  const size_t need = transpose_base_bytes(b->count, nt, seg->ngroups);
   if (!ws || ws_bytes < need)
     BH_FAIL(BH_ERR_WORKSPACE, "bh_forward_transpose: workspace %zu < %zu", ws_bytes, need);
   if (precision == BH_PRECISION_FP32_SPLIT) {
@@ -458,13 +484,32 @@ extern "C" int bh_forward_transpose(const bh_fwd_beams* b, int ndim, int mirror,
     q.rg = rg;
     q.ntu = sq.ntg;
     q.nunits = seg->ngroups / 2;
+    q.win = nullptr;
+    const long long entries = b->count * (mirror ? 2 : 1);
+    if (cull_log2 > 0.0 && nt < 65535 && ws_bytes >= need + bh::beam_windows_bytes(entries)) {
+      if (int rc = bh::beam_windows(ndim, b, mirror, c, ts, nt, seg, cull_log2, (char*)ws + need,
+                                    &q.win, st))
+        return rc;
+    }
     const long long nslot = seg->ngroups * 64;
     hipLaunchKernelGGL(k_gather_resid, dim3(bh::grid1d(nt * nslot, 256)), dim3(256), 0, st, resid,
                        resid_f64, (long long)nt, (long long)seg->S, seg->col, nslot, rg);
     dim3 grid((unsigned)sq.gx, (unsigned)sq.ntt, (unsigned)sq.ntg);
-    if (ndim == 1) hipLaunchKernelGGL(k_fwd_transpose_q<1>, grid, dim3(256), 0, st, q);
-    else if (ndim == 2) hipLaunchKernelGGL(k_fwd_transpose_q<2>, grid, dim3(256), 0, st, q);
-    else hipLaunchKernelGGL(k_fwd_transpose_q<3>, grid, dim3(256), 0, st, q);
+    const int tu = tile_units(q.nunits);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, q); };
+    if (ndim == 1) {
+      if (tu == 8) go(k_fwd_transpose_q<1, 8>);
+      else if (tu == 16) go(k_fwd_transpose_q<1, 16>);
+      else go(k_fwd_transpose_q<1, kTileU>);
+    } else if (ndim == 2) {
+      if (tu == 8) go(k_fwd_transpose_q<2, 8>);
+      else if (tu == 16) go(k_fwd_transpose_q<2, 16>);
+      else go(k_fwd_transpose_q<2, kTileU>);
+    } else {
+      if (tu == 8) go(k_fwd_transpose_q<3, 8>);
+      else if (tu == 16) go(k_fwd_transpose_q<3, 16>);
+      else go(k_fwd_transpose_q<3, kTileU>);
+    }
     BH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_slot_reduce, dim3(bh::grid1d(b->count, 256)), dim3(256), 0, st,
                        (const double*)ws, sq.ntt * sq.ntg, (long long)b->count, grad);

@@ -76,6 +76,46 @@ def test_forward_transpose_matches_oracle(gpu, d, kind):
         assert rel(ref, got) < tol, (prec, E)
 
 
+def test_transpose_windows_change_nothing(gpu):
+    """The FP32-split transpose skips a wave's (beam, mode, 64-row tile) when
+    the beam's activity window misses the tile; with a workspace too small
+    for the windows it evaluates every tile.  Both give the same gradient
+    bit for bit (the skipped tiles hold no pair), on a 64^2 four-edge
+    problem whose beams leave the domain well before the last time row."""
+    import torch
+    import beamax_amd as bx
+    from beamax_amd import _lib
+    from beamax_amd.gb import gb_solvers
+    from beamax_amd.solvers.faces import four_edge_mask
+    from beamax_amd.solvers.linear import FrozenForward
+    N = (64, 64)
+    dom = bx.Domain(N=N, dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    wpt = bx.MSWPT(bx.DyadicDecomposition(2, N, (4, 8), (1, 1)), 2, "rectangular_mirror",
+                   device=gpu)
+    sensors = bx.Sensor(dom, binary_mask=four_edge_mask(N))
+    ts = dom.generate_time_domain()
+    ts = np.concatenate([ts, ts[-1] + (ts[1] - ts[0]) * np.arange(1, 301)])   # 300 more rows
+    rng = np.random.default_rng(72)
+    s = bx.MSGBSolver(800, "top_n", 128, "spatial", gb_solvers.solve_hom_diag, "scan_real",
+                      precision="fp32_split", cull_log2=32.0)
+    F = FrozenForward.from_field(s, _t(rng.standard_normal(N), gpu), dom, sensors, ts, wpt)
+    nt, S = F.shape[0]
+    r = _t(rng.standard_normal((nt, S)), gpu).to(torch.float32).contiguous()
+    lib = _lib.load()
+    full = lib.bh_forward_transpose_workspace_size(F.k, nt, F.seg.ngroups)
+    wbytes = 256 + (2 * F.k * 8 + 255) // 256 * 256          # the windows' share
+    outs = []
+    for nbytes in (full, full - wbytes):
+        g = torch.full((F.k,), complex(np.nan, np.nan), dtype=torch.complex128, device=gpu)
+        ss = F.seg.struct()
+        _lib.call("bh_forward_transpose", C.byref(F.unit.struct()), 2, 1, 1500.0, _lib.ptr(F.ts),
+                  nt, C.byref(ss), 32.0, _lib.PRECISION["fp32_split"], _lib.ptr(r), 0,
+                  _lib.ptr(g), _lib.ptr(F._tws), nbytes, _lib.stream_handle())
+        outs.append(_n(g))
+    assert np.all(np.isfinite(outs[0]))
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_frozen_operator_is_transpose_pair(gpu):
     import torch
     import beamax_amd as bx
