@@ -432,8 +432,11 @@ int bh_forward_sum_traj(const bh_fwd_beams* b, int ndim, int mirror, const doubl
  * <forward(c), resid> = Re sum_k conj(grad[k]) c[k].  Terms outside the
  * envelope bound of bh_forward_sum_culled (cull_log2) are skipped the same
  * way; cull_log2 <= 0 evaluates every term.  The workspace holds one partial
- * per (beam, tile of 64 time rows x 32 point groups); the partials are added
- * in a fixed order (deterministic). */
+ * per (beam, tile of 64 time rows x a tile of point groups), added in a fixed
+ * order (deterministic), the FP32-split form's residual in group order, and
+ * then the beams' activity windows (the FP32-split form skips a beam's time
+ * tiles outside its window); a workspace with room for the partials but not
+ * the windows gives the same result without skipping. */
 size_t bh_forward_transpose_workspace_size(int64_t count, int64_t nt, int64_t ngroups);
 int bh_forward_transpose(const bh_fwd_beams* b, int ndim, int mirror, double c,
                          const double* ts, int64_t nt, const bh_segments* seg, double cull_log2,
