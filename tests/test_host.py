@@ -85,7 +85,7 @@ def test_library_exports_every_declared_symbol():
     from beamax_amd import _lib
     lib = C.CDLL(str(_lib.LIB_PATH))
     text = "\n".join(p.read_text() for p in (ROOT / "include").glob("*.h"))
-    names = set(re.findall(r"\b(bh_[a-z0-9_]+)\s*\(", text))
+    names = set(re.findall(r"\b(bh_[A-Za-z0-9_]+)\s*\(", text))
     assert names, "no declarations found"
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
