@@ -416,6 +416,115 @@ __global__ __launch_bounds__(256) void k_conv3_igemm(ConvArgs a) {
   }
 }
 
+// Halo-tile form of the same convolution for inputs of up to 96 channels:
+// a workgroup owns a tile of TH x 32 output pixels of one image and loads
+// the (TH + 2) x 34 input pixels around it into LDS once (coalesced 16-byte
+// rows, zero outside the image); every k step then reads its A fragments
+// from that halo at the step's tap offset (k = tap C + c, eight channels of
+// one tap per lane group), so each input element is fetched from memory
+// once instead of once per tap.  The weight tile of the k step is staged
+// and double-buffered as in k_conv3_igemm.  The halo's pixel stride is C or
+// C + 8 channels, whichever is an odd number of 16-byte words (16-byte LDS
+// reads of 16 consecutive pixels then hit distinct banks).
+constexpr int HTW = 32;
+
+template <int C8, int TH, int NB>
+__global__ __launch_bounds__(256) void k_conv3_halo(ConvArgs a, int tiles_x, int tiles_y) {
+  constexpr int C = 8 * C8;
+  constexpr int CS = (C8 & 1) ? C : C + 8;
+  constexpr int HH = TH + 2, HW = HTW + 2;
+  constexpr int MB = TH / 2;                 // 16-pixel blocks per wave (TH x 32 / 4 / 16)
+  constexpr int BNc = 16 * NB;
+  constexpr int NG = 9 * C8;                 // k groups of 8
+  constexpr int NSTEP = (NG + 3) / 4;
+  constexpr int BLD = (BNc * 4 + 255) / 256;
+  __shared__ __attribute__((aligned(16))) bfbits halo[HH * HW * CS];
+  __shared__ __attribute__((aligned(16))) bfbits Bs[2][BNc][CLD];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int tb = blockIdx.x;
+  const int tx = tb % tiles_x;
+  tb /= tiles_x;
+  const int ty = tb % tiles_y;
+  const long long img = tb / tiles_y;
+  const int x0 = tx * HTW, y0 = ty * TH;
+  const int n0 = blockIdx.y * BNc;
+  const int K = 9 * C;
+  const uint4 z4 = make_uint4(0, 0, 0, 0);
+  for (int i = t; i < HH * HW * C8; i += 256) {
+    const int ch = i % C8, pix = i / C8;
+    const int hy = pix / HW, hx = pix - hy * HW;
+    const int yy = y0 + hy - 1, xx = x0 + hx - 1;
+    uint4 v = z4;
+    if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.Wd)
+      v = *reinterpret_cast<const uint4*>(a.X + ((img * a.H + yy) * a.Wd + xx) * C + ch * 8);
+    *reinterpret_cast<uint4*>(&halo[pix * CS + ch * 8]) = v;
+  }
+  uint4 rb[BLD];
+  auto gload = [&](int k0) {
+    for (int j = 0; j < BLD; ++j) {
+      const int i = t + 256 * j;
+      const int row = i >> 2, kk = k0 + (i & 3) * 8;
+      const bool ok = i < BNc * 4 && n0 + row < a.Cout && kk < K;
+      rb[j] = ok ? *reinterpret_cast<const uint4*>(a.Wt + (long long)(n0 + row) * K + kk) : z4;
+    }
+  };
+  auto sstore = [&](int buf) {
+    for (int j = 0; j < BLD; ++j) {
+      const int i = t + 256 * j;
+      if (i < BNc * 4) *reinterpret_cast<uint4*>(&Bs[buf][i >> 2][(i & 3) * 8]) = rb[j];
+    }
+  };
+  floatx4 acc[MB][NB];
+  for (int i = 0; i < MB; ++i)
+    for (int j = 0; j < NB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  gload(0);
+  sstore(0);
+  __syncthreads();   // the halo and the first weight tile
+  const int fr = lane & 15, fg = lane >> 4;
+  int hbase[MB];     // halo offset of this lane's pixel in each block (tap 0)
+  for (int i = 0; i < MB; ++i) {
+    const int p = w * (MB * 16) + i * 16 + fr;
+    hbase[i] = ((p / HTW) * HW + (p % HTW)) * CS;
+  }
+  for (int st = 0; st < NSTEP; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < NSTEP;
+    if (more) gload((st + 1) * 32);
+    const int g = st * 4 + fg;
+    const bool gin = g < NG;
+    const int tap = gin ? g / C8 : 0;
+    const int off = ((tap / 3) * HW + (tap % 3)) * CS + (g - tap * C8) * 8;
+    bf16x8 av[MB], bv[NB];
+    for (int i = 0; i < MB; ++i)
+      av[i] = __builtin_bit_cast(
+          bf16x8, gin ? *reinterpret_cast<const uint4*>(&halo[hbase[i] + off]) : z4);
+    for (int j = 0; j < NB; ++j)
+      bv[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Bs[buf][j * 16 + fr][fg * 8]));
+    for (int i = 0; i < MB; ++i)
+      for (int j = 0; j < NB; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    if (more) sstore(buf ^ 1);
+    __syncthreads();
+  }
+  const int cc = lane & 15, cr = (lane >> 4) * 4;
+  for (int j = 0; j < NB; ++j) {
+    const int n = n0 + j * 16 + cc;
+    if (n >= a.Cout) continue;
+    const float bvv = a.bias ? a.bias[n] : 0.f;
+    for (int i = 0; i < MB; ++i)
+      for (int r = 0; r < 4; ++r) {
+        const int p = w * (MB * 16) + i * 16 + cr + r;
+        const int yy = y0 + p / HTW, xx = x0 + p % HTW;
+        if (yy >= a.H || xx >= a.Wd) continue;
+        const long long o = ((img * a.H + yy) * a.Wd + xx) * a.Cout + n;
+        float v = a.alpha * acc[i][j][r] + bvv;
+        if (a.relu) v = fmaxf(v, 0.f);
+        if (a.out_bf16) ((bfbits*)a.Y)[o] = f2bf(v);
+        else ((float*)a.Y)[o] = v;
+      }
+  }
+}
+
 // Weight gradient of the 3x3 convolution without a column buffer:
 //   dW[o][k] = sum_p dZ[p][o] A(p, k)   (A as in k_conv3_igemm)
 // with the pixels as the MFMA reduction dimension.  Both operands are stored
@@ -632,9 +741,46 @@ extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const 
   a.relu = relu ? 1 : 0;
   a.out_bf16 = out_bf16 ? 1 : 0;
   hipStream_t st = (hipStream_t)s;
-  const unsigned gx = (unsigned)((a.P + CM - 1) / CM);
   const int nb = conv_tile(Cout);
   const unsigned gy = (unsigned)((Cout + 16 * nb - 1) / (16 * nb));
+  // inputs of 8, 24 or 48 channels into up to 64 outputs, or 96 channels
+  // into up to 48: halo tiles of 8 x 32 pixels, or 4 x 32 where 8-row tiles
+  // would give fewer than 2048 workgroups.  Wider outputs keep the gathered
+  // form (their accumulators leave the halo kernel two waves per SIMD;
+  // measured on the U-Net's layers at batch 16)
+  const int c8 = C / 8;
+  if (((c8 == 1 || c8 == 3 || c8 == 6) && nb <= 4) || (c8 == 12 && nb <= 3)) {
+    const long long tiles_x = (W + HTW - 1) / HTW;
+    const int th = ((H + 7) / 8) * tiles_x * B * gy >= 2048 ? 8 : 4;
+    const long long tiles_y = (H + th - 1) / th;
+    const long long nt = tiles_x * tiles_y * B;
+    if (nt > 0x7fffffffLL) BH_FAIL(BH_ERR_ARG, "bh_nn_conv3: %lld tiles", nt);
+    const dim3 grid((unsigned)nt, gy);
+    const int tx = (int)tiles_x, ty = (int)tiles_y;
+#define BH_HALO(C8V, THV)                                                                      \
+  do {                                                                                        \
+    if (nb == 1) hipLaunchKernelGGL((k_conv3_halo<C8V, THV, 1>), grid, dim3(256), 0, st, a, tx, ty); \
+    else if (nb == 2) hipLaunchKernelGGL((k_conv3_halo<C8V, THV, 2>), grid, dim3(256), 0, st, a, tx, ty); \
+    else if (nb == 3) hipLaunchKernelGGL((k_conv3_halo<C8V, THV, 3>), grid, dim3(256), 0, st, a, tx, ty); \
+    else if (nb == 4) hipLaunchKernelGGL((k_conv3_halo<C8V, THV, 4>), grid, dim3(256), 0, st, a, tx, ty); \
+    else hipLaunchKernelGGL((k_conv3_halo<C8V, THV, 6>), grid, dim3(256), 0, st, a, tx, ty); \
+  } while (0)
+    if (th == 8) {
+      if (c8 == 1) BH_HALO(1, 8);
+      else if (c8 == 3) BH_HALO(3, 8);
+      else if (c8 == 6) BH_HALO(6, 8);
+      else BH_HALO(12, 8);
+    } else {
+      if (c8 == 1) BH_HALO(1, 4);
+      else if (c8 == 3) BH_HALO(3, 4);
+      else if (c8 == 6) BH_HALO(6, 4);
+      else BH_HALO(12, 4);
+    }
+#undef BH_HALO
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+  }
+  const unsigned gx = (unsigned)((a.P + CM - 1) / CM);
   const dim3 grid(gx, gy);
   if (nb == 1) hipLaunchKernelGGL(k_conv3_igemm<1>, grid, dim3(256), 0, st, a);
   else if (nb == 2) hipLaunchKernelGGL(k_conv3_igemm<2>, grid, dim3(256), 0, st, a);
