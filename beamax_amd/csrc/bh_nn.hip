@@ -695,6 +695,130 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ pa
   }
 }
 
+// Weight gradient for inputs of 24 or 48 channels into up to 64 outputs,
+// image width a multiple of 8: one workgroup holds the whole dW tile (every
+// tap, channel and output channel), so dZ is read once, and the X staging
+// reads each input row segment once for all three horizontal taps: a task
+// (row offset dy, 8-channel group, 8-pixel block) loads the 10 pixels
+// x - 1 .. x + 8 of its row, takes the three 8-pixel windows of the three
+// horizontal taps from them in registers and writes each transposed (8 x 8)
+// into the k rows of its tap.  Steps of 32 pixels, LDS double-buffered with
+// the next step's loads issued before the multiplies; wave w owns the
+// 16-column k blocks w, w + 4, ... for every output block.
+template <int C8, int NB>
+__global__ __launch_bounds__(256) void k_conv3_dw_full(ConvDwArgs a) {
+  constexpr int C = 8 * C8;
+  constexpr int K = 9 * C;
+  constexpr int KB = (K + 15) / 16;          // 16-column k blocks
+  constexpr int KBW = (KB + 3) / 4;          // per wave
+  constexpr int BO = 16 * NB;
+  constexpr int NX = 12 * C8;                // X tasks: 3 rows x C8 groups x 4 blocks
+  constexpr int NZ = 4 * (BO / 8);           // dZ tasks (8-channel groups x 4 blocks)
+  static_assert(NX <= 128 && NZ <= 128, "staging tasks per half of the workgroup");
+  __shared__ __attribute__((aligned(16))) bfbits Xs[2][KB * 16][DWLD];
+  __shared__ __attribute__((aligned(16))) bfbits Zs[2][BO][DWLD];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const long long pbeg = blockIdx.z * a.chunk;
+  const long long pend = min(a.P, pbeg + a.chunk);
+  const uint4 z4 = make_uint4(0, 0, 0, 0);
+  // k rows past 9C are padding of the last block: zero in both buffers
+  for (int i = t; i < 2 * (KB * 16 - K) * (DWLD / 8); i += 256) {
+    const int b = i / ((KB * 16 - K) * (DWLD / 8)), r = i % ((KB * 16 - K) * (DWLD / 8));
+    *reinterpret_cast<uint4*>(&Xs[b][K + r / (DWLD / 8)][(r % (DWLD / 8)) * 8]) = z4;
+  }
+  const bool xst = t < NX;
+  const bool zst = t >= 128 && t - 128 < NZ;
+  const int sl = xst ? t : t - 128;
+  const int pb = sl & 3;
+  const int xg = (sl >> 2) % C8, xdy = (sl >> 2) / C8;   // X task: channel group, row offset
+  const int zg = sl >> 2;                                // dZ task: channel group
+  const bool zin = zst && 8 * zg < a.Cout;
+  uint4 r[10];
+  auto gload = [&](long long p0) {
+    const long long p = p0 + 8 * pb;   // the block's first pixel (8 pixels, one image row)
+    if (xst) {
+      const unsigned q = (unsigned)(p < pend ? p : 0);
+      const int x = (int)(q % (unsigned)a.Wd);
+      const unsigned rr = q / (unsigned)a.Wd;
+      const int yy = (int)(rr % (unsigned)a.H) + xdy - 1;
+      const long long n = rr / (unsigned)a.H;
+      const bool rowok = p < pend && yy >= 0 && yy < a.H;
+      const bfbits* row = a.X + ((n * a.H + yy) * a.Wd) * (long long)C + 8 * xg;
+#pragma unroll
+      for (int i = 0; i < 10; ++i) {
+        const int xx = x - 1 + i;
+        r[i] = (rowok && xx >= 0 && xx < a.Wd)
+                   ? *reinterpret_cast<const uint4*>(row + (long long)xx * C) : z4;
+      }
+    } else if (zst) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        r[i] = (zin && p + i < pend)
+                   ? *reinterpret_cast<const uint4*>(a.dZ + (p + i) * a.Cout + 8 * zg) : z4;
+    }
+  };
+  auto sstore = [&](int buf) {
+    if (xst) {
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        uint4 in[8], o[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in[i] = r[dx + i];
+        transpose8x8(in, o);
+        const int k0 = (3 * xdy + dx) * C + 8 * xg;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) *reinterpret_cast<uint4*>(&Xs[buf][k0 + j][8 * pb]) = o[j];
+      }
+    } else if (zst) {
+      uint4 in[8], o[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) in[i] = r[i];
+      transpose8x8(in, o);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) *reinterpret_cast<uint4*>(&Zs[buf][8 * zg + j][8 * pb]) = o[j];
+    }
+  };
+  floatx4 acc[KBW][NB];
+  for (int i = 0; i < KBW; ++i)
+    for (int j = 0; j < NB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int nsteps = (int)((pend - pbeg + DWP - 1) / DWP);
+  if (nsteps > 0) {
+    gload(pbeg);
+    sstore(0);
+  }
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int st = 0; st < nsteps; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nsteps;
+    if (more) gload(pbeg + (long long)(st + 1) * DWP);
+    bf16x8 zf[NB];
+    for (int j = 0; j < NB; ++j)
+      zf[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Zs[buf][16 * j + fr][fk]));
+    for (int i = 0; i < KBW; ++i) {
+      const int kb = w + 4 * i;
+      if (kb >= KB) break;
+      const bf16x8 xf =
+          __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Xs[buf][16 * kb + fr][fk]));
+      for (int j = 0; j < NB; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(zf[j], xf, acc[i][j], 0, 0, 0);
+    }
+    if (more) sstore(buf ^ 1);
+    __syncthreads();
+  }
+  float* out = a.out + blockIdx.z * a.split_stride;
+  const int cc = lane & 15, cr = (lane >> 4) * 4;
+  for (int i = 0; i < KBW; ++i) {
+    const int k = 16 * (w + 4 * i) + cc;
+    if (k >= K) continue;
+    for (int j = 0; j < NB; ++j)
+      for (int q = 0; q < 4; ++q) {
+        const int o = 16 * j + cr + q;
+        if (o < a.Cout) out[(long long)o * K + k] = acc[i][j][q];
+      }
+  }
+}
+
 // Weights of the input gradient as a convolution of dY:
 // Wf[c][tap' Cout + o] = Wt[o][(8 - tap') C + c]
 __global__ void k_conv3_wflip(const bfbits* __restrict__ Wt, int Cout, int C,
@@ -791,11 +915,20 @@ extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const 
   return BH_OK;
 }
 
+// the whole-tile weight gradient: 24 or 48 input and up to 64 output
+// channels, widths a multiple of 8 (8 input channels give it too few X
+// staging tasks per step: the k-tiled kernel is faster there)
+static bool conv_dw_whole(int W, int C, int Cout) {
+  return (C == 24 || C == 48) && Cout <= 64 && W % 8 == 0;
+}
+
 // pixel split of the weight gradient: about 2048 workgroups (the kernel is
 // bound by the latency of its staging loads, so it needs several resident
-// workgroups per CU), chunks of at least 256 pixels, a multiple of 32
-static void conv_dw_split(long long P, int C, int Cout, long long* chunk, int* nsplit) {
-  const long long tiles = (long long)((9 * C + DWK - 1) / DWK) *
+// workgroups per CU; 1024 of the whole-tile kernel, which does more per
+// step), chunks of at least 256 pixels, a multiple of 32
+static void conv_dw_split(long long P, int W, int C, int Cout, long long* chunk, int* nsplit) {
+  const bool whole = conv_dw_whole(W, C, Cout);
+  const long long tiles = whole ? 2 : (long long)((9 * C + DWK - 1) / DWK) *
                           ((Cout + 16 * conv_tile(Cout) - 1) / (16 * conv_tile(Cout)));
   long long ns = (2048 + tiles - 1) / tiles;
   ns = std::max(1LL, std::min(ns, (P + 255) / 256));
@@ -809,7 +942,7 @@ extern "C" size_t bh_nn_conv3_dw_workspace_size(int64_t B, int H, int W, int C, 
   if (B <= 0 || H <= 0 || W <= 0 || C <= 0 || Cout <= 0) return 0;
   long long chunk;
   int ns;
-  conv_dw_split(B * (long long)H * W, C, Cout, &chunk, &ns);
+  conv_dw_split(B * (long long)H * W, W, C, Cout, &chunk, &ns);
   return ns > 1 ? (size_t)ns * Cout * 9 * C * sizeof(float) : 0;
 }
 
@@ -834,7 +967,7 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
   a.C = C;
   a.Cout = Cout;
   int ns;
-  conv_dw_split(P, C, Cout, &a.chunk, &ns);
+  conv_dw_split(P, W, C, Cout, &a.chunk, &ns);
   const long long n = (long long)Cout * 9 * C;
   a.split_stride = n;
   if (ns > 1) {
@@ -847,6 +980,19 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
   }
   hipStream_t st = (hipStream_t)s;
   const int mo = conv_tile(Cout);
+  if (conv_dw_whole(W, C, Cout)) {
+    const dim3 g1(1, 1, (unsigned)ns);
+#define BH_DWF(C8V)                                                                            \
+  do {                                                                                        \
+    if (mo == 1) hipLaunchKernelGGL((k_conv3_dw_full<C8V, 1>), g1, dim3(256), 0, st, a);       \
+    else if (mo == 2) hipLaunchKernelGGL((k_conv3_dw_full<C8V, 2>), g1, dim3(256), 0, st, a);  \
+    else if (mo == 3) hipLaunchKernelGGL((k_conv3_dw_full<C8V, 3>), g1, dim3(256), 0, st, a);  \
+    else hipLaunchKernelGGL((k_conv3_dw_full<C8V, 4>), g1, dim3(256), 0, st, a);               \
+  } while (0)
+    if (C == 24) BH_DWF(3);
+    else BH_DWF(6);
+#undef BH_DWF
+  } else {
   const dim3 grid((unsigned)((9 * C + DWK - 1) / DWK), (unsigned)((Cout + 16 * mo - 1) / (16 * mo)),
                   (unsigned)ns);
   if (mo == 1) hipLaunchKernelGGL(k_conv3_dw<1>, grid, dim3(256), 0, st, a);
@@ -854,6 +1000,7 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
   else if (mo == 3) hipLaunchKernelGGL(k_conv3_dw<3>, grid, dim3(256), 0, st, a);
   else if (mo == 4) hipLaunchKernelGGL(k_conv3_dw<4>, grid, dim3(256), 0, st, a);
   else hipLaunchKernelGGL(k_conv3_dw<6>, grid, dim3(256), 0, st, a);
+  }
   BH_LAUNCH_CHECK();
   if (ns > 1) {
     hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, st,
