@@ -503,16 +503,19 @@ def warm_start(net: UNet, op, data, image):
 
 
 def evaluate(net: Optional[UNet], problem, seeds, device, tol=1e-3, log=None, graph=True):
-    """CG on the normal equations from zero and from the learned start, for
-    each held-out seed: iterations, seconds (CG alone, and with the
-    network's inference and the least-squares scaling), error against p0.
-    Both runs replay a captured CG iteration (``graph``)."""
+    """CG on the normal equations from zero and from the learned start, each
+    also with the Fourier preconditioner (``reconstruct.FourierPreconditioner``,
+    |k|^-0.5 with floor 0.03), for each held-out seed: iterations, seconds
+    (CG alone, and with the network's inference and the least-squares
+    scaling), error against p0.  Every run replays a captured CG iteration
+    (``graph``)."""
     import torch
-    from .reconstruct import cgnr
+    from .reconstruct import FourierPreconditioner, cgnr
     from .solvers.linear import FrozenForward
     from .workloads import phantom_2d
     dom, wpt, sensors, ts, s, fa = problem
     N = dom.N
+    pc = FourierPreconditioner(N, device=device)
     rows = []
     for sd in seeds:
         p0 = torch.as_tensor(phantom_2d(N, dom.dx, seed=int(sd)), device=device)
@@ -524,6 +527,9 @@ def evaluate(net: Optional[UNet], problem, seeds, device, tol=1e-3, log=None, gr
         row = {"seed": int(sd), "plain_iters": info["iterations"],
                "plain_s": info["seconds"],
                "plain_err": float(((x - p0).norm() / p0.norm()).item())}
+        xp, infop = cgnr(F, d, tol=tol, maxiter=1000, precond=pc, graph=graph)
+        row.update({"pc_iters": infop["iterations"], "pc_s": infop["seconds"],
+                    "pc_err": float(((xp - p0).norm() / p0.norm()).item())})
         if net is not None:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -536,6 +542,10 @@ def evaluate(net: Optional[UNet], problem, seeds, device, tol=1e-3, log=None, gr
                         "start_residual": info1["residuals"][0],
                         "start_err": float(((x0 - p0).norm() / p0.norm()).item()),
                         "learned_err": float(((x1 - p0).norm() / p0.norm()).item())})
+            x2, info2 = cgnr(F, d, x0=x0, tol=tol, maxiter=1000, precond=pc, graph=graph)
+            row.update({"learned_pc_iters": info2["iterations"],
+                        "learned_pc_s": info2["seconds"] + t_init,
+                        "learned_pc_err": float(((x2 - p0).norm() / p0.norm()).item())})
         rows.append(row)
         if log is not None:
             log(" ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
@@ -633,6 +643,17 @@ def _main(argv=None):
         "p90": float(np.percentile(ratios, 90)),
         "iterations_plain_hist": np.bincount([r["plain_iters"] for r in rows]).tolist(),
         "iterations_learned_hist": np.bincount([r["learned_iters"] for r in rows]).tolist()}
+    ratios_pc = np.array([r["learned_pc_s"] / r["plain_s"] for r in rows])
+    result["learned_pc_time_ratio_per_seed"] = {
+        "median": float(np.median(ratios_pc)), "min": float(ratios_pc.min()),
+        "max": float(ratios_pc.max()), "p10": float(np.percentile(ratios_pc, 10)),
+        "p90": float(np.percentile(ratios_pc, 90)),
+        "iterations_hist": np.bincount([r["learned_pc_iters"] for r in rows]).tolist()}
+    result.update({"pc_iters": mean("pc_iters"), "pc_s": mean("pc_s"), "pc_err": mean("pc_err"),
+                   "learned_pc_iters": mean("learned_pc_iters"),
+                   "learned_pc_s": mean("learned_pc_s"), "learned_pc_err": mean("learned_pc_err"),
+                   "pc_time_ratio": mean("pc_s") / mean("plain_s"),
+                   "learned_pc_time_ratio": mean("learned_pc_s") / mean("plain_s")})
     result.update({"eval_seeds": len(rows), "plain_iters": mean("plain_iters"),
                    "plain_s": mean("plain_s"), "learned_iters": mean("learned_iters"),
                    "learned_s": mean("learned_s"), "learned_cg_s": mean("learned_cg_s"),

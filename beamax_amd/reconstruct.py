@@ -10,7 +10,8 @@ direction is made F^T F-conjugate to the last ``window`` directions by
 explicit projection, so the preconditioner may change from step to step
 (a network, for instance).  The stopping test of both is the plain
 normal-equation residual ||F^T (d - F x)|| / ||F^T d||, whatever the
-preconditioner.
+preconditioner.  ``FourierPreconditioner`` is a fixed SPD preconditioner for
+2-D images (a negative power of |k| in the Fourier domain).
 
 Every scalar of an iteration (step length, conjugation weights, residual
 norm) stays a device tensor: an iteration issues no host synchronisation
@@ -57,6 +58,55 @@ class _State:
             self.r = d - op.apply(self.x).to(torch.float64)
         self.g = op.transpose(self.r).to(torch.float64)
         self.gg = _dot(self.g, self.g)
+
+
+class FourierPreconditioner:
+    """M g = Re F^-1[w F g] for a 2-D image g: the Fourier multiplier
+    w(k) = max(|k|, floor)^power / max_k w (|k| in cycles per sample), a
+    symmetric positive definite map, so it can precondition ``cgnr``.  With
+    a negative power it damps the high spatial frequencies, which the
+    normal operator F^T F of the MSGB forward amplifies; the 2-D FFTs are
+    the package's FFT kernel (``bh_fft``, rows then columns, FP64), applied
+    in place in a buffer allocated once (no allocation per call, so a
+    captured CG iteration can include it)."""
+
+    def __init__(self, shape, power: float = -0.5, floor: float = 0.03, device=None):
+        import torch
+        from . import _lib
+        self._lib = _lib
+        self.shape = tuple(int(n) for n in shape)
+        if len(self.shape) != 2:
+            raise ValueError(f"FourierPreconditioner: a 2-D image shape, got {self.shape}")
+        n0, n1 = self.shape
+        dev = torch.device("cuda") if device is None else torch.device(device)
+        k0 = torch.as_tensor(np.fft.fftfreq(n0), dtype=torch.float64, device=dev)
+        k1 = torch.as_tensor(np.fft.fftfreq(n1), dtype=torch.float64, device=dev)
+        kk = torch.sqrt(k0[:, None] ** 2 + k1[None, :] ** 2)
+        w = torch.clamp(kk, min=float(floor)) ** float(power)
+        self.w = (w / w.max()).unsqueeze(-1).contiguous()      # (n0, n1, 1)
+        self.buf = torch.zeros((n0, n1, 2), dtype=torch.float64, device=dev)
+        lib = _lib.load()
+        nb = max(lib.bh_fft_workspace_size(n1, n0), lib.bh_fft_workspace_size(n0, n1))
+        self.ws = _lib.workspace(max(int(nb), 8), dev)
+
+    def _fft2(self, direction: int, scale: float):
+        lib, n0, n1 = self._lib, *self.shape
+        st = lib.stream_handle()
+        # rows: transform t starts at t * n1, unit stride
+        lib.call("bh_fft", lib.ptr(self.buf), n1, n0, 1, 0, n1, 1, direction, 1.0,
+                 lib.ptr(self.ws), self.ws.numel(), st)
+        # columns: transform t starts at t, stride n1
+        lib.call("bh_fft", lib.ptr(self.buf), n0, n1, n1, 1, 0, n1, direction, scale,
+                 lib.ptr(self.ws), self.ws.numel(), st)
+
+    def __call__(self, g):
+        n0, n1 = self.shape
+        self.buf[..., 0].copy_(g.reshape(n0, n1))
+        self.buf[..., 1].zero_()
+        self._fft2(-1, 1.0)
+        self.buf.mul_(self.w)
+        self._fft2(1, 1.0 / (n0 * n1))
+        return self.buf[..., 0].reshape(g.shape)
 
 
 def cgnr(op, data, *, x0=None, tol: float = 1e-3, maxiter: int = 500, precond=None,

@@ -217,3 +217,48 @@ def test_cgnr_graph_replays_match_eager_iterations(gpu):
     assert rel(_n(x1), _n(x2)) < 1e-9
     x3, i3 = fcgnr(F, d, lambda g: g, tol=1e-4, maxiter=200, window=4)
     assert i3["converged"] and abs(i3["iterations"] - i1["iterations"]) <= 2
+
+
+def test_fourier_preconditioner_matches_numpy_and_is_spd(gpu):
+    """The Fourier preconditioner on the package's FFT kernel equals the same
+    multiplier applied with numpy's FFT (FP64, 1e-12), is symmetric and
+    positive on random images (so CG may use it), and inside a captured CG
+    iteration gives the eager iterations' residuals."""
+    import torch
+    import beamax_amd as bx
+    from beamax_amd.gb import gb_solvers
+    from beamax_amd.reconstruct import FourierPreconditioner, cgnr
+    from beamax_amd.solvers.faces import four_edge_mask
+    from beamax_amd.solvers.linear import FrozenForward
+    rng = np.random.default_rng(73)
+    N = (48, 40)
+    pc = FourierPreconditioner(N, power=-0.5, floor=0.03, device=gpu)
+    k0, k1 = np.fft.fftfreq(N[0]), np.fft.fftfreq(N[1])
+    w = np.maximum(np.sqrt(k0[:, None] ** 2 + k1[None, :] ** 2), 0.03) ** -0.5
+    w /= w.max()
+    a, b = rng.standard_normal(N), rng.standard_normal(N)
+    ref = np.real(np.fft.ifft2(np.fft.fft2(a) * w))
+    Ma = _n(pc(_t(a, gpu)))
+    assert rel(ref, Ma) < 1e-12
+    Mb = _n(pc(_t(b, gpu)))
+    assert abs(np.sum(Ma * b) - np.sum(a * Mb)) <= 1e-12 * np.sum(np.abs(Ma * b))
+    assert np.sum(Ma * a) > 0
+    # in a captured CG iteration on a small problem
+    Nn = (64, 64)
+    dom = bx.Domain(N=Nn, dx=(1e-4, 1e-4), c=1500.0, periodic=(False, False))
+    wpt = bx.MSWPT(bx.DyadicDecomposition(2, Nn, (4, 8), (1, 1)), 2, "rectangular_mirror",
+                   device=gpu)
+    sensors = bx.Sensor(dom, binary_mask=four_edge_mask(Nn))
+    ts = dom.generate_time_domain()
+    p = _t(rng.standard_normal(Nn), gpu)
+    s = bx.MSGBSolver(800, "top_n", 128, "spatial", gb_solvers.solve_hom_diag, "scan_real",
+                      precision="fp32_split", cull_log2=40.0)
+    F = FrozenForward.from_field(s, p, dom, sensors, ts, wpt)
+    d = F.apply(p).to(torch.float64) + 0.01 * _t(rng.standard_normal(F.shape[0]), gpu)
+    pc2 = FourierPreconditioner(Nn, device=gpu)
+    x1, i1 = cgnr(F, d, tol=1e-4, maxiter=200, precond=pc2)
+    x2, i2 = cgnr(F, d, tol=1e-4, maxiter=200, precond=pc2, graph=True)
+    assert i1["converged"] and i2["converged"]
+    assert i1["iterations"] == i2["iterations"]
+    np.testing.assert_allclose(i1["residuals"], i2["residuals"], rtol=1e-9)
+    assert rel(_n(x1), _n(x2)) < 1e-9
