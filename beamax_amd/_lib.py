@@ -62,7 +62,7 @@ class Speed(C.Structure):
 
 class BatchSolve(C.Structure):
     _fields_ = [("beams", FwdBeams), ("c", C.c_double), ("ts", C.c_void_p), ("nt", C.c_int64),
-                ("seg", Segs), ("out", C.c_void_p)]
+                ("seg", Segs), ("out", C.c_void_p), ("win", C.c_void_p)]
 
 
 _P = C.c_void_p
@@ -136,6 +136,9 @@ SIGNATURES = {
                                 C.c_double, C.c_double, C.c_double, C.c_double, _I64, _P, _P]),
     "bh_forward_sum_traj": (C.c_int, [C.POINTER(FwdBeams), C.c_int, C.c_int, _P, _P, _I64,
                                       C.POINTER(Segs), C.c_double, C.c_int, _P, _P, _P, _SZ, _P]),
+    "bh_forward_windows_workspace_size": (_SZ, [_I64, C.c_int]),
+    "bh_forward_batch_windows": (C.c_int, [C.POINTER(BatchSolve), C.c_int, C.c_int, C.c_double,
+                                           _P]),
     "bh_forward_batch_plan_size": (_SZ, [C.c_int]),
     "bh_forward_batch_plan": (C.c_int, [C.POINTER(BatchSolve), C.c_int, C.c_int, C.c_double, C.c_int,
                                         _P, _SZ, C.POINTER(C.c_int64), _P]),

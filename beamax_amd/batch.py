@@ -5,7 +5,8 @@ The reference runs solves one at a time (one ``MSGBSolver.forward`` call per
 grid).  ``BatchedForward`` takes a list of forward problems of different
 grids, beam counts, time grids and sensors, preallocates every buffer once,
 and issues per solve the (millisecond) MSWPT / top_n / parameter kernels and
-then a single ``bh_forward_batch_run`` launch that sums all solves' beams
+activity windows, then a single ``bh_forward_batch_run`` launch that sums
+all solves' beams
 (``k_culled_batch``: one 1-D grid, solve s owning a contiguous block range).
 Nothing in ``run`` allocates, copies from the host or synchronises, so
 ``capture`` records the whole batch into one ``torch.cuda.CUDAGraph`` and
@@ -78,6 +79,11 @@ class BatchedForward:
             }
             it["out"] = torch.empty((p.ts.size, it["seg"].S),
                                     dtype=torch.float64 if f64 else torch.float32, device=self.dev)
+            # activity windows of the solve's entries (rebuilt each run;
+            # zeroed so that the tables are valid indices even before)
+            it["win"] = (torch.zeros(max(int(lib.bh_forward_windows_workspace_size(k, 1)), 8),
+                                     dtype=torch.uint8, device=self.dev)
+                         if solver.cull_log2 > 0 else None)
             self.items.append(it)
         self.nsolve = len(self.items)
         self._plan = _lib.workspace(lib.bh_forward_batch_plan_size(self.nsolve), self.dev)
@@ -87,6 +93,7 @@ class BatchedForward:
         self.gpb = None
         self.nblocks = 0
         self.graph = None
+        self._stale_windows = True
         self.set_tile(gpb if gpb is not None else _TUNED.get(self.signature, 8))
 
     def set_tile(self, gpb: int):
@@ -99,6 +106,8 @@ class BatchedForward:
             arr[i].nt = p.ts.size
             arr[i].seg = it["seg"].struct()
             arr[i].out = it["out"].data_ptr()
+            arr[i].win = it["win"].data_ptr() if it["win"] is not None else None
+        self._solves = arr
         nb = C.c_int64(0)
         _lib.call("bh_forward_batch_plan", arr, self.nsolve, self.ndim, self.solver.cull_log2,
                   int(gpb), _lib.ptr(self._plan), self._plan.numel(), C.byref(nb),
@@ -109,6 +118,7 @@ class BatchedForward:
 
     def prepare(self):
         """Per solve: 0.5 * half-masked MSWPT, top_n, beam parameters."""
+        self._stale_windows = True
         s = _lib.stream_handle()
         for p, it in zip(self.problems, self.items):
             w = p.wpt
@@ -119,14 +129,25 @@ class BatchedForward:
             _lib.call("bh_forward_params", C.byref(w.tiling), it["gs"], _lib.ptr(it["idx"]),
                       _lib.ptr(it["val"]), it["k"], it["beams"].struct(), s)
 
+    def windows(self):
+        """Every solve's activity windows (the summation then skips the rows
+        on which an entry reaches no sensor)."""
+        _lib.call("bh_forward_batch_windows", self._solves, self.nsolve, self.ndim,
+                  self.solver.cull_log2, _lib.stream_handle())
+        self._stale_windows = False
+
     def sum(self):
-        """The one batched summation launch."""
+        """The one batched summation launch (after ``prepare`` and ``windows``)."""
+        if self._stale_windows and self.solver.cull_log2 > 0:
+            raise RuntimeError("BatchedForward.sum: the beams changed since the last "
+                               "windows() (run prepare(), windows(), sum(), or run())")
         _lib.call("bh_forward_batch_run", _lib.ptr(self._plan), self.nsolve, self.nblocks,
                   self.ndim, _lib.PRECISION[self.solver.precision], self.gpb, None,
                   _lib.stream_handle())
 
     def run(self):
         self.prepare()
+        self.windows()
         self.sum()
         return self.outputs
 
@@ -139,7 +160,7 @@ class BatchedForward:
                        for p, it in zip(self.problems, self.items)))
 
     def capture(self):
-        """Record prepare + sum into one HIP graph (torch.cuda.CUDAGraph)."""
+        """Record prepare + windows + sum into one HIP graph (torch.cuda.CUDAGraph)."""
         import torch
         self.run()
         torch.cuda.synchronize()
@@ -160,6 +181,7 @@ class BatchedForward:
         """Time the batched launch per tile shape; keep and cache the fastest."""
         import torch
         self.prepare()
+        self.windows()
         best = None
         times = {}
         if tiles is None:

@@ -732,24 +732,50 @@ size_t window_bytes(long long total) {
          round256((size_t)nchunk * 8);
 }
 
+// the window tables of `total` entries carved from ws, in this order
+struct WinTables {
+  double* trange;
+  unsigned long long *key, *key2;
+  long long *idx, *idx2;
+  int2* win;
+  int32_t* perm;
+  int2* rows_act;
+  int2* hull;
+};
+WinTables window_tables(void* ws, long long total) {
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* r = p; p += round256(bytes); return (void*)r; };
+  const long long P = bh::sort_pairs_capacity(total);
+  const long long nchunk = (total + kRecChunk - 1) / kRecChunk;
+  WinTables w;
+  w.trange = (double*)take(16);
+  w.key = (unsigned long long*)take((size_t)P * 8);
+  w.idx = (long long*)take((size_t)P * 8);
+  w.key2 = (unsigned long long*)take((size_t)P * 8);
+  w.idx2 = (long long*)take((size_t)P * 8);
+  w.win = (int2*)take((size_t)total * 8);
+  w.perm = (int32_t*)take((size_t)total * 4);
+  w.rows_act = (int2*)take((size_t)total * 8);
+  w.hull = (int2*)take((size_t)nchunk * 8);
+  return w;
+}
+
 // the windows and the entry order of a forward launch, carved from ws
 int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt, void* ws,
                   const int32_t** perm, const int2** rows_act, const int2** chunk_rows,
                   hipStream_t st) {
-  char* p = (char*)ws;
-  auto take = [&](size_t bytes) { char* r = p; p += round256(bytes); return (void*)r; };
   const long long total = a.total;
-  const long long P = bh::sort_pairs_capacity(total);
-  double* trange = (double*)take(16);
-  unsigned long long* key = (unsigned long long*)take((size_t)P * 8);
-  long long* idx = (long long*)take((size_t)P * 8);
-  unsigned long long* key2 = (unsigned long long*)take((size_t)P * 8);
-  long long* idx2 = (long long*)take((size_t)P * 8);
-  int2* win = (int2*)take((size_t)total * 8);
-  int32_t* pm = (int32_t*)take((size_t)total * 4);
-  int2* ra = (int2*)take((size_t)total * 8);
+  const WinTables wt = window_tables(ws, total);
+  double* trange = wt.trange;
+  unsigned long long* key = wt.key;
+  long long* idx = wt.idx;
+  unsigned long long* key2 = wt.key2;
+  long long* idx2 = wt.idx2;
+  int2* win = wt.win;
+  int32_t* pm = wt.perm;
+  int2* ra = wt.rows_act;
   const long long nchunk = (total + kRecChunk - 1) / kRecChunk;
-  int2* hull = (int2*)take((size_t)nchunk * 8);
+  int2* hull = wt.hull;
   hipLaunchKernelGGL(k_time_range, dim3(1), dim3(256), 0, st, ts, nt, trange);
   WinArgs W{};
   W.b = a.b;
@@ -1067,6 +1093,48 @@ extern "C" int bh_static_sum_culled(const bh_static_beams* b, int ndim, const bh
   return BH_OK;
 }
 
+namespace {
+// a solve of the batch visits its entries in activity-window order when it
+// brings a window workspace and the windows apply (a cut, closed-form beams)
+bool batch_windowed(const bh_batch_solve& q, double cull_log2) {
+  return q.win && cull_log2 > 0.0 && q.beams.count > 0 && 2 * q.beams.count < (1ll << 29) &&
+         q.nt > 0 && q.nt < 65535;
+}
+}  // namespace
+
+extern "C" size_t bh_forward_windows_workspace_size(int64_t count, int mirror) {
+  if (count <= 0) return 0;
+  return window_bytes(count * (mirror ? 2 : 1));
+}
+
+extern "C" int bh_forward_batch_windows(const bh_batch_solve* solves, int nsolve, int ndim,
+                                        double cull_log2, bh_stream s) {
+  bh::clear_error();
+  if (!solves || nsolve <= 0)
+    BH_FAIL(BH_ERR_ARG, "bh_forward_batch_windows: null pointer or nsolve=%d", nsolve);
+  if (ndim < 1 || ndim > 3) BH_FAIL(BH_ERR_ARG, "bh_forward_batch_windows: ndim=%d", ndim);
+  hipStream_t st = (hipStream_t)s;
+  for (int i = 0; i < nsolve; ++i) {
+    const bh_batch_solve& q = solves[i];
+    if (!batch_windowed(q, cull_log2)) continue;
+    if (int rc = check_segments(&q.seg, ndim)) return rc;
+    CulledArgs a{};
+    a.b = q.beams;
+    a.count = q.beams.count;
+    a.total = 2 * q.beams.count;
+    a.npos = q.beams.count;
+    a.mirror = 1;
+    a.c = q.c;
+    a.G = to_segdev(&q.seg);
+    a.E = cull_log2;
+    const int32_t* pm;
+    const int2* ra;
+    const int2* hull;
+    if (int rc = build_windows(ndim, a, q.ts, q.nt, q.win, &pm, &ra, &hull, st)) return rc;
+  }
+  return BH_OK;
+}
+
 extern "C" size_t bh_forward_batch_plan_size(int nsolve) {
   if (nsolve <= 0) return 0;
   return ((size_t)nsolve * sizeof(BatchDesc) + 255) / 256 * 256 + (size_t)(nsolve + 1) * 8;
@@ -1116,6 +1184,13 @@ extern "C" int bh_forward_batch_plan(const bh_batch_solve* solves, int nsolve, i
     a.rows = q.nt;
     a.ostride = 1;
     a.ooff = 0;
+    if (batch_windowed(q, cull_log2)) {
+      // the tables bh_forward_batch_windows fills before every run
+      const WinTables wt = window_tables(q.win, a.total);
+      a.perm = wt.perm;
+      a.rows_act = wt.rows_act;
+      a.chunk_rows = wt.hull;
+    }
     d[i].a = a;
     d[i].ngx = (q.seg.ngroups + gpb - 1) / gpb;
     start[i] = nb;

@@ -357,7 +357,19 @@ typedef struct bh_batch_solve {
   int64_t nt;
   bh_segments seg;
   void* out;
+  /* optional (null: none): bh_forward_windows_workspace_size(beams.count, 1)
+   * bytes of device memory for this solve's activity windows */
+  void* win;
 } bh_batch_solve;
+/* Activity windows (the forward entries' time rows of possible activity, as
+ * bh_forward_sum_culled builds them) of every solve that brings a window
+ * workspace, for a cut > 0; enqueues kernels only (capturable).  A plan
+ * built from the same solves then visits each solve's entries in window
+ * order and skips their inactive rows; call it again whenever the beams
+ * change (before every bh_forward_batch_run). */
+size_t bh_forward_windows_workspace_size(int64_t count, int mirror);
+int bh_forward_batch_windows(const bh_batch_solve* solves, int nsolve, int ndim,
+                             double cull_log2, bh_stream s);
 size_t bh_forward_batch_plan_size(int nsolve);
 int bh_forward_batch_plan(const bh_batch_solve* solves, int nsolve, int ndim, double cull_log2,
                           int gpb, void* plan, size_t plan_bytes, int64_t* nblocks, bh_stream s);
