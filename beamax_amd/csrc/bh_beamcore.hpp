@@ -406,3 +406,97 @@ __device__ inline void hom_diag_state_fast(const double* x0, const double* p0, c
 }
 
 }  // namespace
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// The FP32-split record of a 2-D closed-form beam with alpha = i b (b > 0, the
+// form bh_forward_params gives every forward beam) in real arithmetic.  With
+// s = c0 t / |p| and h = p / |p|:
+//   w_a = i b_a / (1 + i s b_a) = (s b_a^2 + i b_a) / (1 + s^2 b_a^2),
+//   den = 1 - s sum_a h_a^2 w_a,   M = diag(w) + (s / den) v v^T, v_a = w_a h_a
+// (hom_diag_state_fast's Sherman-Morrison form with q = w o p and
+// kappa q q^T = (s / den) v v^T), and c0 t ap = i s bh with
+// bh = h_0^2 b_1 + h_1^2 b_0, so A = a0 (1 + i gamma)^(-1/2), gamma = s bh:
+//   |A| = |a0| (1 + gamma^2)^(-1/4),  arg A = arg a0 - atan(gamma) / 2
+// (the principal root: Re(1 + i gamma) = 1 > 0).  The time-independent part
+// is FastEnt2 (fast_consts2); fast_rec2 adds the time.
+// ---------------------------------------------------------------------------
+struct FastEnt2 {
+  double x0[2], h[2], b[2];
+  double iP;     // 1 / |p|; <= 0: not of this form (the generic closed form applies)
+  double bh;
+  double ks;     // w |p| / (2 pi): k = ks h
+  double om;     // w
+  double amp0;   // 2 |a0|
+  double arg0;   // arg(a0) / (2 pi)
+};
+static_assert(sizeof(FastEnt2) == 96, "FastEnt2 is read as six 16-byte words");
+
+__device__ inline FastEnt2 fast_consts2(const double* x0, const double* p0, const cd* alpha, cd a0,
+                                        double w) {
+  FastEnt2 F;
+  F.iP = -1.0;
+  if (!(alpha[0].re == 0.0 && alpha[1].re == 0.0 && alpha[0].im > 0.0 && alpha[1].im > 0.0))
+    return F;
+  const double P = sqrt(p0[0] * p0[0] + p0[1] * p0[1]);
+  if (!(P > 0.0)) return F;
+  F.iP = 1.0 / P;
+  for (int a = 0; a < 2; ++a) {
+    F.x0[a] = x0[a];
+    F.h[a] = p0[a] * F.iP;
+    F.b[a] = alpha[a].im;
+  }
+  F.bh = F.h[0] * F.h[0] * F.b[1] + F.h[1] * F.h[1] * F.b[0];
+  F.ks = w * P * (1.0 / bh::kTwoPi);
+  F.om = w;
+  F.amp0 = 2.0 * sqrt(a0.re * a0.re + a0.im * a0.im);
+  F.arg0 = safe_angle(a0) * (1.0 / bh::kTwoPi);
+  return F;
+}
+
+// the record of fast_consts2's beam at time t (c0 = the mode's signed speed)
+__device__ inline void fast_rec2(const FastEnt2& F, double c0, double t, RecS<2>& R) {
+  const double ct = c0 * t;
+  const double s = ct * F.iP;
+  double wr[2], wi[2], vr[2], vi[2];
+  double sr = 0.0, si = 0.0;
+  for (int a = 0; a < 2; ++a) {
+    R.x[a] = fma(ct, F.h[a], F.x0[a]);
+    R.k[a] = F.ks * F.h[a];
+    const double sb = s * F.b[a];
+    const double r = frcp1(fma(sb, sb, 1.0));
+    wi[a] = F.b[a] * r;
+    wr[a] = sb * wi[a];
+    vr[a] = wr[a] * F.h[a];
+    vi[a] = wi[a] * F.h[a];
+    sr = fma(vr[a], F.h[a], sr);
+    si = fma(vi[a], F.h[a], si);
+  }
+  // g = s / den, den = 1 - s (sr + i si)
+  const double dr = fma(-s, sr, 1.0), di = -s * si;
+  const double id2 = frcp1(fma(dr, dr, di * di));
+  const double gr = s * dr * id2, gi = -s * di * id2;
+  // g v_a, then M_aa = w_a + (g v_a) v_a and M_01 = (g v_0) v_1
+  double Mr[3], Mi[3];
+  const double g0r = gr * vr[0] - gi * vi[0], g0i = gr * vi[0] + gi * vr[0];
+  const double g1r = gr * vr[1] - gi * vi[1], g1i = gr * vi[1] + gi * vr[1];
+  Mr[0] = wr[0] + (g0r * vr[0] - g0i * vi[0]);
+  Mi[0] = wi[0] + (g0r * vi[0] + g0i * vr[0]);
+  Mr[1] = wr[1] + (g1r * vr[1] - g1i * vi[1]);
+  Mi[1] = wi[1] + (g1r * vi[1] + g1i * vr[1]);
+  Mr[2] = 2.0 * (g0r * vr[1] - g0i * vi[1]);
+  Mi[2] = 2.0 * (g0r * vi[1] + g0i * vr[1]);
+  const double cr = F.om * (1.0 / (2.0 * bh::kTwoPi));
+  const double cm = 0.5 * F.om * bh::kLog2e;
+  for (int q = 0; q < 3; ++q) {
+    R.qr[q] = cr * Mr[q];
+    R.qi[q] = (float)(cm * Mi[q]);
+  }
+  const float gm = (float)(s * F.bh);
+  const float z = fmaf(gm, gm, 1.f);
+  R.amp = (float)F.amp0 * __builtin_amdgcn_rsqf(__builtin_amdgcn_sqrtf(z));
+  R.ph0 = F.arg0 - (double)(atanf(gm) * (float)(0.5 / bh::kTwoPi));
+}
+
+}  // namespace

@@ -54,6 +54,9 @@ struct CulledArgs {
   // with perm: per chunk of kRecChunk positions the hull [first, last] of
   // its entries' windows; a block skips a chunk outside it (null: no skip)
   const int2* chunk_rows;
+  // with perm, 2-D: per position the time-independent constants of its beam
+  // (fast_consts2); null: formed from the beam table when needed
+  const FastEnt2* fast;
   long long rows;      // output rows (nt for the forward, 1 for the static form)
   int ostride;         // output element o goes to out[o * ostride + ooff]
   int ooff;
@@ -185,29 +188,45 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
         const long long e = A.perm ? (long long)A.perm[bi] : bi;
         const bool neg = A.mirror ? e >= A.count : e >= A.npos;
         const long long j = (A.mirror && neg) ? e - A.count : e;
-        double x0[D], p0[D];
-        cd al[D];
-        for (int a = 0; a < D; ++a) {
-          x0[a] = A.b.x0[j * D + a];
-          p0[a] = A.b.p0[j * D + a];
-          al[a] = mkc(A.b.alpha[2 * (j * D + a)], A.b.alpha[2 * (j * D + a) + 1]);
-        }
-        State<D> St;
-        if (A.traj) {
-          // x, p, M (complex), A (complex) of beam bi at this time row
-          const double* y = A.traj + (t * A.total + e) * (2 * D + 2 * D * D + 2);
-          for (int a = 0; a < D; ++a) {
-            St.x[a] = y[a];
-            St.p[a] = y[D + a];
-            for (int e = 0; e < D; ++e)
-              St.M[a][e] = mkc(y[2 * D + 2 * (a * D + e)], y[2 * D + 2 * (a * D + e) + 1]);
+        // FP32-split 2-D records of alpha = i b beams from the window-order
+        // constants (fast_rec2, real arithmetic); other beams, and every beam
+        // of a launch without window tables, by the general closed form
+        // (the same record to rounding)
+        bool fastf = false;
+        if constexpr (D == 2 && !F64) {
+          if (!A.traj) {
+            FastEnt2 F;
+            F.iP = -1.0;
+            if (A.fast) F = A.fast[bi];
+            fastf = F.iP > 0.0;
+            if (fastf) fast_rec2(F, neg ? -A.c : A.c, tv, R);
           }
-          St.A = mkc(y[2 * D + 2 * D * D], y[2 * D + 2 * D * D + 1]);
-        } else {
-          hom_diag_state_fast<D, !F64>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]),
-                                 neg ? -A.c : A.c, tv, St);
         }
-        make_rec<D>(St, A.b.omega[j], R);
+        if (!fastf) {
+          double x0[D], p0[D];
+          cd al[D];
+          for (int a = 0; a < D; ++a) {
+            x0[a] = A.b.x0[j * D + a];
+            p0[a] = A.b.p0[j * D + a];
+            al[a] = mkc(A.b.alpha[2 * (j * D + a)], A.b.alpha[2 * (j * D + a) + 1]);
+          }
+          State<D> St;
+          if (A.traj) {
+            // x, p, M (complex), A (complex) of beam bi at this time row
+            const double* y = A.traj + (t * A.total + e) * (2 * D + 2 * D * D + 2);
+            for (int a = 0; a < D; ++a) {
+              St.x[a] = y[a];
+              St.p[a] = y[D + a];
+              for (int e = 0; e < D; ++e)
+                St.M[a][e] = mkc(y[2 * D + 2 * (a * D + e)], y[2 * D + 2 * (a * D + e) + 1]);
+            }
+            St.A = mkc(y[2 * D + 2 * D * D], y[2 * D + 2 * D * D + 1]);
+          } else {
+            hom_diag_state_fast<D, !F64>(x0, p0, al, mkc(A.b.a0[2 * j], A.b.a0[2 * j + 1]),
+                                         neg ? -A.c : A.c, tv, St);
+          }
+          make_rec<D>(St, A.b.omega[j], R);
+        }
         if (A.phase != 0.0) shift_phase<D>(R, A.phase);
       }
       live = !rec_zero<D>(R);   // zero amplitude contributes exactly 0
@@ -685,14 +704,27 @@ __global__ __launch_bounds__(128) void k_beam_windows(WinArgs W) {
 }
 
 // entries in window order: perm[bi] = sorted entry, rows_act[bi] its window
+// and, in 2-D (fast != null), fast[bi] its beam's fast_consts2
 __global__ void k_window_order(const long long* __restrict__ sidx, const int2* __restrict__ win,
                                long long total, int32_t* __restrict__ perm,
-                               int2* __restrict__ rows_act) {
+                               int2* __restrict__ rows_act, bh_fwd_beams b, long long count,
+                               int mirror, FastEnt2* __restrict__ fast) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= total) return;
   const long long e = sidx[i];
   perm[i] = (int32_t)e;
   rows_act[i] = win[e];
+  if (fast) {
+    const long long j = (mirror && e >= count) ? e - count : e;
+    double x0[2], p0[2];
+    cd al[2];
+    for (int a = 0; a < 2; ++a) {
+      x0[a] = b.x0[j * 2 + a];
+      p0[a] = b.p0[j * 2 + a];
+      al[a] = mkc(b.alpha[2 * (j * 2 + a)], b.alpha[2 * (j * 2 + a) + 1]);
+    }
+    fast[i] = fast_consts2(x0, p0, al, mkc(b.a0[2 * j], b.a0[2 * j + 1]), b.omega[j]);
+  }
 }
 
 // per chunk of kRecChunk positions, the hull of its entries' windows
@@ -729,7 +761,7 @@ size_t window_bytes(long long total) {
   const long long nchunk = (total + kRecChunk - 1) / kRecChunk;
   return round256(16) + 4 * round256((size_t)P * 8) + round256((size_t)total * 8) +
          round256((size_t)total * 4) + round256((size_t)total * 8) +
-         round256((size_t)nchunk * 8);
+         round256((size_t)nchunk * 8) + round256((size_t)total * sizeof(FastEnt2));
 }
 
 // the window tables of `total` entries carved from ws, in this order
@@ -741,6 +773,7 @@ struct WinTables {
   int32_t* perm;
   int2* rows_act;
   int2* hull;
+  FastEnt2* fast;
 };
 WinTables window_tables(void* ws, long long total) {
   char* p = (char*)ws;
@@ -757,13 +790,14 @@ WinTables window_tables(void* ws, long long total) {
   w.perm = (int32_t*)take((size_t)total * 4);
   w.rows_act = (int2*)take((size_t)total * 8);
   w.hull = (int2*)take((size_t)nchunk * 8);
+  w.fast = (FastEnt2*)take((size_t)total * sizeof(FastEnt2));
   return w;
 }
 
 // the windows and the entry order of a forward launch, carved from ws
 int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt, void* ws,
                   const int32_t** perm, const int2** rows_act, const int2** chunk_rows,
-                  hipStream_t st) {
+                  const FastEnt2** fast, hipStream_t st) {
   const long long total = a.total;
   const WinTables wt = window_tables(ws, total);
   double* trange = wt.trange;
@@ -799,8 +833,9 @@ int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt,
   unsigned long long* ko;
   long long* io;
   if (int rc = bh::sort_pairs(key, idx, key2, idx2, total, &ko, &io, st)) return rc;
+  FastEnt2* fe = ndim == 2 ? wt.fast : nullptr;
   hipLaunchKernelGGL(k_window_order, dim3(bh::grid1d(total, 256)), dim3(256), 0, st, io, win,
-                     total, pm, ra);
+                     total, pm, ra, a.b, a.count, a.mirror, fe);
   BH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_chunk_rows, dim3((unsigned)nchunk), dim3(kRecChunk), 0, st, ra, total,
                      hull);
@@ -808,6 +843,7 @@ int build_windows(int ndim, const CulledArgs& a, const double* ts, long long nt,
   *perm = pm;
   *rows_act = ra;
   *chunk_rows = hull;
+  *fast = fe;
   return BH_OK;
 }
 
@@ -983,7 +1019,7 @@ int forward_culled(const char* name, const bh_fwd_beams* b, int ndim, long long 
   if (!traj && cull_log2 > 0.0 && ws && ws_bytes >= part_std + window_bytes(a.total) &&
       a.total < (1ll << 29) && nt < 65535) {
     if (int rc = build_windows(ndim, a, ts, nt, (char*)ws + part_std, &a.perm, &a.rows_act,
-                               &a.chunk_rows, st))
+                               &a.chunk_rows, &a.fast, st))
       return rc;
     part_bytes = part_std;
   }
@@ -1130,7 +1166,8 @@ extern "C" int bh_forward_batch_windows(const bh_batch_solve* solves, int nsolve
     const int32_t* pm;
     const int2* ra;
     const int2* hull;
-    if (int rc = build_windows(ndim, a, q.ts, q.nt, q.win, &pm, &ra, &hull, st)) return rc;
+    const FastEnt2* fe;
+    if (int rc = build_windows(ndim, a, q.ts, q.nt, q.win, &pm, &ra, &hull, &fe, st)) return rc;
   }
   return BH_OK;
 }
@@ -1190,6 +1227,7 @@ extern "C" int bh_forward_batch_plan(const bh_batch_solve* solves, int nsolve, i
       a.perm = wt.perm;
       a.rows_act = wt.rows_act;
       a.chunk_rows = wt.hull;
+      a.fast = ndim == 2 ? wt.fast : nullptr;
     }
     d[i].a = a;
     d[i].ngx = (q.seg.ngroups + gpb - 1) / gpb;

@@ -496,6 +496,77 @@ def test_culled_forward_windows_keep_every_pair(gpu, d, kind):
             assert np.max(np.abs(o0 - o1)) <= tol * np.max(np.abs(o0)), (prec, E)
 
 
+@pytest.mark.parametrize("n", [256, 600])
+def test_long_run_forward_matches_dense(gpu, n):
+    """The FP32-split culled forward over long runs (four-edge sensors of an
+    n x n grid: runs of n points, 4 and 10 groups) against the dense FP64
+    oracle sum: rel-L2 <= 1e-5 and every point within the certified bound of
+    the cut (plus the FP32 rounding), with and without the beam split and the
+    activity windows; the evaluated (beam, group) pairs are the same in all
+    three, and without a cut they are every pair.  The complex sum: its real
+    part is half the real sum bit for bit, and it matches the FP64 kernel."""
+    import torch
+    from beamax_amd import _lib
+    from beamax_amd.segments import Segments
+    from beamax_amd.solvers.msgb_solver import ForwardBeams
+    from oracle import cpu
+    rng = np.random.default_rng(60 + n)
+    d, dx = 2, 1e-4
+    L = n * dx
+    B = 300
+    x0, p0, om, alpha, a0 = _random_beams(rng, B, d, L, 0.5 / dx)
+    fb = ForwardBeams(B, d, gpu)
+    for name, v in (("x0", x0), ("p0", p0), ("omega", om), ("alpha", alpha), ("a0", a0)):
+        getattr(fb, name).copy_(_t(v, gpu))
+    pts = _grid_points("edges", n, d, dx)
+    ts = np.arange(0, 1.2 * L / 1500.0, 2e-7)
+    seg = Segments(pts, [dx] * d, gpu)
+    dup = lambda v: np.concatenate([v, v])
+    ref = cpu.forward_sum(dup(x0), dup(p0), dup(alpha), dup(a0), dup(om),
+                          np.concatenate([np.ones(B), -np.ones(B)]), 1500.0, ts, pts)
+    amp_sum = 2.0 * 2.0 * np.sum(np.abs(a0))
+    ts_d = _t(ts, gpu)
+    lib = _lib.load()
+    sizes = {"windows": lib.bh_forward_culled_workspace_size(B, 1, ts.size, seg.S),
+             "split": lib.bh_forward_sum_culled_workspace_size(ts.size, seg.S), "none": 0}
+
+    def run(E, wsname, complex_out=0, prec="fp32_split"):
+        nb = sizes[wsname]
+        ws = _lib.workspace(nb, gpu) if nb else None
+        f64 = prec == "fp64"
+        dt = (torch.complex128 if f64 else torch.complex64) if complex_out else \
+            (torch.float64 if f64 else torch.float32)
+        out = torch.full((ts.size, seg.S), np.nan, dtype=dt, device=gpu)
+        cnt = torch.zeros(1, dtype=torch.uint64, device=gpu)
+        ss = seg.struct()
+        _lib.call("bh_forward_sum_modes", C.byref(fb.struct()), d, 0, 1, 1500.0, _lib.ptr(ts_d),
+                  ts.size, C.byref(ss), E, _lib.PRECISION[prec], complex_out, _lib.ptr(out),
+                  _lib.ptr(cnt), _lib.ptr(ws), nb, _lib.stream_handle())
+        return _n(out), int(cnt.item())
+
+    dense_pairs = 2 * B * ts.size * seg.ngroups
+    for E in (32.0, 40.0, 0.0):
+        counts = {}
+        for wsname in sizes:
+            got, pairs = run(E, wsname)
+            got = got.astype(np.float64)
+            assert np.all(np.isfinite(got))
+            assert rel(ref, got) < 1e-5, (E, wsname)
+            if E > 0:
+                bound = 2.0 ** -E * amp_sum
+                assert np.max(np.abs(got - ref)) <= bound + 1e-5 * np.max(np.abs(ref)), (E, wsname)
+            counts[wsname] = pairs
+        assert len(set(counts.values())) == 1, (E, counts)
+        assert 0 < counts["none"] <= dense_pairs
+        if E == 0.0:
+            assert counts["none"] == dense_pairs
+    re32, _ = run(32.0, "windows")
+    z32, _ = run(32.0, "windows", complex_out=1)
+    z64, _ = run(0.0, "none", complex_out=1, prec="fp64")
+    assert np.array_equal(z32.real, (0.5 * re32).astype(np.float32))
+    assert rel(z64, z32.astype(np.complex128)) < 1e-5
+
+
 @pytest.mark.parametrize("d", [2, 3])
 def test_culled_static_matches_dense(gpu, d):
     import torch
