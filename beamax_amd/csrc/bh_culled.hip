@@ -44,7 +44,8 @@ struct CulledArgs {
   double scale;
   void* out;
   unsigned long long* pairs;
-  double* part;        // split > 1: partial sums [split][rows][S] (FP64)
+  void* part;          // split > 1: partial sums [split][rows][S] (FP64; FP32 in the
+                       // FP32-split form, whose output is FP32)
   int split;
   // forward entries in window order (null: entries in table order): position
   // bi holds entry perm[bi] (its beam and mode as above), which can only be
@@ -392,7 +393,8 @@ __device__ __forceinline__ void culled_body(const CulledArgs& A, long long bx, l
     if (col < 0) continue;
     const long long o = STATIC ? col : t * A.G.S + col;
     if (A.split > 1) {
-      A.part[bz * A.rows * A.G.S + o] = acc[i];
+      if constexpr (F64) ((double*)A.part)[bz * A.rows * A.G.S + o] = acc[i];
+      else ((float*)A.part)[bz * A.rows * A.G.S + o] = (float)acc[i];
     } else {
       const long long oo = o * A.ostride + A.ooff;
       if constexpr (F64) ((double*)A.out)[oo] = A.scale * acc[i];
@@ -436,12 +438,13 @@ void k_culled_batch(const BatchDesc* __restrict__ d,
 }
 
 // ordered sum of the split partials (deterministic)
-__global__ void k_combine(const double* __restrict__ part, int split, long long n, double scale,
+template <typename P>
+__global__ void k_combine(const P* __restrict__ part, int split, long long n, double scale,
                           int f64, void* out, int ostride, int ooff) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int z = 0; z < split; ++z) s += part[(long long)z * n + i];
+  for (int z = 0; z < split; ++z) s += (double)part[(long long)z * n + i];
   const long long o = i * ostride + ooff;
   if (f64) ((double*)out)[o] = scale * s;
   else ((float*)out)[o] = (float)(scale * s);
@@ -946,7 +949,8 @@ int launch_culled(CulledArgs a, long long ny, long long nchunk, void* part_ws, s
   else if constexpr (F64) hipLaunchKernelGGL((k_culled_sum<D, 1, F64, STATIC>), grid, dim3(256), 0, s, a);
   if (a.split > 1) {
     const long long n = ny * a.G.S;
-    hipLaunchKernelGGL(k_combine, dim3(bh::grid1d(n, 256)), dim3(256), 0, s, (const double*)part_ws,
+    using P = typename std::conditional<F64, double, float>::type;
+    hipLaunchKernelGGL(k_combine<P>, dim3(bh::grid1d(n, 256)), dim3(256), 0, s, (const P*)part_ws,
                        a.split, n, a.scale, (int)F64, a.out, a.ostride, a.ooff);
   }
   return BH_OK;

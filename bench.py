@@ -40,12 +40,13 @@ FP32_VALU_PEAK_TFLOPS = 157.3
 # issue rate: 256 CUs x 64 lanes x 2.4 GHz / 4 / 2
 TRANS_BOUND_EVALS = 256 * 64 * 2.4e9 / 4 / 2
 # HBM bytes per forward launch of the culled FP32-split kernel on this
-# workload (8 beam ranges): rocprofv3 FETCH_SIZE (KiB raw, x2.0) + WRITE_SIZE
-# (KiB raw, x1.0) of the steady-state dispatch, both scaled by what the
-# counters read for a 512 MiB copy (profiles/k_culled_sum_msgb2d_512.md), at
-# cull_log2 = 32 (52461.3 and 308558.8 KiB).  Counters cannot be read from
-# inside the process, so other settings report null.
-MEASURED_FWD_TRAFFIC = {32.0: 52461.25 * 2.0 * 1024 + 308558.78125 * 1.0 * 1024}
+# workload (8 beam ranges, FP32 partial sums): rocprofv3 FETCH_SIZE (KiB raw,
+# x2.0) + WRITE_SIZE (KiB raw, x1.0) of the steady-state dispatch, both
+# scaled by what the counters read for a 512 MiB copy
+# (profiles/k_culled_sum_msgb2d_512.md), at cull_log2 = 32 (58556.9 and
+# 163857.4 KiB).  Counters cannot be read from inside the process, so other
+# settings report null.
+MEASURED_FWD_TRAFFIC = {32.0: 58556.875 * 2.0 * 1024 + 163857.375 * 1.0 * 1024}
 
 
 def build_problem(device, cull_log2=32.0, seed=0):
