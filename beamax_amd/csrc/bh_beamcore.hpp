@@ -439,19 +439,31 @@ __device__ inline FastEnt2 fast_consts2(const double* x0, const double* p0, cons
   F.iP = -1.0;
   if (!(alpha[0].re == 0.0 && alpha[1].re == 0.0 && alpha[0].im > 0.0 && alpha[1].im > 0.0))
     return F;
-  const double P = sqrt(p0[0] * p0[0] + p0[1] * p0[1]);
-  if (!(P > 0.0)) return F;
-  F.iP = 1.0 / P;
+  // 1 / |p| and 2 |a0| from the hardware estimates with one Newton step
+  // (about 1e-15 relative; the IEEE forms outside [1e-200, 1e200]); arg a0
+  // in FP32 like the general record's arg A (a few 1e-8 cycles)
+  const double P2 = p0[0] * p0[0] + p0[1] * p0[1];
+  if (!(P2 > 0.0)) return F;
+  double iP;
+  if (P2 > 1e-200 && P2 < 1e200) {
+    const double y = __builtin_amdgcn_rsq(P2);
+    iP = y * fma(-0.5 * P2 * y, y, 1.5);
+  } else {
+    iP = 1.0 / sqrt(P2);
+  }
+  F.iP = iP;
   for (int a = 0; a < 2; ++a) {
     F.x0[a] = x0[a];
-    F.h[a] = p0[a] * F.iP;
+    F.h[a] = p0[a] * iP;
     F.b[a] = alpha[a].im;
   }
   F.bh = F.h[0] * F.h[0] * F.b[1] + F.h[1] * F.h[1] * F.b[0];
-  F.ks = w * P * (1.0 / bh::kTwoPi);
+  F.ks = w * (P2 * iP) * (1.0 / bh::kTwoPi);
   F.om = w;
-  F.amp0 = 2.0 * sqrt(a0.re * a0.re + a0.im * a0.im);
-  F.arg0 = safe_angle(a0) * (1.0 / bh::kTwoPi);
+  F.amp0 = 2.0 * fsqrt1(a0.re * a0.re + a0.im * a0.im);
+  const float are = (float)a0.re, aim = (float)a0.im;
+  F.arg0 = (double)(atan2f(aim, (are == 0.f && aim == 0.f) ? 1e-12f : are) *
+                    (float)(1.0 / bh::kTwoPi));
   return F;
 }
 

@@ -229,15 +229,6 @@ void k_fwd_transpose_q(TQArgs A) {
   const double ld = (double)lane;
   const float lf = (float)lane;
   for (long long k = (long long)blockIdx.x * 4 + wv; k < A.count; k += nw) {
-    double x0[D], p0[D];
-    cd al[D];
-    for (int a = 0; a < D; ++a) {
-      x0[a] = A.b.x0[k * D + a];
-      p0[a] = A.b.p0[k * D + a];
-      al[a] = mkc(A.b.alpha[2 * (k * D + a)], A.b.alpha[2 * (k * D + a) + 1]);
-    }
-    const cd a0 = mkc(A.b.a0[2 * k], A.b.a0[2 * k + 1]);
-    const double w = A.b.omega[k];
     double acc_re = 0.0, acc_im = 0.0;
     for (int md = 0; md < A.modes; ++md) {
       if (A.win) {
@@ -248,9 +239,31 @@ void k_fwd_transpose_q(TQArgs A) {
       RecS<D> R;
       bool live = tvalid;
       if (live) {
-        State<D> St;
-        hom_diag_state_fast<D, true>(x0, p0, al, a0, md ? -A.c : A.c, tv, St);
-        make_rec<D>(St, w, R);
+        double x0[D], p0[D];
+        cd al[D];
+        for (int a = 0; a < D; ++a) {
+          x0[a] = A.b.x0[k * D + a];
+          p0[a] = A.b.p0[k * D + a];
+          al[a] = mkc(A.b.alpha[2 * (k * D + a)], A.b.alpha[2 * (k * D + a) + 1]);
+        }
+        const cd a0 = mkc(A.b.a0[2 * k], A.b.a0[2 * k + 1]);
+        const double w = A.b.omega[k];
+        // 2-D beams with alpha = i b: the forward's records in real
+        // arithmetic (fast_rec2, the same records as the windowed forward
+        // sum); the test is the same for the whole wave (one beam)
+        bool done = false;
+        if constexpr (D == 2) {
+          const FastEnt2 F = fast_consts2(x0, p0, al, a0, w);
+          if (F.iP > 0.0) {
+            fast_rec2(F, md ? -A.c : A.c, tv, R);
+            done = true;
+          }
+        }
+        if (!done) {
+          State<D> St;
+          hom_diag_state_fast<D, true>(x0, p0, al, a0, md ? -A.c : A.c, tv, St);
+          make_rec<D>(St, w, R);
+        }
         live = !rec_zero<D>(R);
       }
       if (!live) zero_rec<D>(R);
