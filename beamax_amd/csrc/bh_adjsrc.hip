@@ -1,22 +1,86 @@
 // Adjoint source (spectral time derivative) and the Fourier crop along time.
+//
+// The (time, sensor) arrays are transformed along time.  Internally they are
+// held sensor-major -- one contiguous time series per sensor -- so that the
+// FFTs read and write whole rows; the layout changes on the way in and out
+// through 32 x 32 tiles in LDS (coalesced on both sides).  Transforming the
+// time-major array in place instead read every element 8 KB from the last
+// (512 sensors of complex doubles per time step): the 2414-point transforms
+// of one 512-sensor face took 236 us that way, 205 us on rows.
 #include "bh_internal.hpp"
 
 namespace {
 
-__global__ void k_load_windowed(const double* __restrict__ d, const double* __restrict__ w,
-                                int w2d, long long nt, long long ns, double2* __restrict__ X) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= nt * ns) return;
-  double v = d[i];
-  if (w) v = (w2d ? w[i] : w[i / ns]) * v;
-  X[i] = make_double2(v, 0.0);
+constexpr int kTile = 32;
+
+// X[j nt + t] = w(t, j) d[t ns_in + c_j]  (c_j = cols[j] or j; w = wt[j], or
+// the window w1[t] / w2[t ns + j], or 1), as complex
+__global__ __launch_bounds__(256) void k_load_t(const double* __restrict__ d, long long nt,
+                                                long long ns_in, const long long* __restrict__ cols,
+                                                const double* __restrict__ wt,
+                                                const double* __restrict__ w1,
+                                                const double* __restrict__ w2, long long ns,
+                                                double2* __restrict__ X) {
+  __shared__ double tile[kTile][kTile + 1];
+  const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+  const long long t0 = blockIdx.x * (long long)kTile, j0 = blockIdx.y * (long long)kTile;
+  const long long j = j0 + tx;
+  const bool jin = j < ns;
+  const long long cj = jin ? (cols ? cols[j] : j) : 0;
+  for (int r = ty; r < kTile; r += 256 / kTile) {
+    const long long t = t0 + r;
+    double v = 0.0;
+    if (jin && t < nt) {
+      v = d[t * ns_in + cj];
+      if (wt) v *= wt[j];
+      if (w1) v = w1[t] * v;
+      if (w2) v = w2[t * ns + j] * v;
+    }
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  for (int r = ty; r < kTile; r += 256 / kTile) {
+    const long long jj = j0 + r, t = t0 + tx;
+    if (jj < ns && t < nt) X[jj * nt + t] = make_double2(tile[tx][r], 0.0);
+  }
 }
 
-// X[k, s] *= 2 pi i f_k,  f = fftfreq(nt, dt) (msgb_solver.py:98-103)
+// out[t ns + j] = scale_j Re Y[j nt + t]  (scale_j = -c_j^2, or 1)
+__global__ __launch_bounds__(256) void k_store_t(const double2* __restrict__ Y, long long nt,
+                                                 long long ns, const double* __restrict__ c,
+                                                 double* __restrict__ out) {
+  __shared__ double tile[kTile][kTile + 1];
+  const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+  const long long t0 = blockIdx.x * (long long)kTile, j0 = blockIdx.y * (long long)kTile;
+  for (int r = ty; r < kTile; r += 256 / kTile) {
+    const long long jj = j0 + r, t = t0 + tx;
+    tile[r][tx] = (jj < ns && t < nt) ? Y[jj * nt + t].x : 0.0;
+  }
+  __syncthreads();
+  const long long j = j0 + tx;
+  for (int r = ty; r < kTile; r += 256 / kTile) {
+    const long long t = t0 + r;
+    if (j < ns && t < nt) {
+      const double v = tile[tx][r];
+      if (c) {
+        const double cs = c[j];
+        out[t * ns + j] = -(cs * cs) * v;
+      } else {
+        out[t * ns + j] = v;
+      }
+    }
+  }
+}
+
+dim3 tiles(long long nt, long long ns) {
+  return dim3((unsigned)((nt + kTile - 1) / kTile), (unsigned)((ns + kTile - 1) / kTile));
+}
+
+// X[s, k] *= 2 pi i f_k,  f = fftfreq(nt, dt) (msgb_solver.py:98-103)
 __global__ void k_deriv_mult(double2* __restrict__ X, long long nt, long long ns, double dt) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= nt * ns) return;
-  const long long k = i / ns;
+  const long long k = i % nt;
   const long long half = (nt - 1) / 2 + 1;
   const double val = 1.0 / ((double)nt * dt);
   const double f = (double)(k < half ? k : k - nt) * val;
@@ -25,41 +89,15 @@ __global__ void k_deriv_mult(double2* __restrict__ X, long long nt, long long ns
   X[i] = make_double2(-m * x.y, m * x.x);
 }
 
-__global__ void k_store_source(const double2* __restrict__ X, const double* __restrict__ c,
-                               long long nt, long long ns, double* __restrict__ out) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= nt * ns) return;
-  const double cs = c[i % ns];
-  out[i] = -(cs * cs) * X[i].x;
-}
-
-// gather selected columns (times weight) of an (nt, ns_in) real array
-__global__ void k_load_cols(const double* __restrict__ d, long long nt, long long ns_in,
-                            const long long* __restrict__ cols, const double* __restrict__ wt,
-                            long long ns, double2* __restrict__ X) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= nt * ns) return;
-  const long long t = i / ns, j = i - t * ns;
-  const long long cj = cols ? cols[j] : j;
-  double v = d[t * ns_in + cj];
-  if (wt) v *= wt[j];
-  X[i] = make_double2(v, 0.0);
-}
-
-// keep frequencies f in [-m/2, m - m/2): Y[f mod m] = X[f mod n]
+// keep frequencies f in [-m/2, m - m/2): Y[j, f mod m] = X[j, f mod n]
 __global__ void k_crop(const double2* __restrict__ X, long long n, long long m, long long ns,
                        double2* __restrict__ Y) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= m * ns) return;
-  const long long r = i / ns, j = i - r * ns;
+  const long long j = i / m, r = i - j * m;
   const long long f = r < (m - m / 2) ? r : r - m;
   const long long src = f >= 0 ? f : f + n;
-  Y[i] = X[src * ns + j];
-}
-
-__global__ void k_real_part(const double2* __restrict__ Y, long long n, double* __restrict__ out) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i < n) out[i] = Y[i].x;
+  Y[i] = X[j * n + src];
 }
 
 }  // namespace
@@ -85,17 +123,17 @@ extern "C" int bh_adjoint_source(const double* data, int64_t nt, int64_t ns, dou
   void* fws = (char*)ws + (size_t)(nt * ns) * sizeof(double2);
   const size_t fws_bytes = ws_bytes - (size_t)(nt * ns) * sizeof(double2);
   const unsigned g = bh::grid1d(nt * ns, 256);
-  hipLaunchKernelGGL(k_load_windowed, dim3(g), dim3(256), 0, st, data, window, window_2d,
-                     (long long)nt, (long long)ns, X);
+  // X sensor-major: row j holds sensor j's time series
+  hipLaunchKernelGGL(k_load_t, tiles(nt, ns), dim3(256), 0, st, data, (long long)nt,
+                     (long long)ns, nullptr, nullptr, (window && !window_2d) ? window : nullptr,
+                     (window && window_2d) ? window : nullptr, (long long)ns, X);
   BH_LAUNCH_CHECK();
-  if (int rc = bh::fft_strided(X, nt, ns, ns, 1, nt * ns, ns, -1, 1.0, fws, fws_bytes, st))
-    return rc;
+  if (int rc = bh::fft_strided(X, nt, ns, 1, 0, nt, 1, -1, 1.0, fws, fws_bytes, st)) return rc;
   hipLaunchKernelGGL(k_deriv_mult, dim3(g), dim3(256), 0, st, X, (long long)nt, (long long)ns, dt);
-  if (int rc = bh::fft_strided(X, nt, ns, ns, 1, nt * ns, ns, +1, 1.0 / (double)nt, fws,
-                               fws_bytes, st))
+  if (int rc = bh::fft_strided(X, nt, ns, 1, 0, nt, 1, +1, 1.0 / (double)nt, fws, fws_bytes, st))
     return rc;
-  hipLaunchKernelGGL(k_store_source, dim3(g), dim3(256), 0, st, X, c_src, (long long)nt,
-                     (long long)ns, out);
+  hipLaunchKernelGGL(k_store_t, tiles(nt, ns), dim3(256), 0, st, X, (long long)nt, (long long)ns,
+                     c_src, out);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }
@@ -123,20 +161,21 @@ extern "C" int bh_fourier_crop_time(const double* data, int64_t nt, int64_t ns_i
   double2* Y = X + nt * ns;
   void* fws = (char*)(Y + nt_out * ns);
   const size_t fws_bytes = ws_bytes - (size_t)((nt + nt_out) * ns) * sizeof(double2);
-  hipLaunchKernelGGL(k_load_cols, dim3(bh::grid1d(nt * ns, 256)), dim3(256), 0, st, data,
-                     (long long)nt, (long long)ns_in, (const long long*)cols, col_weight,
+  // X, Y sensor-major: row j holds column j's time series
+  hipLaunchKernelGGL(k_load_t, tiles(nt, ns), dim3(256), 0, st, data, (long long)nt,
+                     (long long)ns_in, (const long long*)cols, col_weight, nullptr, nullptr,
                      (long long)ns, X);
   BH_LAUNCH_CHECK();
-  if (int rc = bh::fft_strided(X, nt, ns, ns, 1, nt * ns, ns, -1, 1.0 / sqrt((double)nt), fws,
+  if (int rc = bh::fft_strided(X, nt, ns, 1, 0, nt, 1, -1, 1.0 / sqrt((double)nt), fws,
                                fws_bytes, st))
     return rc;
   hipLaunchKernelGGL(k_crop, dim3(bh::grid1d(nt_out * ns, 256)), dim3(256), 0, st, X,
                      (long long)nt, (long long)nt_out, (long long)ns, Y);
-  if (int rc = bh::fft_strided(Y, nt_out, ns, ns, 1, nt_out * ns, ns, +1,
-                               1.0 / sqrt((double)nt_out), fws, fws_bytes, st))
+  if (int rc = bh::fft_strided(Y, nt_out, ns, 1, 0, nt_out, 1, +1, 1.0 / sqrt((double)nt_out),
+                               fws, fws_bytes, st))
     return rc;
-  hipLaunchKernelGGL(k_real_part, dim3(bh::grid1d(nt_out * ns, 256)), dim3(256), 0, st, Y,
-                     (long long)(nt_out * ns), out);
+  hipLaunchKernelGGL(k_store_t, tiles(nt_out, ns), dim3(256), 0, st, Y, (long long)nt_out,
+                     (long long)ns, nullptr, out);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }

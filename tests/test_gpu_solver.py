@@ -190,6 +190,13 @@ def test_adjoint_source_matches_oracle(gpu):
         got = _n(s.adjoint_source(torch.as_tensor(data, device=gpu), 1e-8, 1500.0, win))
         ref = oadj.adjoint_source(data, 1e-8, np.full(9, 1500.0), win)
         assert rel(ref, got) < 1e-12
+        # a window per (time, detector) and a speed per detector
+        win2 = rng.uniform(0.5, 1.5, (nt, 9))
+        cs = rng.uniform(1400.0, 1600.0, 9)
+        got = _n(s.adjoint_source(torch.as_tensor(data, device=gpu), 1e-8,
+                                  torch.as_tensor(cs, device=gpu), win2))
+        ref = oadj.adjoint_source(data, 1e-8, cs, win2)
+        assert rel(ref, got) < 1e-12
 
 
 def test_fourier_crop_matches_reference_recipe(gpu):
