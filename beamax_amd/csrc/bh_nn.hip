@@ -336,13 +336,23 @@ struct ConvArgs {
   int relu, out_bf16;
 };
 
-template <int NB>
-__global__ __launch_bounds__(256) void k_conv3_igemm(ConvArgs a) {
+template <int NB, int KS>
+__global__ __launch_bounds__(256 * KS) void k_conv3_igemm(ConvArgs a) {
+  // KS k groups of four waves: group g multiplies the k steps g, g + KS, ...
+  // into its own accumulators (with its own LDS buffers), and group 0 adds
+  // the other groups' tiles at the end in group order.  Two groups give a
+  // launch of few workgroups (the 32^2 and 64^2 layers: 256-512) eight
+  // waves per workgroup to hide each other's staging latency.
   constexpr int BNc = 16 * NB;
   constexpr int BLD = (BNc * 4 + 255) / 256;   // weight segments per thread
-  __shared__ __attribute__((aligned(16))) bfbits As[2][CM][CLD];
-  __shared__ __attribute__((aligned(16))) bfbits Bs[2][BNc][CLD];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  constexpr int AS_E = KS * 2 * CM * CLD, BS_E = KS * 2 * BNc * CLD;
+  constexpr int RED_B = KS > 1 ? (KS - 1) * 4 * 2 * NB * 64 * 16 : 0;   // other groups' tiles
+  constexpr int SM_B = (AS_E + BS_E) * 2 > RED_B ? (AS_E + BS_E) * 2 : RED_B;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[SM_B];
+  auto As = reinterpret_cast<bfbits(*)[2][CM][CLD]>(smem);
+  auto Bs = reinterpret_cast<bfbits(*)[2][BNc][CLD]>(smem + AS_E * 2);
+  const int g = KS > 1 ? (int)(threadIdx.x >> 8) : 0;
+  const int t = threadIdx.x & 255, lane = t & 63, w = t >> 6;
   const long long p0 = blockIdx.x * (long long)CM;
   const int n0 = blockIdx.y * BNc;
   const int K = 9 * a.C;
@@ -359,6 +369,7 @@ __global__ __launch_bounds__(256) void k_conv3_igemm(ConvArgs a) {
     pn[h] = q / ((long long)a.Wd * a.H);
   }
   uint4 ra[2], rb[BLD];
+  // k steps past the last read zeros (a group may have one step fewer)
   auto gload = [&](int k0) {
     const int k = k0 + seg;
     const bool kin = k < K;
@@ -368,48 +379,70 @@ __global__ __launch_bounds__(256) void k_conv3_igemm(ConvArgs a) {
     for (int h = 0; h < 2; ++h) {
       const int yy = py[h] + dy, xx = px[h] + dx;
       const bool ok = pv[h] && kin && yy >= 0 && yy < a.H && xx >= 0 && xx < a.Wd;
-      ra[h] = ok ? *reinterpret_cast<const uint4*>(
-                       a.X + ((pn[h] * a.H + yy) * a.Wd + xx) * (long long)a.C + ci)
-                 : make_uint4(0, 0, 0, 0);
+      ra[h] = load16_or_zero(ok, a.X + ((pn[h] * a.H + yy) * a.Wd + xx) * (long long)a.C + ci);
     }
     for (int j = 0; j < BLD; ++j) {
       const int i = t + 256 * j;
       const int row = i >> 2, kk = k0 + (i & 3) * 8;
       const bool ok = i < BNc * 4 && n0 + row < a.Cout && kk < K;
-      rb[j] = ok ? *reinterpret_cast<const uint4*>(a.Wt + (long long)(n0 + row) * K + kk)
-                 : make_uint4(0, 0, 0, 0);
+      rb[j] = load16_or_zero(ok, a.Wt + (long long)(n0 + row) * K + kk);
     }
   };
   auto sstore = [&](int buf) {
-    for (int h = 0; h < 2; ++h) *reinterpret_cast<uint4*>(&As[buf][(t >> 2) + 64 * h][seg]) = ra[h];
+    for (int h = 0; h < 2; ++h)
+      *reinterpret_cast<uint4*>(&As[g][buf][(t >> 2) + 64 * h][seg]) = ra[h];
     for (int j = 0; j < BLD; ++j) {
       const int i = t + 256 * j;
-      if (i < BNc * 4) *reinterpret_cast<uint4*>(&Bs[buf][i >> 2][(i & 3) * 8]) = rb[j];
+      if (i < BNc * 4) *reinterpret_cast<uint4*>(&Bs[g][buf][i >> 2][(i & 3) * 8]) = rb[j];
     }
   };
   floatx4 acc[2][NB];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < NB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int nsteps = (K + CK - 1) / CK;
-  gload(0);
+  const int niter = (nsteps + KS - 1) / KS;   // the same for every group
+  gload(g * CK);
   sstore(0);
   __syncthreads();
   const int fr = lane & 15, fk = (lane >> 4) * 8;
-  for (int st = 0; st < nsteps; ++st) {
-    const int buf = st & 1;
-    const bool more = st + 1 < nsteps;
-    if (more) gload((st + 1) * CK);
+  for (int it = 0; it < niter; ++it) {
+    const int buf = it & 1;
+    const bool more = it + 1 < niter;
+    if (more) gload(((it + 1) * KS + g) * CK);
     bf16x8 av[2], bv[NB];
     for (int i = 0; i < 2; ++i)
-      av[i] = __builtin_bit_cast(bf16x8,
-                                 *reinterpret_cast<const uint4*>(&As[buf][w * 32 + i * 16 + fr][fk]));
+      av[i] = __builtin_bit_cast(
+          bf16x8, *reinterpret_cast<const uint4*>(&As[g][buf][w * 32 + i * 16 + fr][fk]));
     for (int j = 0; j < NB; ++j)
-      bv[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Bs[buf][j * 16 + fr][fk]));
+      bv[j] = __builtin_bit_cast(bf16x8,
+                                 *reinterpret_cast<const uint4*>(&Bs[g][buf][j * 16 + fr][fk]));
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < NB; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
     if (more) sstore(buf ^ 1);
     __syncthreads();
+  }
+  if constexpr (KS > 1) {
+    // the other groups' accumulator tiles through LDS (the operand buffers
+    // are free after the loop's last barrier), added by group 0 in order
+    float4* red = reinterpret_cast<float4*>(smem);
+    if (g > 0)
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < NB; ++j) {
+          const floatx4 v = acc[i][j];
+          red[((((g - 1) * 4 + w) * 2 + i) * NB + j) * 64 + lane] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    __syncthreads();
+    if (g > 0) return;
+    for (int q = 1; q < KS; ++q)
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < NB; ++j) {
+          const float4 v = red[((((q - 1) * 4 + w) * 2 + i) * NB + j) * 64 + lane];
+          acc[i][j][0] += v.x;
+          acc[i][j][1] += v.y;
+          acc[i][j][2] += v.z;
+          acc[i][j][3] += v.w;
+        }
   }
   const int cc = lane & 15, cr = (lane >> 4) * 4;
   for (int j = 0; j < NB; ++j) {
@@ -924,11 +957,20 @@ extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const 
   }
   const unsigned gx = (unsigned)((a.P + CM - 1) / CM);
   const dim3 grid(gx, gy);
-  if (nb == 1) hipLaunchKernelGGL(k_conv3_igemm<1>, grid, dim3(256), 0, st, a);
-  else if (nb == 2) hipLaunchKernelGGL(k_conv3_igemm<2>, grid, dim3(256), 0, st, a);
-  else if (nb == 3) hipLaunchKernelGGL(k_conv3_igemm<3>, grid, dim3(256), 0, st, a);
-  else if (nb == 4) hipLaunchKernelGGL(k_conv3_igemm<4>, grid, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(k_conv3_igemm<6>, grid, dim3(256), 0, st, a);
+  // fewer than four workgroups per CU: two k groups per workgroup
+  if ((long long)gx * gy < 1024) {
+    if (nb == 1) hipLaunchKernelGGL((k_conv3_igemm<1, 2>), grid, dim3(512), 0, st, a);
+    else if (nb == 2) hipLaunchKernelGGL((k_conv3_igemm<2, 2>), grid, dim3(512), 0, st, a);
+    else if (nb == 3) hipLaunchKernelGGL((k_conv3_igemm<3, 2>), grid, dim3(512), 0, st, a);
+    else if (nb == 4) hipLaunchKernelGGL((k_conv3_igemm<4, 2>), grid, dim3(512), 0, st, a);
+    else hipLaunchKernelGGL((k_conv3_igemm<6, 2>), grid, dim3(512), 0, st, a);
+  } else {
+    if (nb == 1) hipLaunchKernelGGL((k_conv3_igemm<1, 1>), grid, dim3(256), 0, st, a);
+    else if (nb == 2) hipLaunchKernelGGL((k_conv3_igemm<2, 1>), grid, dim3(256), 0, st, a);
+    else if (nb == 3) hipLaunchKernelGGL((k_conv3_igemm<3, 1>), grid, dim3(256), 0, st, a);
+    else if (nb == 4) hipLaunchKernelGGL((k_conv3_igemm<4, 1>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_conv3_igemm<6, 1>), grid, dim3(256), 0, st, a);
+  }
   BH_LAUNCH_CHECK();
   return BH_OK;
 }
