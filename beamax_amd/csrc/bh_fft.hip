@@ -9,7 +9,14 @@
 // Stage s with radix R over sub-length Ns produces output o from
 //   k = o mod Ns, q = (o / Ns) mod R, j = (o / (Ns R)) Ns + k,
 //   out[o] = sum_r in[j + r n/R] w^(r (n/(Ns R)) (k + q Ns)),  w = e^(dir 2 pi i / n)
-// which is the radix-R Stockham butterfly with its twiddle folded in.
+// which is the radix-R Stockham butterfly with its twiddle folded in.  For
+// n a power of two (radices 4, then one 2) the stages run as butterflies
+// instead: thread j of a stage reads in[j + r n/R] (r < R), multiplies
+// input r by w^(r m k) (k = j mod Ns, m = n / (Ns R)) and writes the R-point
+// DFT of the products to out[(j - k) R + q Ns + k] (q < R) -- the same
+// outputs, with the index arithmetic and twiddle reads of one butterfly
+// shared by its R outputs (the per-output form spent most of its time on
+// divisions by Ns and R).
 #include "bh_internal.hpp"
 
 namespace {
@@ -20,6 +27,7 @@ struct FftArgs {
   double2* data;
   long long batch, inner, inner_stride, outer_stride, elem_stride;
   int n, nst, tpb, dir;
+  int pow2;   // every radix 4 or 2
   double scale;
   int radix[kMaxStages];
 };
@@ -30,6 +38,7 @@ bool make_plan(int64_t n, FftArgs& a) {
   int64_t m = n;
   while (m % 4 == 0 && a.nst < kMaxStages) { a.radix[a.nst++] = 4; m /= 4; }
   while (m % 2 == 0 && a.nst < kMaxStages) { a.radix[a.nst++] = 2; m /= 2; }
+  a.pow2 = m == 1 ? 1 : 0;
   for (int64_t f = 3; m > 1 && a.nst < kMaxStages; f += 2) {
     if (f * f > m) { a.radix[a.nst++] = (int)m; m = 1; break; }
     while (m % f == 0 && a.nst < kMaxStages) { a.radix[a.nst++] = (int)f; m /= f; }
@@ -47,16 +56,71 @@ __global__ __launch_bounds__(1024) void k_fft_lds(FftArgs a) {
   double2* b0 = sm;
   double2* b1 = sm + tot;
   double2* tw = sm + 2 * tot;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    double sv, cv;
-    sincospi(2.0 * (double)i / (double)n, &sv, &cv);
-    tw[i] = make_double2(cv, a.dir * sv);
+  // twiddles w^i = (cos, dir sin)(2 pi i / n): sincospi for the first
+  // octant (n divisible by 8) or half turn (n even), the rest by the exact
+  // symmetries (c, s)(pi/2 - x) = (s, c)(x), (c, s)(x + pi/2) = (-s, c)(x)
+  // and (c, s)(x + pi) = (-c, -s)(x); each entry written once
+  const double dr = (double)a.dir;
+  if ((n & 7) == 0) {
+    const int n4 = n >> 2, n8 = n >> 3;
+    for (int i = threadIdx.x; i <= n8; i += blockDim.x) {
+      double sv, cv;
+      sincospi(2.0 * (double)i / (double)n, &sv, &cv);
+      double c1 = cv, s1 = sv, c2 = sv, s2 = cv;   // angles x and pi/2 - x
+      for (int q = 0; q < 4; ++q) {
+        tw[i + q * n4] = make_double2(c1, dr * s1);
+        if (i > 0 && i < n8) tw[n4 - i + q * n4] = make_double2(c2, dr * s2);
+        double t = c1; c1 = -s1; s1 = t;
+        t = c2; c2 = -s2; s2 = t;
+      }
+    }
+  } else if ((n & 1) == 0) {
+    const int n2 = n >> 1;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      double sv, cv;
+      sincospi(2.0 * (double)i / (double)n, &sv, &cv);
+      tw[i] = make_double2(cv, dr * sv);
+      tw[i + n2] = make_double2(-cv, -dr * sv);
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      double sv, cv;
+      sincospi(2.0 * (double)i / (double)n, &sv, &cv);
+      tw[i] = make_double2(cv, dr * sv);
+    }
   }
   const long long t0 = (long long)blockIdx.x * tpb;
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    int tr, e;
+  // the block's global loads are all issued before the first is stored to
+  // LDS (a loop storing each before issuing the next waited for one memory
+  // round trip per iteration); up to kLoads per thread, the rest in a loop
+  constexpr int kLoads = 8;
+  auto src_of = [&](int idx, int& tr, int& e) {
     if (a.elem_stride == 1) { tr = idx / n; e = idx - tr * n; }
     else { e = idx / tpb; tr = idx - e * tpb; }
+  };
+  double2 lv[kLoads];
+#pragma unroll
+  for (int q = 0; q < kLoads; ++q) {
+    const int idx = threadIdx.x + q * blockDim.x;
+    lv[q] = make_double2(0.0, 0.0);
+    if (idx < tot) {
+      int tr, e;
+      src_of(idx, tr, e);
+      if (t0 + tr < a.batch) lv[q] = a.data[fft_base(t0 + tr, a) + (long long)e * a.elem_stride];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kLoads; ++q) {
+    const int idx = threadIdx.x + q * blockDim.x;
+    if (idx < tot) {
+      int tr, e;
+      src_of(idx, tr, e);
+      b0[tr * n + e] = lv[q];
+    }
+  }
+  for (int idx = threadIdx.x + kLoads * blockDim.x; idx < tot; idx += blockDim.x) {
+    int tr, e;
+    src_of(idx, tr, e);
     const long long t = t0 + tr;
     double2 v = make_double2(0.0, 0.0);
     if (t < a.batch) v = a.data[fft_base(t, a) + (long long)e * a.elem_stride];
@@ -64,6 +128,47 @@ __global__ __launch_bounds__(1024) void k_fft_lds(FftArgs a) {
   }
   __syncthreads();
   int Ns = 1;
+  if (a.pow2) {
+    for (int st = 0; st < a.nst; ++st) {
+      const int R = a.radix[st];
+      const int T = n / R;
+      const int m = n / (Ns * R);
+      for (int bi = threadIdx.x; bi < tpb * T; bi += blockDim.x) {
+        const int tr = bi / T;
+        const int j = bi - tr * T;
+        const int k = j & (Ns - 1);
+        const double2* src = b0 + tr * n + j;
+        double2* dst = b1 + tr * n + (j - k) * R + k;
+        const int e = m * k;   // r e < n for r < R
+        auto cmul = [](double2 x, double2 w) {
+          return make_double2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+        };
+        if (R == 4) {
+          const double2 v0 = src[0];
+          const double2 v1 = cmul(src[T], tw[e]);
+          const double2 v2 = cmul(src[2 * T], tw[2 * e]);
+          const double2 v3 = cmul(src[3 * T], tw[3 * e]);
+          const double2 a0 = make_double2(v0.x + v2.x, v0.y + v2.y);
+          const double2 a1 = make_double2(v0.x - v2.x, v0.y - v2.y);
+          const double2 a2 = make_double2(v1.x + v3.x, v1.y + v3.y);
+          // (v1 - v3) times e^(dir pi i / 2) = dir i
+          const double2 a3 = make_double2(-dr * (v1.y - v3.y), dr * (v1.x - v3.x));
+          dst[0] = make_double2(a0.x + a2.x, a0.y + a2.y);
+          dst[Ns] = make_double2(a1.x + a3.x, a1.y + a3.y);
+          dst[2 * Ns] = make_double2(a0.x - a2.x, a0.y - a2.y);
+          dst[3 * Ns] = make_double2(a1.x - a3.x, a1.y - a3.y);
+        } else {
+          const double2 v0 = src[0];
+          const double2 v1 = cmul(src[T], tw[e]);
+          dst[0] = make_double2(v0.x + v1.x, v0.y + v1.y);
+          dst[Ns] = make_double2(v0.x - v1.x, v0.y - v1.y);
+        }
+      }
+      __syncthreads();
+      double2* tmp = b0; b0 = b1; b1 = tmp;
+      Ns *= R;
+    }
+  } else
   for (int st = 0; st < a.nst; ++st) {
     const int R = a.radix[st];
     const int T = n / R;
@@ -211,7 +316,9 @@ int fft_strided(double2* data, int64_t n, int64_t batch, int64_t inner, int64_t 
     const unsigned grid = (unsigned)((batch + a.tpb - 1) / a.tpb);
     // a workgroup that holds one long transform takes 16 waves, so that the
     // CU it fills alone still has four waves per SIMD to hide LDS latency
-    const unsigned threads = (a.tpb == 1 && n >= 1024) ? 1024 : 256;
+    // (the butterfly form of a power of two has n / 4 butterflies per
+    // transform and stage: 256 threads)
+    const unsigned threads = (a.tpb == 1 && n >= 1024 && !a.pow2) ? 1024 : 256;
     hipLaunchKernelGGL(k_fft_lds, dim3(grid), dim3(threads), lds, s, a);
     BH_LAUNCH_CHECK();
     return BH_OK;
