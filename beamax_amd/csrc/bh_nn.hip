@@ -585,14 +585,11 @@ __global__ __launch_bounds__(256) void k_conv3_halo(ConvArgs a, int tiles_x, int
 
 // Weight gradient of the 3x3 convolution without a column buffer:
 //   dW[o][k] = sum_p dZ[p][o] A(p, k)   (A as in k_conv3_igemm)
-// with the pixels as the MFMA reduction dimension.  Both operands are stored
-// pixel-major, so each 32-pixel step stages them transposed: a staging thread
-// loads an 8 pixel x 8 channel block (eight 16-byte loads, X gathered at the
-// block's tap, zero outside the image), transposes it in registers and writes
-// eight 16-byte LDS rows of 8 pixels.  Workgroup tile: 16 MO output channels x
-// 128 k, four waves of 16 MO x 32; grid z splits the pixels into chunks whose
-// FP32 partial tiles are added in order afterwards (deterministic).
-constexpr int DWK = 128, DWP = 32, DWLD = DWP + 8;
+// with the pixels as the MFMA reduction dimension, in 32-pixel steps; grid z
+// splits the pixels into chunks whose FP32 partial tiles are added in order
+// afterwards (deterministic).  Both operands lie pixel-major in memory; the
+// kernels below stage them that way and let the LDS read transpose them.
+constexpr int DWK = 128, DWP = 32;
 
 struct ConvDwArgs {
   const bfbits* X;
@@ -603,126 +600,6 @@ struct ConvDwArgs {
   long long chunk;          // pixels per split, a multiple of DWP
   long long split_stride;   // floats between split partials
 };
-
-// o[j] = column j of the 8 x 8 bf16 block whose row i is r[i]
-__device__ inline void transpose8x8(const uint4 (&r)[8], uint4 (&o)[8]) {
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    unsigned lo[4], hi[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const unsigned x = (&r[2 * b].x)[j], y = (&r[2 * b + 1].x)[j];
-      lo[b] = (x & 0xffffu) | (y << 16);
-      hi[b] = (x >> 16) | (y & 0xffff0000u);
-    }
-    o[2 * j] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
-    o[2 * j + 1] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
-  }
-}
-
-template <int MO>
-__global__ __launch_bounds__(256) void k_conv3_dw(ConvDwArgs a) {
-  constexpr int BO = 16 * MO;
-  __shared__ __attribute__((aligned(16))) bfbits Zs[2][BO][DWLD];
-  __shared__ __attribute__((aligned(16))) bfbits Xs[2][DWK][DWLD];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int k0 = blockIdx.x * DWK, o0 = blockIdx.y * BO;
-  const int K = 9 * a.C;
-  const long long pbeg = blockIdx.z * a.chunk;
-  const long long pend = min(a.P, pbeg + a.chunk);
-  // staging roles, by wave: waves 0 and 1 stage the dZ blocks (lane < 8 MO:
-  // pixel block lane % 4, channel group lane / 4) of even and odd steps,
-  // waves 2 and 3 the X blocks (pixel block, k group) of even and odd steps;
-  // a wave's loads are issued two steps before the step that multiplies
-  // them, with one register set per thread
-  const int par = w & 1;
-  const bool zst = w < 2 && lane < 8 * MO;
-  const bool xst = w >= 2;
-  const int sb = lane & 3, sg = lane >> 2;
-  bool gin = false;
-  int ci = 0, dy = 0, dx = 0;
-  if (zst) gin = o0 + 8 * sg < a.Cout;
-  if (xst) {
-    const int k = k0 + 8 * sg;
-    gin = k < K;
-    const int tap = gin ? k / a.C : 0;
-    ci = gin ? k - tap * a.C : 0;
-    dy = tap / 3 - 1;
-    dx = tap % 3 - 1;
-  }
-  uint4 r[8];
-  auto gload = [&](long long p0) {
-    const long long pb = p0 + 8 * sb;
-    if (zst) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        r[i] = load16_or_zero(gin && pb + i < pend, a.dZ + (pb + i) * a.Cout + o0 + 8 * sg);
-    } else if (xst) {
-      const unsigned q = (unsigned)(pb < pend ? pb : 0);
-      int x = (int)(q % (unsigned)a.Wd);
-      const unsigned rr = q / (unsigned)a.Wd;
-      int y = (int)(rr % (unsigned)a.H);
-      long long n = rr / (unsigned)a.H;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int yy = y + dy, xx = x + dx;
-        const bool ok = gin && pb + i < pend && yy >= 0 && yy < a.H && xx >= 0 && xx < a.Wd;
-        r[i] = load16_or_zero(ok, a.X + ((n * a.H + yy) * a.Wd + xx) * (long long)a.C + ci);
-        if (++x == a.Wd) {
-          x = 0;
-          if (++y == a.H) {
-            y = 0;
-            ++n;
-          }
-        }
-      }
-    }
-  };
-  auto sstore = [&](int buf) {
-    if (!zst && !xst) return;
-    uint4 o[8];
-    transpose8x8(r, o);
-    bfbits(*dst)[DWLD] = zst ? Zs[buf] : Xs[buf];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) *reinterpret_cast<uint4*>(&dst[8 * sg + j][8 * sb]) = o[j];
-  };
-  floatx4 acc[MO][2];
-  for (int i = 0; i < MO; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const int nsteps = (int)((pend - pbeg + DWP - 1) / DWP);
-  if (par < nsteps) gload(pbeg + (long long)par * DWP);
-  if (par == 0 && nsteps > 0) sstore(0);
-  __syncthreads();
-  const int fr = lane & 15, fk = (lane >> 4) * 8;
-  for (int st = 0; st < nsteps; ++st) {
-    const int buf = st & 1;
-    // this step's parity has stored step st: its registers take step st + 2;
-    // the other parity holds step st + 1 and stores it after the multiplies
-    if (par == buf && st + 2 < nsteps) gload(pbeg + (long long)(st + 2) * DWP);
-    bf16x8 za[MO], xb[2];
-    for (int i = 0; i < MO; ++i)
-      za[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Zs[buf][16 * i + fr][fk]));
-    for (int j = 0; j < 2; ++j)
-      xb[j] = __builtin_bit_cast(
-          bf16x8, *reinterpret_cast<const uint4*>(&Xs[buf][32 * w + 16 * j + fr][fk]));
-    for (int i = 0; i < MO; ++i)
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(za[i], xb[j], acc[i][j], 0, 0, 0);
-    if (par != buf && st + 1 < nsteps) sstore(buf ^ 1);
-    __syncthreads();
-  }
-  float* out = a.out + blockIdx.z * a.split_stride;
-  const int cc = lane & 15, cr = (lane >> 4) * 4;
-  for (int j = 0; j < 2; ++j) {
-    const int k = k0 + 32 * w + 16 * j + cc;
-    if (k >= K) continue;
-    for (int i = 0; i < MO; ++i)
-      for (int q = 0; q < 4; ++q) {
-        const int o = o0 + 16 * i + cr + q;
-        if (o < a.Cout) out[(long long)o * K + k] = acc[i][j][q];
-      }
-  }
-}
 
 // out[i] = sum_z part[z][i] in a fixed order for many splits: a block owns
 // 16 outputs; its 16 phases each add a contiguous run of splits in order,
@@ -749,108 +626,122 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ pa
 }
 
 // Weight gradient for inputs of 24 or 48 channels into up to 64 outputs,
-// image width a multiple of 8: one workgroup holds the whole dW tile (every
-// tap, channel and output channel), so dZ is read once, and the X staging
-// reads each input row segment once for all three horizontal taps: a task
-// (row offset dy, 8-channel group, 8-pixel block) loads the 10 pixels
-// x - 1 .. x + 8 of its row, takes the three 8-pixel windows of the three
-// horizontal taps from them in registers and writes each transposed (8 x 8)
-// into the k rows of its tap.  Steps of 32 pixels, LDS double-buffered with
-// the next step's loads issued before the multiplies; wave w owns the
-// 16-column k blocks w, w + 4, ... for every output block.
+// image width a multiple of 8, with gfx950's transposing LDS reads: the
+// operands are staged as they lie in memory -- per 32-pixel step, for each
+// row offset dy and 8-pixel block the 10 input pixels x - 1 .. x + 8 of its
+// row (all C channels), and the 32 pixels' dZ rows -- with plain 16-byte
+// copies, and each MFMA operand (8 consecutive pixels of one channel per
+// lane) is read with two ds_read_b64_tr_b16: in a 16-lane group, source lane
+// 4 r + q points at 4 channels (quad q) of pixel r, and output lane i gets
+// channel i of the four pixels.  For the X operand the quad's address holds
+// the tap: 16 k columns (tap, channel) of a block span up to two taps, but
+// a 4-channel quad never straddles one (C is a multiple of 8).  No register
+// transposes, no transposed LDS writes.  One workgroup holds the whole dW
+// tile (waves own the 16-column k blocks w, w + 4, ...); steps of 32 pixels,
+// LDS double-buffered with the next step's loads issued before the
+// multiplies.
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+__device__ inline v4s lds_tr16(const bfbits* p) {
+  return __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) v4s*)(p));
+}
+
+// 8 consecutive rows (pixels) of one column: rows p .. p + 3 at lo, p + 4 .. p + 7 at hi
+__device__ inline bf16x8 tr_operand(const bfbits* lo, const bfbits* hi) {
+  const v8s v = __builtin_shufflevector(lds_tr16(lo), lds_tr16(hi), 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
 template <int C8, int NB>
-__global__ __launch_bounds__(256) void k_conv3_dw_full(ConvDwArgs a) {
+__global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
   constexpr int C = 8 * C8;
   constexpr int K = 9 * C;
   constexpr int KB = (K + 15) / 16;          // 16-column k blocks
   constexpr int KBW = (KB + 3) / 4;          // per wave
   constexpr int BO = 16 * NB;
-  constexpr int NX = 12 * C8;                // X tasks: 3 rows x C8 groups x 4 blocks
-  constexpr int NZ = 4 * (BO / 8);           // dZ tasks (8-channel groups x 4 blocks)
-  static_assert(NX <= 128 && NZ <= 128, "staging tasks per half of the workgroup");
-  __shared__ __attribute__((aligned(16))) bfbits Xs[2][KB * 16][DWLD];
-  __shared__ __attribute__((aligned(16))) bfbits Zs[2][BO][DWLD];
+  constexpr int XR = C + 8;                  // staged pixel row (elements): 16 B of padding
+  constexpr int ZR = BO + 8;
+  constexpr int NXT = 3 * 4 * 10 * C8;       // X copies per step (16 B each)
+  constexpr int NZT = 32 * (BO / 8);         // dZ copies per step
+  constexpr int NXL = (NXT + 255) / 256, NZL = (NZT + 255) / 256;
+  __shared__ __attribute__((aligned(16))) bfbits Xs[2][3][4][10][XR];
+  __shared__ __attribute__((aligned(16))) bfbits Zs[2][32][ZR];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const long long pbeg = blockIdx.z * a.chunk;
   const long long pend = min(a.P, pbeg + a.chunk);
-  const uint4 z4 = make_uint4(0, 0, 0, 0);
-  // k rows past 9C are padding of the last block: zero in both buffers
-  for (int i = t; i < 2 * (KB * 16 - K) * (DWLD / 8); i += 256) {
-    const int b = i / ((KB * 16 - K) * (DWLD / 8)), r = i % ((KB * 16 - K) * (DWLD / 8));
-    *reinterpret_cast<uint4*>(&Xs[b][K + r / (DWLD / 8)][(r % (DWLD / 8)) * 8]) = z4;
-  }
-  const bool xst = t < NX;
-  const bool zst = t >= 128 && t - 128 < NZ;
-  const int sl = xst ? t : t - 128;
-  const int pb = sl & 3;
-  const int xg = (sl >> 2) % C8, xdy = (sl >> 2) / C8;   // X task: channel group, row offset
-  const int zg = sl >> 2;                                // dZ task: channel group
-  const bool zin = zst && 8 * zg < a.Cout;
-  uint4 r[10];
+  uint4 xr[NXL], zr[NZL];
   auto gload = [&](long long p0) {
-    const long long p = p0 + 8 * pb;   // the block's first pixel (8 pixels, one image row)
-    if (xst) {
-      const unsigned q = (unsigned)(p < pend ? p : 0);
-      const int x = (int)(q % (unsigned)a.Wd);
-      const unsigned rr = q / (unsigned)a.Wd;
-      const int yy = (int)(rr % (unsigned)a.H) + xdy - 1;
+#pragma unroll
+    for (int q = 0; q < NXL; ++q) {
+      const int i = t + 256 * q;
+      const int cg = i % C8, px = (i / C8) % 10, b = (i / (10 * C8)) % 4, dy = i / (40 * C8);
+      const long long p = p0 + 8 * b;   // the block's first pixel (one image row)
+      bool ok = i < NXT && p < pend;
+      const unsigned qq = (unsigned)(ok ? p : 0);
+      const int x = (int)(qq % (unsigned)a.Wd) - 1 + px;
+      const unsigned rr = qq / (unsigned)a.Wd;
+      const int y = (int)(rr % (unsigned)a.H) + dy - 1;
       const long long n = rr / (unsigned)a.H;
-      const bool rowok = p < pend && yy >= 0 && yy < a.H;
-      const bfbits* row = a.X + ((n * a.H + yy) * a.Wd) * (long long)C + 8 * xg;
+      ok = ok && y >= 0 && y < a.H && x >= 0 && x < a.Wd;
+      xr[q] = load16_or_zero(ok, a.X + ((n * a.H + (ok ? y : 0)) * a.Wd + (ok ? x : 0)) * (long long)C + 8 * cg);
+    }
 #pragma unroll
-      for (int i = 0; i < 10; ++i) {
-        const int xx = x - 1 + i;
-        r[i] = load16_or_zero(rowok && xx >= 0 && xx < a.Wd, row + (long long)xx * C);
-      }
-    } else if (zst) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        r[i] = load16_or_zero(zin && p + i < pend, a.dZ + (p + i) * a.Cout + 8 * zg);
+    for (int q = 0; q < NZL; ++q) {
+      const int i = t + 256 * q;
+      const int og = i % (BO / 8), pi = i / (BO / 8);
+      const bool ok = i < NZT && p0 + pi < pend && 8 * og < a.Cout;
+      zr[q] = load16_or_zero(ok, a.dZ + (ok ? (p0 + pi) * a.Cout + 8 * og : 0));
     }
   };
   auto sstore = [&](int buf) {
-    if (xst) {
 #pragma unroll
-      for (int dx = 0; dx < 3; ++dx) {
-        uint4 in[8], o[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) in[i] = r[dx + i];
-        transpose8x8(in, o);
-        const int k0 = (3 * xdy + dx) * C + 8 * xg;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) *reinterpret_cast<uint4*>(&Xs[buf][k0 + j][8 * pb]) = o[j];
+    for (int q = 0; q < NXL; ++q) {
+      const int i = t + 256 * q;
+      if (i < NXT) {
+        const int cg = i % C8, px = (i / C8) % 10, b = (i / (10 * C8)) % 4, dy = i / (40 * C8);
+        *reinterpret_cast<uint4*>(&Xs[buf][dy][b][px][8 * cg]) = xr[q];
       }
-    } else if (zst) {
-      uint4 in[8], o[8];
+    }
 #pragma unroll
-      for (int i = 0; i < 8; ++i) in[i] = r[i];
-      transpose8x8(in, o);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) *reinterpret_cast<uint4*>(&Zs[buf][8 * zg + j][8 * pb]) = o[j];
+    for (int q = 0; q < NZL; ++q) {
+      const int i = t + 256 * q;
+      if (i < NZT) *reinterpret_cast<uint4*>(&Zs[buf][i / (BO / 8)][8 * (i % (BO / 8))]) = zr[q];
     }
   };
   floatx4 acc[KBW][NB];
   for (int i = 0; i < KBW; ++i)
     for (int j = 0; j < NB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // this lane's source position in its 16-lane group: pixel r of the
+  // group's 8-pixel block (g = block), 4-column quad qd
+  const int g = lane >> 4, r = (lane & 15) >> 2, qd = lane & 3;
+  // per owned k block: the staged (dy, pixel shift, channel) of this lane's
+  // quad (columns past 9 C read tap 8, channel 0: their outputs are dropped)
+  int xoff[KBW];
+  for (int i = 0; i < KBW; ++i) {
+    int col = 16 * (w + 4 * i) + 4 * qd;
+    if (col >= K) col = 8 * C;
+    const int tap = col / C, c = col - tap * C;
+    xoff[i] = ((((tap / 3) * 4 + g) * 10) + r + tap % 3) * XR + c;   // dy index, block g, px = r + dx + 1
+  }
   const int nsteps = (int)((pend - pbeg + DWP - 1) / DWP);
   if (nsteps > 0) {
     gload(pbeg);
     sstore(0);
   }
   __syncthreads();
-  const int fr = lane & 15, fk = (lane >> 4) * 8;
   for (int st = 0; st < nsteps; ++st) {
     const int buf = st & 1;
     const bool more = st + 1 < nsteps;
     if (more) gload(pbeg + (long long)(st + 1) * DWP);
+    const bfbits* xb = &Xs[buf][0][0][0][0];
     bf16x8 zf[NB];
     for (int j = 0; j < NB; ++j)
-      zf[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Zs[buf][16 * j + fr][fk]));
+      zf[j] = tr_operand(&Zs[buf][8 * g + r][16 * j + 4 * qd], &Zs[buf][8 * g + 4 + r][16 * j + 4 * qd]);
     for (int i = 0; i < KBW; ++i) {
       const int kb = w + 4 * i;
       if (kb >= KB) break;
-      const bf16x8 xf =
-          __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Xs[buf][16 * kb + fr][fk]));
+      const bf16x8 xf = tr_operand(xb + xoff[i], xb + xoff[i] + 4 * XR);
       for (int j = 0; j < NB; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(zf[j], xf, acc[i][j], 0, 0, 0);
     }
@@ -865,6 +756,110 @@ __global__ __launch_bounds__(256) void k_conv3_dw_full(ConvDwArgs a) {
     for (int j = 0; j < NB; ++j)
       for (int q = 0; q < 4; ++q) {
         const int o = 16 * j + cr + q;
+        if (o < a.Cout) out[(long long)o * K + k] = acc[i][j][q];
+      }
+  }
+}
+
+// The k-tiled weight gradient (tile: 16 MO output channels x 128 k columns,
+// four waves of 16 MO x 32) with the same transposing LDS reads: per 32-pixel
+// step the tile's 16 column octets (tap, 8 channels) are staged pixel-major
+// -- octet o holds the 32 pixels' 8 channels at its tap, one 16-byte copy
+// per (pixel, octet), consecutive threads taking consecutive octets of one
+// pixel -- and the dZ rows likewise; an operand lane group reads its 16
+// columns as 4 quads of the two octets of its k block.
+template <int MO>
+__global__ __launch_bounds__(256) void k_conv3_dwk_tr(ConvDwArgs a) {
+  constexpr int BO = 16 * MO;
+  constexpr int XO = 32 * 8 + 32;            // one octet's 32 pixels (+ 64 B between octets)
+  constexpr int ZR = BO + 8;
+  constexpr int NXT = 16 * 32, NZT = 32 * (BO / 8);
+  constexpr int NXL = NXT / 256, NZL = (NZT + 255) / 256;
+  __shared__ __attribute__((aligned(16))) bfbits Xs[2][16][XO];
+  __shared__ __attribute__((aligned(16))) bfbits Zs[2][32][ZR];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int k0 = blockIdx.x * DWK, o0 = blockIdx.y * BO;
+  const int K = 9 * a.C;
+  const long long pbeg = blockIdx.z * a.chunk;
+  const long long pend = min(a.P, pbeg + a.chunk);
+  // this thread's X octets (i % 16 for i = t + 256 q: the same for every q)
+  // and their taps
+  const int oc = t & 15;
+  const int kx = k0 + 8 * oc;
+  const bool kin = kx < K;
+  const int tap = kin ? kx / a.C : 0;
+  const int ci = kin ? kx - tap * a.C : 0;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  uint4 xr[NXL], zr[NZL];
+  auto gload = [&](long long p0) {
+#pragma unroll
+    for (int q = 0; q < NXL; ++q) {
+      const int px = (t + 256 * q) >> 4;
+      const long long p = p0 + px;
+      bool ok = kin && p < pend;
+      const unsigned qq = (unsigned)(ok ? p : 0);
+      const int x = (int)(qq % (unsigned)a.Wd) + dx;
+      const unsigned rr = qq / (unsigned)a.Wd;
+      const int y = (int)(rr % (unsigned)a.H) + dy;
+      const long long n = rr / (unsigned)a.H;
+      ok = ok && y >= 0 && y < a.H && x >= 0 && x < a.Wd;
+      xr[q] = load16_or_zero(
+          ok, a.X + ((n * a.H + (ok ? y : 0)) * a.Wd + (ok ? x : 0)) * (long long)a.C + ci);
+    }
+#pragma unroll
+    for (int q = 0; q < NZL; ++q) {
+      const int i = t + 256 * q;
+      const int og = i % (BO / 8), pi = i / (BO / 8);
+      const bool ok = i < NZT && p0 + pi < pend && o0 + 8 * og < a.Cout;
+      zr[q] = load16_or_zero(ok, a.dZ + (ok ? (p0 + pi) * a.Cout + o0 + 8 * og : 0));
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < NXL; ++q)
+      *reinterpret_cast<uint4*>(&Xs[buf][oc][((t + 256 * q) >> 4) * 8]) = xr[q];
+#pragma unroll
+    for (int q = 0; q < NZL; ++q) {
+      const int i = t + 256 * q;
+      if (i < NZT) *reinterpret_cast<uint4*>(&Zs[buf][i / (BO / 8)][8 * (i % (BO / 8))]) = zr[q];
+    }
+  };
+  floatx4 acc[MO][2];
+  for (int i = 0; i < MO; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int g = lane >> 4, r = (lane & 15) >> 2, qd = lane & 3;
+  const int nsteps = (int)((pend - pbeg + DWP - 1) / DWP);
+  if (nsteps > 0) {
+    gload(pbeg);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int st = 0; st < nsteps; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nsteps;
+    if (more) gload(pbeg + (long long)(st + 1) * DWP);
+    bf16x8 za[MO], xb[2];
+    for (int i = 0; i < MO; ++i)
+      za[i] = tr_operand(&Zs[buf][8 * g + r][16 * i + 4 * qd], &Zs[buf][8 * g + 4 + r][16 * i + 4 * qd]);
+    for (int j = 0; j < 2; ++j) {
+      // k block 2 w + j: octets 4 w + 2 j and 4 w + 2 j + 1
+      const bfbits* xo = &Xs[buf][4 * w + 2 * j + (qd >> 1)][0];
+      xb[j] = tr_operand(xo + (8 * g + r) * 8 + 4 * (qd & 1), xo + (8 * g + 4 + r) * 8 + 4 * (qd & 1));
+    }
+    for (int i = 0; i < MO; ++i)
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(za[i], xb[j], acc[i][j], 0, 0, 0);
+    if (more) sstore(buf ^ 1);
+    __syncthreads();
+  }
+  float* out = a.out + blockIdx.z * a.split_stride;
+  const int cc = lane & 15, cr = (lane >> 4) * 4;
+  for (int j = 0; j < 2; ++j) {
+    const int k = k0 + 32 * w + 16 * j + cc;
+    if (k >= K) continue;
+    for (int i = 0; i < MO; ++i)
+      for (int q = 0; q < 4; ++q) {
+        const int o = o0 + 16 * i + cr + q;
         if (o < a.Cout) out[(long long)o * K + k] = acc[i][j][q];
       }
   }
@@ -1047,10 +1042,10 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
     const dim3 g1(1, 1, (unsigned)ns);
 #define BH_DWF(C8V)                                                                            \
   do {                                                                                        \
-    if (mo == 1) hipLaunchKernelGGL((k_conv3_dw_full<C8V, 1>), g1, dim3(256), 0, st, a);       \
-    else if (mo == 2) hipLaunchKernelGGL((k_conv3_dw_full<C8V, 2>), g1, dim3(256), 0, st, a);  \
-    else if (mo == 3) hipLaunchKernelGGL((k_conv3_dw_full<C8V, 3>), g1, dim3(256), 0, st, a);  \
-    else hipLaunchKernelGGL((k_conv3_dw_full<C8V, 4>), g1, dim3(256), 0, st, a);               \
+    if (mo == 1) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 1>), g1, dim3(256), 0, st, a);         \
+    else if (mo == 2) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 2>), g1, dim3(256), 0, st, a);    \
+    else if (mo == 3) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 3>), g1, dim3(256), 0, st, a);    \
+    else hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 4>), g1, dim3(256), 0, st, a);                 \
   } while (0)
     if (C == 24) BH_DWF(3);
     else BH_DWF(6);
@@ -1058,11 +1053,11 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
   } else {
   const dim3 grid((unsigned)((9 * C + DWK - 1) / DWK), (unsigned)((Cout + 16 * mo - 1) / (16 * mo)),
                   (unsigned)ns);
-  if (mo == 1) hipLaunchKernelGGL(k_conv3_dw<1>, grid, dim3(256), 0, st, a);
-  else if (mo == 2) hipLaunchKernelGGL(k_conv3_dw<2>, grid, dim3(256), 0, st, a);
-  else if (mo == 3) hipLaunchKernelGGL(k_conv3_dw<3>, grid, dim3(256), 0, st, a);
-  else if (mo == 4) hipLaunchKernelGGL(k_conv3_dw<4>, grid, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(k_conv3_dw<6>, grid, dim3(256), 0, st, a);
+  if (mo == 1) hipLaunchKernelGGL(k_conv3_dwk_tr<1>, grid, dim3(256), 0, st, a);
+  else if (mo == 2) hipLaunchKernelGGL(k_conv3_dwk_tr<2>, grid, dim3(256), 0, st, a);
+  else if (mo == 3) hipLaunchKernelGGL(k_conv3_dwk_tr<3>, grid, dim3(256), 0, st, a);
+  else if (mo == 4) hipLaunchKernelGGL(k_conv3_dwk_tr<4>, grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_conv3_dwk_tr<6>, grid, dim3(256), 0, st, a);
   }
   BH_LAUNCH_CHECK();
   if (ns > 1) {
