@@ -347,7 +347,10 @@ __global__ __launch_bounds__(256 * KS) void k_conv3_igemm(ConvArgs a) {
   constexpr int BLD = (BNc * 4 + 255) / 256;   // weight segments per thread
   constexpr int AS_E = KS * 2 * CM * CLD, BS_E = KS * 2 * BNc * CLD;
   constexpr int RED_B = KS > 1 ? (KS - 1) * 4 * 2 * NB * 64 * 16 : 0;   // other groups' tiles
-  constexpr int SM_B = (AS_E + BS_E) * 2 > RED_B ? (AS_E + BS_E) * 2 : RED_B;
+  constexpr int OR = BNc + 8;                                            // staged output row
+  constexpr int OUT_B = CM * OR * 2;
+  constexpr int SM_B0 = (AS_E + BS_E) * 2 > RED_B ? (AS_E + BS_E) * 2 : RED_B;
+  constexpr int SM_B = SM_B0 > OUT_B ? SM_B0 : OUT_B;
   __shared__ __attribute__((aligned(16))) unsigned char smem[SM_B];
   auto As = reinterpret_cast<bfbits(*)[2][CM][CLD]>(smem);
   auto Bs = reinterpret_cast<bfbits(*)[2][BNc][CLD]>(smem + AS_E * 2);
@@ -433,18 +436,46 @@ __global__ __launch_bounds__(256 * KS) void k_conv3_igemm(ConvArgs a) {
           red[((((g - 1) * 4 + w) * 2 + i) * NB + j) * 64 + lane] = make_float4(v[0], v[1], v[2], v[3]);
         }
     __syncthreads();
-    if (g > 0) return;
-    for (int q = 1; q < KS; ++q)
-      for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < NB; ++j) {
-          const float4 v = red[((((q - 1) * 4 + w) * 2 + i) * NB + j) * 64 + lane];
-          acc[i][j][0] += v.x;
-          acc[i][j][1] += v.y;
-          acc[i][j][2] += v.z;
-          acc[i][j][3] += v.w;
-        }
+    if (g == 0)
+      for (int q = 1; q < KS; ++q)
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < NB; ++j) {
+            const float4 v = red[((((q - 1) * 4 + w) * 2 + i) * NB + j) * 64 + lane];
+            acc[i][j][0] += v.x;
+            acc[i][j][1] += v.y;
+            acc[i][j][2] += v.z;
+            acc[i][j][3] += v.w;
+          }
+    if (a.out_bf16) __syncthreads();   // the tiles are read before the output is staged
+    else if (g > 0) return;
   }
   const int cc = lane & 15, cr = (lane >> 4) * 4;
+  if (a.out_bf16) {
+    // the bf16 output tile (128 pixels x BNc channels) through LDS, so that
+    // it leaves in 16-byte stores of 8 channels (by every thread of the
+    // workgroup) instead of one 2-byte store per value
+    bfbits* so = reinterpret_cast<bfbits*>(smem);
+    if (g == 0)
+      for (int j = 0; j < NB; ++j) {
+        const int n = n0 + j * 16 + cc;
+        const float bv = (a.bias && n < a.Cout) ? a.bias[n] : 0.f;
+        for (int i = 0; i < 2; ++i)
+          for (int r = 0; r < 4; ++r) {
+            float v = a.alpha * acc[i][j][r] + bv;
+            if (a.relu) v = fmaxf(v, 0.f);
+            so[(w * 32 + i * 16 + cr + r) * OR + j * 16 + cc] = f2bf(v);
+          }
+      }
+    __syncthreads();
+    const int nv8 = (min(BNc, a.Cout - n0)) >> 3;
+    for (int e = threadIdx.x; e < CM * (BNc / 8); e += 256 * KS) {
+      const int p = e / (BNc / 8), c8 = e - p * (BNc / 8);
+      if (c8 >= nv8 || p0 + p >= a.P) continue;
+      *reinterpret_cast<uint4*>((bfbits*)a.Y + (p0 + p) * a.Cout + n0 + 8 * c8) =
+          *reinterpret_cast<const uint4*>(&so[p * OR + 8 * c8]);
+    }
+    return;
+  }
   for (int j = 0; j < NB; ++j) {
     const int n = n0 + j * 16 + cc;
     if (n >= a.Cout) continue;
@@ -455,8 +486,7 @@ __global__ __launch_bounds__(256 * KS) void k_conv3_igemm(ConvArgs a) {
         if (p >= a.P) continue;
         float v = a.alpha * acc[i][j][r] + bv;
         if (a.relu) v = fmaxf(v, 0.f);
-        if (a.out_bf16) ((bfbits*)a.Y)[p * a.Cout + n] = f2bf(v);
-        else ((float*)a.Y)[p * a.Cout + n] = v;
+        ((float*)a.Y)[p * a.Cout + n] = v;
       }
   }
 }
@@ -485,8 +515,15 @@ __global__ __launch_bounds__(256) void k_conv3_halo(ConvArgs a, int tiles_x, int
   constexpr int NG = 9 * C8;                 // k groups of 8
   constexpr int NSTEP = (NG + 3) / 4;
   constexpr int BLD = (BNc * 4 + 255) / 256;
-  __shared__ __attribute__((aligned(16))) bfbits halo[HH * HW * CS];
-  __shared__ __attribute__((aligned(16))) bfbits Bs[2][BNc][CLD];
+  // the halo and the weight tiles, and after the k steps the bf16 output
+  // tile (TH x 32 pixels x BNc channels, rows padded by 8 channels), so that
+  // it leaves in 16-byte stores of 8 channels instead of one 2-byte store
+  // per value
+  constexpr int HALO_E = HH * HW * CS, BS_E = 2 * BNc * CLD, OR = BNc + 8;
+  constexpr int SM_E = HALO_E + BS_E > TH * HTW * OR ? HALO_E + BS_E : TH * HTW * OR;
+  __shared__ __attribute__((aligned(16))) bfbits smem[SM_E];
+  bfbits* const halo = smem;
+  auto Bs = reinterpret_cast<bfbits(*)[BNc][CLD]>(smem + HALO_E);
   __shared__ uint4 zlds;   // k groups past 9 C read these zeros
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) zlds = make_uint4(0, 0, 0, 0);
@@ -565,6 +602,29 @@ __global__ __launch_bounds__(256) void k_conv3_halo(ConvArgs a, int tiles_x, int
     __syncthreads();
   }
   const int cc = lane & 15, cr = (lane >> 4) * 4;
+  if (a.out_bf16) {
+    // (the loop's last barrier: no wave still reads the halo or weights)
+    for (int j = 0; j < NB; ++j) {
+      const int n = n0 + j * 16 + cc;
+      const float bvv = (a.bias && n < a.Cout) ? a.bias[n] : 0.f;
+      for (int i = 0; i < MB; ++i)
+        for (int r = 0; r < 4; ++r) {
+          float v = a.alpha * acc[i][j][r] + bvv;
+          if (a.relu) v = fmaxf(v, 0.f);
+          smem[(w * (MB * 16) + i * 16 + cr + r) * OR + j * 16 + cc] = f2bf(v);
+        }
+    }
+    __syncthreads();
+    const int nv8 = (min(BNc, a.Cout - n0)) >> 3;   // 8-channel chunks of this tile
+    for (int e = t; e < TH * HTW * (BNc / 8); e += 256) {
+      const int p = e / (BNc / 8), c8 = e - p * (BNc / 8);
+      const int yy = y0 + p / HTW, xx = x0 + p % HTW;
+      if (c8 >= nv8 || yy >= a.H || xx >= a.Wd) continue;
+      *reinterpret_cast<uint4*>((bfbits*)a.Y + ((img * a.H + yy) * a.Wd + xx) * a.Cout + n0 + 8 * c8) =
+          *reinterpret_cast<const uint4*>(&smem[p * OR + 8 * c8]);
+    }
+    return;
+  }
   for (int j = 0; j < NB; ++j) {
     const int n = n0 + j * 16 + cc;
     if (n >= a.Cout) continue;
@@ -577,8 +637,7 @@ __global__ __launch_bounds__(256) void k_conv3_halo(ConvArgs a, int tiles_x, int
         const long long o = ((img * a.H + yy) * a.Wd + xx) * a.Cout + n;
         float v = a.alpha * acc[i][j][r] + bvv;
         if (a.relu) v = fmaxf(v, 0.f);
-        if (a.out_bf16) ((bfbits*)a.Y)[o] = f2bf(v);
-        else ((float*)a.Y)[o] = v;
+        ((float*)a.Y)[o] = v;
       }
   }
 }
