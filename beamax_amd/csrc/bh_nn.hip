@@ -656,7 +656,8 @@ struct ConvDwArgs {
   float* out;
   long long P;
   int H, Wd, C, Cout;
-  long long chunk;          // pixels per split, a multiple of DWP
+  long long chunk;          // per split: pixels (a multiple of DWP), or 4x8 tiles
+  long long ntiles;         // 4x8-pixel tiles (the whole-tile kernel)
   long long split_stride;   // floats between split partials
 };
 
@@ -685,10 +686,11 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ pa
 }
 
 // Weight gradient for inputs of 24 or 48 channels into up to 64 outputs,
-// image width a multiple of 8, with gfx950's transposing LDS reads: the
-// operands are staged as they lie in memory -- per 32-pixel step, for each
-// row offset dy and 8-pixel block the 10 input pixels x - 1 .. x + 8 of its
-// row (all C channels), and the 32 pixels' dZ rows -- with plain 16-byte
+// image width a multiple of 8, with gfx950's transposing LDS reads.  A step
+// is a tile of 4 image rows x 8 pixels; the operands are staged as they lie
+// in memory -- the tile's 6 x 10 input pixels (rows y - 1 .. y + 4, pixels
+// x - 1 .. x + 8, all C channels: 1.9 staged pixels per pixel, 3.75 with
+// one-row steps of 32 pixels) and its 32 dZ rows -- with plain 16-byte
 // copies, and each MFMA operand (8 consecutive pixels of one channel per
 // lane) is read with two ds_read_b64_tr_b16: in a 16-lane group, source lane
 // 4 r + q points at 4 channels (quad q) of pixel r, and output lane i gets
@@ -696,9 +698,9 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ pa
 // the tap: 16 k columns (tap, channel) of a block span up to two taps, but
 // a 4-channel quad never straddles one (C is a multiple of 8).  No register
 // transposes, no transposed LDS writes.  One workgroup holds the whole dW
-// tile (waves own the 16-column k blocks w, w + 4, ...); steps of 32 pixels,
-// LDS double-buffered with the next step's loads issued before the
-// multiplies.
+// tile (waves own the 16-column k blocks w, w + 4, ...); grid z splits the
+// tiles (image-major, then row group, then 8-pixel column) into chunks; LDS
+// double-buffered with later steps' loads issued before the multiplies.
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef short v8s __attribute__((ext_vector_type(8)));
 
@@ -721,14 +723,15 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
   constexpr int BO = 16 * NB;
   constexpr int XR = C + 8;                  // staged pixel row (elements): 16 B of padding
   constexpr int ZR = BO + 8;
-  constexpr int NXT = 3 * 4 * 10 * C8;       // X copies per step (16 B each)
+  constexpr int NXT = 6 * 10 * C8;           // X copies per step (16 B each)
   constexpr int NZT = 32 * (BO / 8);         // dZ copies per step
   constexpr int NXL = (NXT + 255) / 256, NZL = (NZT + 255) / 256;
-  __shared__ __attribute__((aligned(16))) bfbits Xs[2][3][4][10][XR];
+  __shared__ __attribute__((aligned(16))) bfbits Xs[2][6][10][XR];
   __shared__ __attribute__((aligned(16))) bfbits Zs[2][32][ZR];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const long long pbeg = blockIdx.z * a.chunk;
-  const long long pend = min(a.P, pbeg + a.chunk);
+  const long long tbeg = blockIdx.z * a.chunk;
+  const long long tend = min(a.ntiles, tbeg + a.chunk);
+  const unsigned WB = (unsigned)a.Wd >> 3, HT = ((unsigned)a.H + 3) >> 2;
   // 24 input channels: two register sets, the loads of step st + 2 issued
   // while step st multiplies (every load is unconditional -- past the chunk
   // it reads the zero block -- so the waits before the LDS stores count
@@ -736,27 +739,27 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
   // one step ahead (the second set's registers cost more than it hides)
   constexpr bool AHEAD2 = C8 == 3;
   uint4 xr0[NXL], zr0[NZL], xr1[AHEAD2 ? NXL : 1], zr1[AHEAD2 ? NZL : 1];
-  auto gload = [&](long long p0, auto& xr, auto& zr) {
+  auto gload = [&](long long tile, auto& xr, auto& zr) {
+    const bool live = tile < tend;
+    const unsigned tq = (unsigned)(live ? tile : 0);
+    const unsigned rest = tq / WB;
+    const int x0 = 8 * (int)(tq - rest * WB), y0 = 4 * (int)(rest % HT);
+    const long long img = (long long)(rest / HT) * a.H;
 #pragma unroll
     for (int q = 0; q < NXL; ++q) {
       const int i = t + 256 * q;
-      const int cg = i % C8, px = (i / C8) % 10, b = (i / (10 * C8)) % 4, dy = i / (40 * C8);
-      const long long p = p0 + 8 * b;   // the block's first pixel (one image row)
-      bool ok = i < NXT && p < pend;
-      const unsigned qq = (unsigned)(ok ? p : 0);
-      const int x = (int)(qq % (unsigned)a.Wd) - 1 + px;
-      const unsigned rr = qq / (unsigned)a.Wd;
-      const int y = (int)(rr % (unsigned)a.H) + dy - 1;
-      const long long n = rr / (unsigned)a.H;
-      ok = ok && y >= 0 && y < a.H && x >= 0 && x < a.Wd;
-      xr[q] = load16_or_zero(ok, a.X + ((n * a.H + (ok ? y : 0)) * a.Wd + (ok ? x : 0)) * (long long)C + 8 * cg);
+      const int cg = i % C8, px = (i / C8) % 10, sr = i / (10 * C8);
+      const int y = y0 - 1 + sr, x = x0 - 1 + px;
+      const bool ok = live && i < NXT && y >= 0 && y < a.H && x >= 0 && x < a.Wd;
+      xr[q] = load16_or_zero(ok, a.X + ((img + (ok ? y : 0)) * a.Wd + (ok ? x : 0)) * (long long)C + 8 * cg);
     }
 #pragma unroll
     for (int q = 0; q < NZL; ++q) {
       const int i = t + 256 * q;
       const int og = i % (BO / 8), pi = i / (BO / 8);
-      const bool ok = i < NZT && p0 + pi < pend && 8 * og < a.Cout;
-      zr[q] = load16_or_zero(ok, a.dZ + (ok ? (p0 + pi) * a.Cout + 8 * og : 0));
+      const int y = y0 + (pi >> 3), x = x0 + (pi & 7);
+      const bool ok = live && i < NZT && y < a.H && 8 * og < a.Cout;
+      zr[q] = load16_or_zero(ok, a.dZ + (ok ? ((img + y) * a.Wd + x) * a.Cout + 8 * og : 0));
     }
   };
   auto sstore = [&](int buf, const auto& xr, const auto& zr) {
@@ -764,8 +767,8 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
     for (int q = 0; q < NXL; ++q) {
       const int i = t + 256 * q;
       if (i < NXT) {
-        const int cg = i % C8, px = (i / C8) % 10, b = (i / (10 * C8)) % 4, dy = i / (40 * C8);
-        *reinterpret_cast<uint4*>(&Xs[buf][dy][b][px][8 * cg]) = xr[q];
+        const int cg = i % C8, px = (i / C8) % 10, sr = i / (10 * C8);
+        *reinterpret_cast<uint4*>(&Xs[buf][sr][px][8 * cg]) = xr[q];
       }
     }
 #pragma unroll
@@ -778,7 +781,7 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
   for (int i = 0; i < KBW; ++i)
     for (int j = 0; j < NB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   // this lane's source position in its 16-lane group: pixel r of the
-  // group's 8-pixel block (g = block), 4-column quad qd
+  // group's 8-pixel block (g = tile row), 4-column quad qd
   const int g = lane >> 4, r = (lane & 15) >> 2, qd = lane & 3;
   // per owned k block: the staged (dy, pixel shift, channel) of this lane's
   // quad (columns past 9 C read tap 8, channel 0: their outputs are dropped)
@@ -787,11 +790,11 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
     int col = 16 * (w + 4 * i) + 4 * qd;
     if (col >= K) col = 8 * C;
     const int tap = col / C, c = col - tap * C;
-    xoff[i] = ((((tap / 3) * 4 + g) * 10) + r + tap % 3) * XR + c;   // dy index, block g, px = r + dx + 1
+    xoff[i] = ((g + tap / 3) * 10 + r + tap % 3) * XR + c;   // staged row g + dy + 1, px = r + dx + 1
   }
-  const int nsteps = (int)((pend - pbeg + DWP - 1) / DWP);
+  const int nsteps = (int)(tend - tbeg);
   auto mul = [&](int buf) {
-    const bfbits* xb = &Xs[buf][0][0][0][0];
+    const bfbits* xb = &Xs[buf][0][0][0];
     bf16x8 zf[NB];
     for (int j = 0; j < NB; ++j)
       zf[j] = tr_operand(&Zs[buf][8 * g + r][16 * j + 4 * qd], &Zs[buf][8 * g + 4 + r][16 * j + 4 * qd]);
@@ -805,34 +808,34 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
   };
   if constexpr (AHEAD2) {
     if (nsteps > 0) {
-      gload(pbeg, xr0, zr0);
-      gload(pbeg + DWP, xr1, zr1);
+      gload(tbeg, xr0, zr0);
+      gload(tbeg + 1, xr1, zr1);
       sstore(0, xr0, zr0);
     }
     __syncthreads();
     for (int st = 0; st < nsteps; st += 2) {
       // step st in buffer 0 (set 0 free, set 1 holds step st + 1)
-      gload(pbeg + (long long)(st + 2) * DWP, xr0, zr0);
+      gload(tbeg + st + 2, xr0, zr0);
       mul(0);
       sstore(1, xr1, zr1);
       __syncthreads();
       if (st + 1 >= nsteps) break;
       // step st + 1 in buffer 1 (set 0 holds step st + 2)
-      gload(pbeg + (long long)(st + 3) * DWP, xr1, zr1);
+      gload(tbeg + st + 3, xr1, zr1);
       mul(1);
       sstore(0, xr0, zr0);
       __syncthreads();
     }
   } else {
     if (nsteps > 0) {
-      gload(pbeg, xr0, zr0);
+      gload(tbeg, xr0, zr0);
       sstore(0, xr0, zr0);
     }
     __syncthreads();
     for (int st = 0; st < nsteps; ++st) {
       const int buf = st & 1;
       const bool more = st + 1 < nsteps;
-      if (more) gload(pbeg + (long long)(st + 1) * DWP, xr0, zr0);
+      if (more) gload(tbeg + st + 1, xr0, zr0);
       mul(buf);
       if (more) sstore(buf ^ 1, xr0, zr0);
       __syncthreads();
@@ -1071,12 +1074,23 @@ static bool conv_dw_whole(int W, int C, int Cout) {
 // bound by the latency of its staging loads, so it needs several resident
 // workgroups per CU; 512 of the whole-tile kernel, which does more per
 // step), chunks of at least 512 pixels (shorter chunks spend more on their
-// partial tiles than they gain in parallelism), a multiple of 32.  Measured
+// partial tiles than they gain in parallelism), a multiple of 32 (of the
+// whole-tile kernel: whole 4x8 tiles, *ntiles of them in all).  Measured
 // on the U-Net's twelve layer shapes at batch 16 against 2048 workgroups
 // and 256-pixel chunks: 1150 -> 1040 us of weight gradients in all.
-static void conv_dw_split(long long P, int W, int C, int Cout, long long* chunk, int* nsplit) {
-  const bool whole = conv_dw_whole(W, C, Cout);
-  const long long tiles = whole ? 2 : (long long)((9 * C + DWK - 1) / DWK) *
+static void conv_dw_split(long long B, int H, int W, int C, int Cout, long long* chunk,
+                          int* nsplit, long long* ntiles) {
+  if (conv_dw_whole(W, C, Cout)) {
+    const long long nt = B * ((H + 3) / 4) * (W / 8);
+    const long long ns = std::max(1LL, std::min(512LL, (nt + 15) / 16));
+    const long long ch = (nt + ns - 1) / ns;
+    *chunk = ch;
+    *nsplit = (int)((nt + ch - 1) / ch);
+    *ntiles = nt;
+    return;
+  }
+  const long long P = B * (long long)H * W;
+  const long long tiles = (long long)((9 * C + DWK - 1) / DWK) *
                           ((Cout + 16 * conv_tile(Cout) - 1) / (16 * conv_tile(Cout)));
   long long ns = (1024 + tiles - 1) / tiles;
   ns = std::max(1LL, std::min(ns, (P + 511) / 512));
@@ -1084,13 +1098,14 @@ static void conv_dw_split(long long P, int W, int C, int Cout, long long* chunk,
   ch = (ch + DWP - 1) / DWP * DWP;
   *chunk = ch;
   *nsplit = (int)((P + ch - 1) / ch);
+  *ntiles = 0;
 }
 
 extern "C" size_t bh_nn_conv3_dw_workspace_size(int64_t B, int H, int W, int C, int Cout) {
   if (B <= 0 || H <= 0 || W <= 0 || C <= 0 || Cout <= 0) return 0;
-  long long chunk;
+  long long chunk, nt;
   int ns;
-  conv_dw_split(B * (long long)H * W, W, C, Cout, &chunk, &ns);
+  conv_dw_split(B, H, W, C, Cout, &chunk, &ns, &nt);
   return ns > 1 ? (size_t)ns * Cout * 9 * C * sizeof(float) : 0;
 }
 
@@ -1115,7 +1130,7 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
   a.C = C;
   a.Cout = Cout;
   int ns;
-  conv_dw_split(P, W, C, Cout, &a.chunk, &ns);
+  conv_dw_split(B, H, W, C, Cout, &a.chunk, &ns, &a.ntiles);
   const long long n = (long long)Cout * 9 * C;
   a.split_stride = n;
   if (ns > 1) {
