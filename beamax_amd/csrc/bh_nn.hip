@@ -685,8 +685,9 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ pa
   }
 }
 
-// Weight gradient for inputs of 24 or 48 channels into up to 64 outputs,
-// image width a multiple of 8, with gfx950's transposing LDS reads.  A step
+// Weight gradient of a slice of 24 or 48 input channels (grid x) into a tile
+// of up to 64 output channels (grid y), image width a multiple of 8, with
+// gfx950's transposing LDS reads.  A step
 // is a tile of 4 image rows x 8 pixels; the operands are staged as they lie
 // in memory -- the tile's 6 x 10 input pixels (rows y - 1 .. y + 4, pixels
 // x - 1 .. x + 8, all C channels: 1.9 staged pixels per pixel, 3.75 with
@@ -698,7 +699,8 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ pa
 // the tap: 16 k columns (tap, channel) of a block span up to two taps, but
 // a 4-channel quad never straddles one (C is a multiple of 8).  No register
 // transposes, no transposed LDS writes.  One workgroup holds the whole dW
-// tile (waves own the 16-column k blocks w, w + 4, ...); grid z splits the
+// tile of its slice (9 taps x C8 channel octets, as k columns tap * C + c;
+// waves own the 16-column k blocks w, w + 4, ...); grid z splits the
 // tiles (image-major, then row group, then 8-pixel column) into chunks; LDS
 // double-buffered with later steps' loads issued before the multiplies.
 typedef short v4s __attribute__((ext_vector_type(4)));
@@ -731,6 +733,9 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const long long tbeg = blockIdx.z * a.chunk;
   const long long tend = min(a.ntiles, tbeg + a.chunk);
+  const int c0 = blockIdx.x * C, o0 = blockIdx.y * BO;
+  const bfbits* Xc = a.X + c0;
+  const bfbits* dZo = a.dZ + o0;
   const unsigned WB = (unsigned)a.Wd >> 3, HT = ((unsigned)a.H + 3) >> 2;
   // 24 input channels: two register sets, the loads of step st + 2 issued
   // while step st multiplies (every load is unconditional -- past the chunk
@@ -751,15 +756,15 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
       const int cg = i % C8, px = (i / C8) % 10, sr = i / (10 * C8);
       const int y = y0 - 1 + sr, x = x0 - 1 + px;
       const bool ok = live && i < NXT && y >= 0 && y < a.H && x >= 0 && x < a.Wd;
-      xr[q] = load16_or_zero(ok, a.X + ((img + (ok ? y : 0)) * a.Wd + (ok ? x : 0)) * (long long)C + 8 * cg);
+      xr[q] = load16_or_zero(ok, Xc + ((img + (ok ? y : 0)) * a.Wd + (ok ? x : 0)) * (long long)a.C + 8 * cg);
     }
 #pragma unroll
     for (int q = 0; q < NZL; ++q) {
       const int i = t + 256 * q;
       const int og = i % (BO / 8), pi = i / (BO / 8);
       const int y = y0 + (pi >> 3), x = x0 + (pi & 7);
-      const bool ok = live && i < NZT && y < a.H && 8 * og < a.Cout;
-      zr[q] = load16_or_zero(ok, a.dZ + (ok ? ((img + y) * a.Wd + x) * a.Cout + 8 * og : 0));
+      const bool ok = live && i < NZT && y < a.H && o0 + 8 * og < a.Cout;
+      zr[q] = load16_or_zero(ok, dZo + (ok ? ((img + y) * a.Wd + x) * a.Cout + 8 * og : 0));
     }
   };
   auto sstore = [&](int buf, const auto& xr, const auto& zr) {
@@ -842,14 +847,15 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
     }
   }
   float* out = a.out + blockIdx.z * a.split_stride;
-  const int cc = lane & 15, cr = (lane >> 4) * 4;
+  const int cc = lane & 15, cr = (lane >> 4) * 4, KF = 9 * a.C;
   for (int i = 0; i < KBW; ++i) {
     const int k = 16 * (w + 4 * i) + cc;
     if (k >= K) continue;
+    const int tap = k / C, kf = tap * a.C + c0 + (k - tap * C);
     for (int j = 0; j < NB; ++j)
       for (int q = 0; q < 4; ++q) {
-        const int o = 16 * j + cr + q;
-        if (o < a.Cout) out[(long long)o * K + k] = acc[i][j][q];
+        const int o = o0 + 16 * j + cr + q;
+        if (o < a.Cout) out[(long long)o * KF + kf] = acc[i][j][q];
       }
   }
 }
@@ -1063,17 +1069,29 @@ extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const 
   return BH_OK;
 }
 
-// the whole-tile weight gradient: 24 or 48 input and up to 64 output
-// channels, widths a multiple of 8 (8 input channels give it too few X
-// staging tasks per step: the k-tiled kernel is faster there)
+// the whole-tile weight gradient: input channels a multiple of 24, widths a
+// multiple of 8 (8 input channels give it too few X staging tasks per step:
+// the k-tiled kernel is faster there); slices of 48 input channels where
+// they divide C, else 24; output tiles of up to 64 channels (48 where 64 do
+// not divide Cout: 96 outputs as 2 x 48, not 64 + 32).  Slices of 24 channels
+// except into 32 outputs or fewer: the 48-channel kernel's registers (208 at
+// 48 outputs against 112) cost more occupancy than the second slice's dZ
+// re-read costs bandwidth, except in the narrowest output tiles.  Batch 16,
+// all slices of 48 where they divide C -> this rule: [128, 96, 48] 70 -> 67
+// us, [32, 192, 192] 64 -> 44, [256, 48, 24] 95 either way (102 with 24).
 static bool conv_dw_whole(int W, int C, int Cout) {
-  return (C == 24 || C == 48) && Cout <= 64 && W % 8 == 0;
+  (void)Cout;
+  return C % 24 == 0 && W % 8 == 0;
+}
+static int conv_dw_slice(int C, int Cout) { return C % 48 == 0 && Cout <= 32 ? 48 : 24; }
+static int conv_dw_otile(int Cout) {
+  return Cout <= 64 ? conv_tile(Cout) : Cout % 64 == 0 ? 4 : 3;
 }
 
 // pixel split of the weight gradient: about 1024 workgroups (the kernel is
 // bound by the latency of its staging loads, so it needs several resident
-// workgroups per CU; 512 of the whole-tile kernel, which does more per
-// step), chunks of at least 512 pixels (shorter chunks spend more on their
+// workgroups per CU; 512 or 1024 of the whole-tile kernel, which does more
+// per step), chunks of at least 512 pixels (shorter chunks spend more on their
 // partial tiles than they gain in parallelism), a multiple of 32 (of the
 // whole-tile kernel: whole 4x8 tiles, *ntiles of them in all).  Measured
 // on the U-Net's twelve layer shapes at batch 16 against 2048 workgroups
@@ -1082,7 +1100,14 @@ static void conv_dw_split(long long B, int H, int W, int C, int Cout, long long*
                           int* nsplit, long long* ntiles) {
   if (conv_dw_whole(W, C, Cout)) {
     const long long nt = B * ((H + 3) / 4) * (W / 8);
-    const long long ns = std::max(1LL, std::min(512LL, (nt + 15) / 16));
+    const int nb = conv_dw_otile(Cout);
+    const long long blocks = (long long)(C / conv_dw_slice(C, Cout)) * ((Cout + 16 * nb - 1) / (16 * nb));
+    // 1024 workgroups of a lone 24-channel tile (<= 112 registers: 4 resident
+    // per CU hide the staging loads better; [256, 24, 24] 57 -> 49 us), else
+    // 512 (more splits of the wider kernels add more partial-sum traffic
+    // than they hide: [128, 96, 48] 68 -> 76 us at 1024)
+    const long long target = blocks == 1 && conv_dw_slice(C, Cout) == 24 ? 1024 : 512;
+    const long long ns = std::max(1LL, std::min((target + blocks - 1) / blocks, (nt + 15) / 16));
     const long long ch = (nt + ns - 1) / ns;
     *chunk = ch;
     *nsplit = (int)((nt + ch - 1) / ch);
@@ -1144,15 +1169,16 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
   hipStream_t st = (hipStream_t)s;
   const int mo = conv_tile(Cout);
   if (conv_dw_whole(W, C, Cout)) {
-    const dim3 g1(1, 1, (unsigned)ns);
+    const int cs = conv_dw_slice(C, Cout), nb = conv_dw_otile(Cout);
+    const dim3 g1((unsigned)(C / cs), (unsigned)((Cout + 16 * nb - 1) / (16 * nb)), (unsigned)ns);
 #define BH_DWF(C8V)                                                                            \
   do {                                                                                        \
-    if (mo == 1) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 1>), g1, dim3(256), 0, st, a);         \
-    else if (mo == 2) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 2>), g1, dim3(256), 0, st, a);    \
-    else if (mo == 3) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 3>), g1, dim3(256), 0, st, a);    \
+    if (nb == 1) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 1>), g1, dim3(256), 0, st, a);         \
+    else if (nb == 2) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 2>), g1, dim3(256), 0, st, a);    \
+    else if (nb == 3) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 3>), g1, dim3(256), 0, st, a);    \
     else hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 4>), g1, dim3(256), 0, st, a);                 \
   } while (0)
-    if (C == 24) BH_DWF(3);
+    if (cs == 24) BH_DWF(3);
     else BH_DWF(6);
 #undef BH_DWF
   } else {
