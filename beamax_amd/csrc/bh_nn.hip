@@ -208,6 +208,92 @@ __global__ void k_relu_bwd(const bfbits* __restrict__ dY, const bfbits* __restri
   if (i < n) dZ[i] = bf2f(Yact[i]) > 0.f ? dY[i] : (bfbits)0;
 }
 
+// The same element-wise kernels on 16-byte chunks of 8 channels (C a
+// multiple of 8, 16-byte aligned operands; the kernels above otherwise):
+// one chunk per thread, the same FP32 operations per value in the same
+// order, so the results are bit-identical.  The 2x2 kernels index by the
+// small image's pixel p = (n h + y) w + x: its 2x2 block in the large image
+// is rows 2 (n h + y) + dy, columns 2 x + dx.
+__device__ inline float bf_lo(unsigned u) { return __uint_as_float(u << 16); }
+__device__ inline float bf_hi(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
+__device__ inline unsigned bf_pack(float lo, float hi) {
+  return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+}
+
+// large (2h, 2w) image from the small (h, w) one: nearest upsampling
+// (copy), or the adjoint of average pooling (0.25 x, rounded to bf16)
+__global__ void k_spread2_v8(const uint4* __restrict__ X, long long n8, int w, int C8, int quarter,
+                             uint4* __restrict__ Y) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const int cg = (int)(i % C8);
+  const long long p = i / C8;
+  const int x = (int)(p % w);
+  const long long q = p / w;
+  uint4 v = X[i];
+  if (quarter) {
+    unsigned* u = reinterpret_cast<unsigned*>(&v);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = bf_pack(0.25f * bf_lo(u[k]), 0.25f * bf_hi(u[k]));
+  }
+#pragma unroll
+  for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx)
+      Y[((2 * q + dy) * 2 * w + 2 * x + dx) * C8 + cg] = v;
+}
+
+// small (h, w) image from the large one: 2x2 sums in (dy, dx) order, times
+// 0.25 (average pooling) or 1 (adjoint of nearest upsampling)
+__global__ void k_gather2_v8(const uint4* __restrict__ X, long long n8, int w, int C8, float scale,
+                             uint4* __restrict__ Y) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const int cg = (int)(i % C8);
+  const long long p = i / C8;
+  const int x = (int)(p % w);
+  const long long q = p / w;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+      const uint4 v = X[((2 * q + dy) * 2 * w + 2 * x + dx) * C8 + cg];
+      const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[2 * k] += bf_lo(u[k]);
+        s[2 * k + 1] += bf_hi(u[k]);
+      }
+    }
+  Y[i] = uint4{bf_pack(scale * s[0], scale * s[1]), bf_pack(scale * s[2], scale * s[3]),
+               bf_pack(scale * s[4], scale * s[5]), bf_pack(scale * s[6], scale * s[7])};
+}
+
+__global__ void k_add_bf16_v8(const uint4* __restrict__ X, const uint4* __restrict__ Y,
+                              long long n8, uint4* __restrict__ Z) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const uint4 a = X[i], b = Y[i];
+  Z[i] = uint4{bf_pack(bf_lo(a.x) + bf_lo(b.x), bf_hi(a.x) + bf_hi(b.x)),
+               bf_pack(bf_lo(a.y) + bf_lo(b.y), bf_hi(a.y) + bf_hi(b.y)),
+               bf_pack(bf_lo(a.z) + bf_lo(b.z), bf_hi(a.z) + bf_hi(b.z)),
+               bf_pack(bf_lo(a.w) + bf_lo(b.w), bf_hi(a.w) + bf_hi(b.w))};
+}
+
+__device__ inline unsigned relu_mask2(unsigned d, unsigned y) {
+  return (bf_lo(y) > 0.f ? d & 0xffffu : 0u) | (bf_hi(y) > 0.f ? d & 0xffff0000u : 0u);
+}
+
+__global__ void k_relu_bwd_v8(const uint4* __restrict__ dY, const uint4* __restrict__ Yact,
+                              long long n8, uint4* __restrict__ dZ) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const uint4 d = dY[i], y = Yact[i];
+  dZ[i] = uint4{relu_mask2(d.x, y.x), relu_mask2(d.y, y.y), relu_mask2(d.z, y.z),
+                relu_mask2(d.w, y.w)};
+}
+
 // column sums of a [P][C] bf16 matrix in two fixed-order passes.  Pass 1:
 // block b sums rows [b * chunk, (b + 1) * chunk); its threads form G row
 // groups of C columns (coalesced along C), group sums added in group order.
@@ -234,6 +320,39 @@ __global__ __launch_bounds__(256) void k_colsum_part(const bfbits* __restrict__ 
       part[blockIdx.x * (long long)C + cb + t] = tot;
     }
     __syncthreads();
+  }
+}
+
+// pass 1 on 16-byte chunks (C a multiple of 8, at most 2048 columns, X
+// 16-byte aligned): G = 256 / (C / 8) row groups of C / 8 threads, each
+// summing 8 columns
+__global__ __launch_bounds__(256) void k_colsum_part_v8(const uint4* __restrict__ X, long long P,
+                                                       int C8, long long chunk,
+                                                       float* __restrict__ part) {
+  __shared__ float red[256][9];
+  const long long p0 = blockIdx.x * chunk;
+  const long long p1 = (p0 + chunk < P) ? p0 + chunk : P;
+  const int t = threadIdx.x, G = 256 / C8;
+  const int cg = t % C8, r = t / C8;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (r < G)
+    for (long long p = p0 + r; p < p1; p += G) {
+      const uint4 v = X[p * C8 + cg];
+      const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[2 * k] += bf_lo(u[k]);
+        s[2 * k + 1] += bf_hi(u[k]);
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) red[t][k] = s[k];
+  __syncthreads();
+  const int C = 8 * C8;
+  for (int c = t; c < C; c += 256) {
+    float tot = 0.f;
+    for (int g = 0; g < G; ++g) tot += red[g * C8 + c / 8][c % 8];
+    part[blockIdx.x * (long long)C + c] = tot;
   }
 }
 
@@ -1285,7 +1404,15 @@ extern "C" int bh_nn_pool2(const void* X, int64_t B, int H, int W, int C, void* 
   if (!X || !Y) BH_FAIL(BH_ERR_ARG, "bh_nn_pool2: null pointer");
   if (H % 2 || W % 2) BH_FAIL(BH_ERR_ARG, "bh_nn_pool2: H and W must be even");
   hipStream_t st = (hipStream_t)s;
-  if (!backward) {
+  if (C % 8 == 0 && aligned16(X) && aligned16(Y)) {
+    const long long n8 = B * (long long)(H / 2) * (W / 2) * (C / 8);
+    if (!backward)
+      hipLaunchKernelGGL(k_gather2_v8, dim3(bh::grid1d(n8, 256)), dim3(256), 0, st,
+                         (const uint4*)X, n8, W / 2, C / 8, 0.25f, (uint4*)Y);
+    else
+      hipLaunchKernelGGL(k_spread2_v8, dim3(bh::grid1d(n8, 256)), dim3(256), 0, st,
+                         (const uint4*)X, n8, W / 2, C / 8, 1, (uint4*)Y);
+  } else if (!backward) {
     const long long n = B * (long long)(H / 2) * (W / 2) * C;
     hipLaunchKernelGGL(k_pool2, dim3(bh::grid1d(n, 256)), dim3(256), 0, st, (const bfbits*)X,
                        (long long)B, H, W, C, (bfbits*)Y);
@@ -1305,7 +1432,15 @@ extern "C" int bh_nn_up2(const void* X, int64_t B, int H, int W, int C, void* Y,
   if (int rc = check_img("bh_nn_up2", B, H, W, C)) return rc;
   if (!X || !Y) BH_FAIL(BH_ERR_ARG, "bh_nn_up2: null pointer");
   hipStream_t st = (hipStream_t)s;
-  if (!backward) {
+  if (C % 8 == 0 && aligned16(X) && aligned16(Y)) {
+    const long long n8 = B * (long long)H * W * (C / 8);
+    if (!backward)
+      hipLaunchKernelGGL(k_spread2_v8, dim3(bh::grid1d(n8, 256)), dim3(256), 0, st,
+                         (const uint4*)X, n8, W, C / 8, 0, (uint4*)Y);
+    else
+      hipLaunchKernelGGL(k_gather2_v8, dim3(bh::grid1d(n8, 256)), dim3(256), 0, st,
+                         (const uint4*)X, n8, W, C / 8, 1.0f, (uint4*)Y);
+  } else if (!backward) {
     // X (B, H, W, C) -> Y (B, 2H, 2W, C)
     const long long n = B * 4LL * H * W * C;
     hipLaunchKernelGGL(k_up2, dim3(bh::grid1d(n, 256)), dim3(256), 0, st, (const bfbits*)X,
@@ -1324,8 +1459,12 @@ extern "C" int bh_nn_add(const void* X, const void* Y, int64_t n, void* Z, bh_st
   bh::clear_error();
   if (n <= 0) return BH_OK;
   if (!X || !Y || !Z) BH_FAIL(BH_ERR_ARG, "bh_nn_add: null pointer");
-  hipLaunchKernelGGL(k_add_bf16, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
-                     (const bfbits*)X, (const bfbits*)Y, (long long)n, (bfbits*)Z);
+  if (n % 8 == 0 && aligned16(X) && aligned16(Y) && aligned16(Z))
+    hipLaunchKernelGGL(k_add_bf16_v8, dim3(bh::grid1d(n / 8, 256)), dim3(256), 0, (hipStream_t)s,
+                       (const uint4*)X, (const uint4*)Y, (long long)n / 8, (uint4*)Z);
+  else
+    hipLaunchKernelGGL(k_add_bf16, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
+                       (const bfbits*)X, (const bfbits*)Y, (long long)n, (bfbits*)Z);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }
@@ -1334,8 +1473,12 @@ extern "C" int bh_nn_relu_bwd(const void* dY, const void* Yact, int64_t n, void*
   bh::clear_error();
   if (n <= 0) return BH_OK;
   if (!dY || !Yact || !dZ) BH_FAIL(BH_ERR_ARG, "bh_nn_relu_bwd: null pointer");
-  hipLaunchKernelGGL(k_relu_bwd, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
-                     (const bfbits*)dY, (const bfbits*)Yact, (long long)n, (bfbits*)dZ);
+  if (n % 8 == 0 && aligned16(dY) && aligned16(Yact) && aligned16(dZ))
+    hipLaunchKernelGGL(k_relu_bwd_v8, dim3(bh::grid1d(n / 8, 256)), dim3(256), 0, (hipStream_t)s,
+                       (const uint4*)dY, (const uint4*)Yact, (long long)n / 8, (uint4*)dZ);
+  else
+    hipLaunchKernelGGL(k_relu_bwd, dim3(bh::grid1d(n, 256)), dim3(256), 0, (hipStream_t)s,
+                       (const bfbits*)dY, (const bfbits*)Yact, (long long)n, (bfbits*)dZ);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }
@@ -1357,7 +1500,10 @@ extern "C" int bh_nn_colsum(const void* X, int64_t P, int C, float* out, void* w
   const size_t need = (size_t)nb * (size_t)C * sizeof(float);
   if (nb > 0 && (!ws || ws_bytes < need))
     BH_FAIL(BH_ERR_ARG, "bh_nn_colsum: workspace of %zu bytes needed", need);
-  if (nb > 0)
+  if (nb > 0 && C % 8 == 0 && C <= 2048 && aligned16(X))
+    hipLaunchKernelGGL(k_colsum_part_v8, dim3((unsigned)nb), dim3(256), 0, st, (const uint4*)X,
+                       (long long)P, C / 8, ch, (float*)ws);
+  else if (nb > 0)
     hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)nb), dim3(256), 0, st, (const bfbits*)X,
                        (long long)P, C, ch, (float*)ws);
   hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)C), dim3(256), 0, st, (const float*)ws, nb, C,

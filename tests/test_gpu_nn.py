@@ -166,6 +166,63 @@ def test_implicit_conv_kernels_match_torch(gpu, cin, cout, shape):
     assert rel(dW.cpu().numpy(), dW_ref.numpy()) < 1e-5
 
 
+@pytest.mark.parametrize("C,shape", [(8, (2, 6, 10)), (24, (3, 4, 8)), (5, (2, 6, 4))])
+def test_elementwise_kernels_bit_exact(gpu, C, shape):
+    """Pooling, upsampling, their adjoints, the skip add and the ReLU
+    backward, bit for bit against the same FP32 operations on the CPU (2x2
+    sums in (dy, dx) order, round to nearest even), both on 16-byte aligned
+    operands with C a multiple of 8 (the 8-channel chunk kernels) and on
+    operands one element off alignment (the per-element kernels); the column
+    sums of the bias gradient on both paths against an FP64 sum."""
+    import torch
+    from beamax_amd import _lib
+    B, H, W = shape
+    g = torch.Generator().manual_seed(C * 7 + H)
+    big = torch.randn(B, H, W, C, generator=g).to(torch.bfloat16)
+    small = torch.randn(B, H // 2, W // 2, C, generator=g).to(torch.bfloat16)
+    other = torch.randn(B, H, W, C, generator=g).to(torch.bfloat16)
+    f = big.float()
+    quad = ((f[:, 0::2, 0::2] + f[:, 0::2, 1::2]) + f[:, 1::2, 0::2]) + f[:, 1::2, 1::2]
+    rep = lambda t: t.repeat_interleave(2, 1).repeat_interleave(2, 2)
+    want = {
+        ("pool2", 0): (big, (0.25 * quad).to(torch.bfloat16)),
+        ("pool2", 1): (small, rep((0.25 * small.float()).to(torch.bfloat16))),
+        ("up2", 0): (small, rep(small)),
+        ("up2", 1): (big, quad.to(torch.bfloat16)),
+    }
+    st = _lib.stream_handle()
+
+    def dev(t, off):
+        buf = torch.zeros(t.numel() + 8, dtype=torch.bfloat16, device=gpu)
+        v = buf[off:off + t.numel()]
+        v.copy_(t.reshape(-1).to(gpu))
+        return v
+
+    bits = lambda t: t.reshape(-1).view(torch.int16).cpu()
+    for off in (0, 1):
+        for (name, bwd), (x, ref) in want.items():
+            xd, yd = dev(x, off), dev(torch.zeros_like(ref), off)
+            b, h, w = (B, H, W) if name == "pool2" else (B, H // 2, W // 2)
+            _lib.call(f"bh_nn_{name}", _lib.ptr(xd), b, h, w, C, _lib.ptr(yd), bwd, st)
+            assert torch.equal(bits(yd), bits(ref)), (name, bwd, off)
+        n = big.numel()
+        xd, od, zd = dev(big, off), dev(other, off), dev(torch.zeros_like(big), off)
+        _lib.call("bh_nn_add", _lib.ptr(xd), _lib.ptr(od), n, _lib.ptr(zd), st)
+        assert torch.equal(bits(zd), bits((f + other.float()).to(torch.bfloat16))), ("add", off)
+        _lib.call("bh_nn_relu_bwd", _lib.ptr(od), _lib.ptr(xd), n, _lib.ptr(zd), st)
+        relu_ref = torch.where(f > 0, other, torch.zeros_like(other))
+        assert torch.equal(bits(zd), bits(relu_ref)), ("relu_bwd", off)
+        # column sums (bias gradients) of big as [P][C]: FP32 partials
+        P = n // C
+        lib = _lib.load()
+        ws = torch.empty(lib.bh_nn_colsum_workspace_size(P, C) // 4 + 1, dtype=torch.float32,
+                         device=gpu)
+        cs = torch.empty(C, dtype=torch.float32, device=gpu)
+        _lib.call("bh_nn_colsum", _lib.ptr(xd), P, C, _lib.ptr(cs), _lib.ptr(ws),
+                  ws.numel() * 4, st)
+        assert rel(cs.cpu().numpy(), f.reshape(P, C).double().sum(0).numpy()) < 1e-6, off
+
+
 class _Twin:
     """FP32 CPU twin of ``UNet`` (same weights, bf16-rounded as the device
     uses them; activations rounded to bf16 where the device stores them)."""
