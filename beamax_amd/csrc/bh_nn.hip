@@ -1224,7 +1224,8 @@ static void conv_dw_split(long long B, int H, int W, int C, int Cout, long long*
     // 1024 workgroups of a lone 24-channel tile (<= 112 registers: 4 resident
     // per CU hide the staging loads better; [256, 24, 24] 57 -> 49 us), else
     // 512 (more splits of the wider kernels add more partial-sum traffic
-    // than they hide: [128, 96, 48] 68 -> 76 us at 1024)
+    // than they hide: [128, 96, 48] 68 -> 76 us at 1024; fewer hide less
+    // latency: 80 us at 384, 92 at 256)
     const long long target = blocks == 1 && conv_dw_slice(C, Cout) == 24 ? 1024 : 512;
     const long long ns = std::max(1LL, std::min((target + blocks - 1) / blocks, (nt + 15) / 16));
     const long long ch = (nt + ns - 1) / ns;
