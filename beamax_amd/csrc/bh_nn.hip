@@ -856,12 +856,12 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
   const bfbits* Xc = a.X + c0;
   const bfbits* dZo = a.dZ + o0;
   const unsigned WB = (unsigned)a.Wd >> 3, HT = ((unsigned)a.H + 3) >> 2;
-  // 24 input channels: two register sets, the loads of step st + 2 issued
+  // 8 or 24 input channels: two register sets, the loads of step st + 2 issued
   // while step st multiplies (every load is unconditional -- past the chunk
   // it reads the zero block -- so the waits before the LDS stores count
   // older loads only; [256, 24, 24] at batch 16: 83 -> 68 us); 48 channels:
   // one step ahead (the second set's registers cost more than it hides)
-  constexpr bool AHEAD2 = C8 == 3;
+  constexpr bool AHEAD2 = C8 <= 3;
   uint4 xr0[NXL], zr0[NZL], xr1[AHEAD2 ? NXL : 1], zr1[AHEAD2 ? NZL : 1];
   auto gload = [&](long long tile, auto& xr, auto& zr) {
     const bool live = tile < tend;
@@ -979,7 +979,9 @@ __global__ __launch_bounds__(256) void k_conv3_dw_tr(ConvDwArgs a) {
   }
 }
 
-// The k-tiled weight gradient (tile: 16 MO output channels x 128 k columns,
+// The k-tiled weight gradient, for the shapes the whole-tile kernel does not
+// take (widths not a multiple of 8, other channel counts; tile: 16 MO
+// output channels x 128 k columns,
 // four waves of 16 MO x 32) with the same transposing LDS reads: per 32-pixel
 // step the tile's 16 column octets (tap, 8 channels) are staged pixel-major
 // -- octet o holds the 32 pixels' 8 channels at its tap, one 16-byte copy
@@ -1188,10 +1190,10 @@ extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const 
   return BH_OK;
 }
 
-// the whole-tile weight gradient: input channels a multiple of 24, widths a
-// multiple of 8 (8 input channels give it too few X staging tasks per step:
-// the k-tiled kernel is faster there); slices of 48 input channels where
-// they divide C, else 24; output tiles of up to 64 channels (48 where 64 do
+// the whole-tile weight gradient: 8 input channels (one slice: [256, 8, 24]
+// at batch 16 55 -> 40 us against the k-tiled kernel) or a multiple of 24,
+// widths a multiple of 8; slices of 48 input channels where they divide C,
+// else 24; output tiles of up to 64 channels (48 where 64 do
 // not divide Cout: 96 outputs as 2 x 48, not 64 + 32).  Slices of 24 channels
 // except into 32 outputs or fewer: the 48-channel kernel's registers (208 at
 // 48 outputs against 112) cost more occupancy than the second slice's dZ
@@ -1200,9 +1202,11 @@ extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const 
 // us, [32, 192, 192] 64 -> 44, [256, 48, 24] 95 either way (102 with 24).
 static bool conv_dw_whole(int W, int C, int Cout) {
   (void)Cout;
-  return C % 24 == 0 && W % 8 == 0;
+  return (C == 8 || C % 24 == 0) && W % 8 == 0;
 }
-static int conv_dw_slice(int C, int Cout) { return C % 48 == 0 && Cout <= 32 ? 48 : 24; }
+static int conv_dw_slice(int C, int Cout) {
+  return C == 8 ? 8 : C % 48 == 0 && Cout <= 32 ? 48 : 24;
+}
 static int conv_dw_otile(int Cout) {
   return Cout <= 64 ? conv_tile(Cout) : Cout % 64 == 0 ? 4 : 3;
 }
@@ -1221,12 +1225,13 @@ static void conv_dw_split(long long B, int H, int W, int C, int Cout, long long*
     const long long nt = B * ((H + 3) / 4) * (W / 8);
     const int nb = conv_dw_otile(Cout);
     const long long blocks = (long long)(C / conv_dw_slice(C, Cout)) * ((Cout + 16 * nb - 1) / (16 * nb));
-    // 1024 workgroups of a lone 24-channel tile (<= 112 registers: 4 resident
-    // per CU hide the staging loads better; [256, 24, 24] 57 -> 49 us), else
+    // 1024 workgroups of a lone 8- or 24-channel tile (<= 112 registers: 4
+    // resident per CU hide the staging loads better; [256, 24, 24] 57 -> 49
+    // us, [256, 8, 24] 48 -> 40), else
     // 512 (more splits of the wider kernels add more partial-sum traffic
     // than they hide: [128, 96, 48] 68 -> 76 us at 1024; fewer hide less
     // latency: 80 us at 384, 92 at 256)
-    const long long target = blocks == 1 && conv_dw_slice(C, Cout) == 24 ? 1024 : 512;
+    const long long target = blocks == 1 && conv_dw_slice(C, Cout) <= 24 ? 1024 : 512;
     const long long ns = std::max(1LL, std::min((target + blocks - 1) / blocks, (nt + 15) / 16));
     const long long ch = (nt + ns - 1) / ns;
     *chunk = ch;
@@ -1298,7 +1303,8 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
     else if (nb == 3) hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 3>), g1, dim3(256), 0, st, a);    \
     else hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 4>), g1, dim3(256), 0, st, a);                 \
   } while (0)
-    if (cs == 24) BH_DWF(3);
+    if (cs == 8) BH_DWF(1);
+    else if (cs == 24) BH_DWF(3);
     else BH_DWF(6);
 #undef BH_DWF
   } else {

@@ -108,16 +108,16 @@ def test_conv_forward_and_gradients_match_torch(gpu):
     (192, 96, (1, 5, 7)), (16, 8, (2, 64, 40)), (64, 64, (1, 3, 3)), (24, 24, (4, 48, 64)),
     (8, 24, (2, 16, 24)), (48, 64, (2, 24, 32)), (24, 8, (3, 8, 16)), (48, 48, (1, 40, 8)),
     (48, 24, (2, 32, 32)), (24, 16, (3, 7, 24)), (48, 32, (2, 1, 16)),
-    (96, 80, (2, 9, 16)), (72, 192, (1, 6, 8)), (192, 96, (2, 12, 24))])
+    (96, 80, (2, 9, 16)), (72, 192, (1, 6, 8)), (192, 96, (2, 12, 24)), (8, 16, (3, 5, 16))])
 def test_implicit_conv_kernels_match_torch(gpu, cin, cout, shape):
     """The three convolution entry points on their own, over every output
     tile width (8 to 192 channels, several k and channel tiles), images
     smaller than a tile, widths that are not multiples of 8, heights that are
     not multiples of the weight gradient's 4-row tiles, and pixel counts
     that split the weight gradient into many chunks (both weight-gradient
-    kernels: the whole-tile one for input channels a multiple of 24 -- slices
-    of 24 or 48, output tiles of 48 or 64 -- at widths that are multiples of
-    8, the k-tiled one otherwise; both
+    kernels: the whole-tile one for 8 input channels or a multiple of 24 --
+    slices of 8, 24 or 48, output tiles of 48 or 64 -- at widths that are
+    multiples of 8, the k-tiled one otherwise; both
     forward kernels: halo tiles and gathered operands): forward (bias, ReLU,
     alpha, bf16 and FP32 output), input gradient through the flipped weights,
     weight gradient, all against FP32 torch on the CPU."""
