@@ -804,8 +804,9 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ pa
   }
 }
 
-// Weight gradient of a slice of 24 or 48 input channels (grid x) into a tile
-// of up to 64 output channels (grid y), image width a multiple of 8, with
+// Weight gradient of a slice of 8, 24 or 48 input channels (grid x; the
+// launch uses 8 and 24) into a tile of up to 64 output channels (grid y),
+// image width a multiple of 8, with
 // gfx950's transposing LDS reads.  A step
 // is a tile of 4 image rows x 8 pixels; the operands are staged as they lie
 // in memory -- the tile's 6 x 10 input pixels (rows y - 1 .. y + 4, pixels
@@ -1191,22 +1192,19 @@ extern "C" int bh_nn_conv3(const void* X, int64_t B, int H, int W, int C, const 
 }
 
 // the whole-tile weight gradient: 8 input channels (one slice: [256, 8, 24]
-// at batch 16 55 -> 40 us against the k-tiled kernel) or a multiple of 24,
-// widths a multiple of 8; slices of 48 input channels where they divide C,
-// else 24; output tiles of up to 64 channels (48 where 64 do
-// not divide Cout: 96 outputs as 2 x 48, not 64 + 32).  Slices of 24 channels
-// except into 32 outputs or fewer: the 48-channel kernel's registers (208 at
-// 48 outputs against 112) cost more occupancy than the second slice's dZ
-// re-read costs bandwidth, except in the narrowest output tiles.  Batch 16,
-// all slices of 48 where they divide C -> this rule: [128, 96, 48] 70 -> 67
-// us, [32, 192, 192] 64 -> 44, [256, 48, 24] 95 either way (102 with 24).
+// at batch 16 55 -> 40 us against the k-tiled kernel) or a multiple of 24
+// (slices of 24), widths a multiple of 8; output tiles of up to 64 channels
+// (48 where 64 do not divide Cout: 96 outputs as 2 x 48, not 64 + 32).
+// Slices of 24 also where 48 divide C: the 48-channel kernel's registers
+// (160-208 against 92-112) cost more occupancy than the second slice's dZ
+// re-read costs bandwidth.  Batch 16, slices of 48 -> 24: [128, 96, 48] 70
+// -> 67 us, [32, 192, 192] 64 -> 44, [256, 48, 24] 95 -> 81 (1024
+// workgroups, below).
 static bool conv_dw_whole(int W, int C, int Cout) {
   (void)Cout;
   return (C == 8 || C % 24 == 0) && W % 8 == 0;
 }
-static int conv_dw_slice(int C, int Cout) {
-  return C == 8 ? 8 : C % 48 == 0 && Cout <= 32 ? 48 : 24;
-}
+static int conv_dw_slice(int C) { return C == 8 ? 8 : 24; }
 static int conv_dw_otile(int Cout) {
   return Cout <= 64 ? conv_tile(Cout) : Cout % 64 == 0 ? 4 : 3;
 }
@@ -1224,14 +1222,14 @@ static void conv_dw_split(long long B, int H, int W, int C, int Cout, long long*
   if (conv_dw_whole(W, C, Cout)) {
     const long long nt = B * ((H + 3) / 4) * (W / 8);
     const int nb = conv_dw_otile(Cout);
-    const long long blocks = (long long)(C / conv_dw_slice(C, Cout)) * ((Cout + 16 * nb - 1) / (16 * nb));
-    // 1024 workgroups of a lone 8- or 24-channel tile (<= 112 registers: 4
+    const long long blocks = (long long)(C / conv_dw_slice(C)) * ((Cout + 16 * nb - 1) / (16 * nb));
+    // 1024 workgroups for one or two dW tiles per split (<= 112 registers: 4
     // resident per CU hide the staging loads better; [256, 24, 24] 57 -> 49
     // us, [256, 8, 24] 48 -> 40), else
     // 512 (more splits of the wider kernels add more partial-sum traffic
     // than they hide: [128, 96, 48] 68 -> 76 us at 1024; fewer hide less
     // latency: 80 us at 384, 92 at 256)
-    const long long target = blocks == 1 && conv_dw_slice(C, Cout) <= 24 ? 1024 : 512;
+    const long long target = blocks <= 2 ? 1024 : 512;
     const long long ns = std::max(1LL, std::min((target + blocks - 1) / blocks, (nt + 15) / 16));
     const long long ch = (nt + ns - 1) / ns;
     *chunk = ch;
@@ -1294,7 +1292,7 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
   hipStream_t st = (hipStream_t)s;
   const int mo = conv_tile(Cout);
   if (conv_dw_whole(W, C, Cout)) {
-    const int cs = conv_dw_slice(C, Cout), nb = conv_dw_otile(Cout);
+    const int cs = conv_dw_slice(C), nb = conv_dw_otile(Cout);
     const dim3 g1((unsigned)(C / cs), (unsigned)((Cout + 16 * nb - 1) / (16 * nb)), (unsigned)ns);
 #define BH_DWF(C8V)                                                                            \
   do {                                                                                        \
@@ -1304,8 +1302,7 @@ extern "C" int bh_nn_conv3_dw(const void* X, const void* dZ, int64_t B, int H, i
     else hipLaunchKernelGGL((k_conv3_dw_tr<C8V, 4>), g1, dim3(256), 0, st, a);                 \
   } while (0)
     if (cs == 8) BH_DWF(1);
-    else if (cs == 24) BH_DWF(3);
-    else BH_DWF(6);
+    else BH_DWF(3);
 #undef BH_DWF
   } else {
   const dim3 grid((unsigned)((9 * C + DWK - 1) / DWK), (unsigned)((Cout + 16 * mo - 1) / (16 * mo)),
